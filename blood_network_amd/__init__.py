@@ -1,0 +1,12 @@
+"""MI355X-native AMG-preconditioned PCG for vessel-network pressure systems.
+
+The numerics live in ``libbnamg.so`` (host C set-up + gfx950 HIP kernels,
+C ABI in ``include/amg.h`` and ``include/bn_amg_ext.h``).  Importing the
+package loads that library and fails if it is missing.
+"""
+from ._lib import device_available, last_error, lib
+from .amg import AMGSolver, SolveReport
+from .netgen import WORKLOADS, VesselNetwork, make_network
+
+__all__ = ["AMGSolver", "SolveReport", "VesselNetwork", "make_network", "WORKLOADS",
+           "device_available", "last_error", "lib"]

@@ -1,0 +1,156 @@
+"""ctypes binding of libbnamg.so (the MI355X AMG-PCG library).
+
+The library is built in-tree (``make -C blood_network_amd/csrc``, or
+``__graft_entry__.build()``).  Importing this module loads it and raises
+if it is missing: there is no Python or CPU fallback for the solve.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbnamg.so")
+
+IP = C.POINTER(C.c_int)
+DP = C.POINTER(C.c_double)
+VP = C.c_void_p
+
+
+class CSR(C.Structure):
+    """dcsr_mat (C_amg1/include/agg_lib.h:37-44)."""
+    _fields_ = [("row", C.c_int), ("col", C.c_int), ("nnz", C.c_int),
+                ("ia", IP), ("ja", IP), ("val", DP)]
+
+
+class Options(C.Structure):
+    """bn_options (include/bn_amg_ext.h)."""
+    _fields_ = [("coarsening", C.c_int), ("threshold", C.c_double),
+                ("min_size", C.c_int), ("max_levels", C.c_int), ("nu", C.c_int),
+                ("smoother", C.c_int), ("omega", C.c_double),
+                ("precision", C.c_int), ("coarse_solver", C.c_int),
+                ("fine_ic", C.c_int), ("ic_sweeps", C.c_int),
+                ("max_colours", C.c_int), ("flexible", C.c_int),
+                ("stop", C.c_int), ("symmetrize", C.c_int), ("device", C.c_int),
+                ("verbose", C.c_int)]
+
+
+class SolveInfo(C.Structure):
+    """bn_solve_info (include/bn_amg_ext.h)."""
+    _fields_ = [("iterations", C.c_int), ("rel_bnorm", C.c_double),
+                ("relres", C.c_double), ("solve_ms", C.c_double),
+                ("wall_ms", C.c_double), ("setup_ms", C.c_double),
+                ("upload_ms", C.c_double), ("num_levels", C.c_int)]
+
+
+class NetParams(C.Structure):
+    """bn_netgen_params (include/bn_netgen.h)."""
+    _fields_ = [("L", C.c_int), ("depth", C.c_int), ("n_trees", C.c_int),
+                ("seed", C.c_uint64), ("p_art", C.c_double), ("p_ven", C.c_double),
+                ("mu", C.c_double), ("spacing", C.c_double), ("cap_r", C.c_double),
+                ("cap_sigma", C.c_double), ("root_r", C.c_double)]
+
+
+class Network(C.Structure):
+    """bn_network (include/bn_netgen.h)."""
+    _fields_ = [("n_nodes", C.c_int), ("n", C.c_int), ("n_edges", C.c_int),
+                ("nnz", C.c_int), ("n_dirichlet", C.c_int),
+                ("ia", IP), ("ja", IP), ("val", DP), ("b", DP),
+                ("eu", IP), ("ev", IP), ("radius", DP), ("length", DP), ("g", DP),
+                ("pos_uv", IP), ("pos_vu", IP), ("diag_pos", IP),
+                ("p_dirichlet", DP), ("dirichlet_is_art", IP), ("node_unknown", IP)]
+
+
+BN_ERR_NO_DEVICE = -1000001
+BN_ERR_HIP = -1000002
+BN_ERR_ARG = -1000003
+
+SMOOTHER_L1, SMOOTHER_GS, SMOOTHER_CF_GS, SMOOTHER_JACOBI = 0, 1, 2, 3
+COARSEN_RS, COARSEN_PMIS, COARSEN_HMIS = 0, 1, 2
+PREC_FP64, PREC_FP32 = 0, 1
+COARSE_AUTO, COARSE_DENSE, COARSE_ICPCG = 0, 1, 2
+IC_EXACT, IC_JACOBI = 0, 1
+STOP_BNORM, STOP_RELRES = 0, 1
+
+
+def _load():
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is missing: build it with `make -C blood_network_amd/csrc` "
+            "(the solve has no CPU fallback)")
+    lib = C.CDLL(LIB_PATH)
+    sig = {
+        "amg_solver_create": (VP, [C.c_int, C.c_int, IP, IP, DP, C.c_double, C.c_int, C.c_int, C.c_int]),
+        "amg_solver_update": (C.c_int, [VP, C.c_int, C.c_int, IP, IP, DP]),
+        "amg_solver_set_matrix": (C.c_int, [VP, C.c_int, C.c_int, IP, IP, DP]),
+        "amg_solver_solve": (C.c_int, [VP, DP, DP, C.c_double, C.c_int, C.c_int]),
+        "amg_solver_set_smoother": (None, [VP, C.c_int, C.c_double]),
+        "amg_solver_size": (C.c_int, [VP]),
+        "amg_solver_num_levels": (C.c_int, [VP]),
+        "amg_solver_free": (None, [VP]),
+        "bn_options_default": (None, [C.POINTER(Options)]),
+        "bn_solver_create": (VP, [C.c_int, C.c_int, IP, IP, DP, C.POINTER(Options)]),
+        "bn_solver_solve_device": (C.c_int, [VP, VP, VP, C.c_double, C.c_int, C.c_int, C.POINTER(SolveInfo)]),
+        "bn_solver_solve_info": (C.c_int, [VP, DP, DP, C.c_double, C.c_int, C.c_int, C.POINTER(SolveInfo)]),
+        "bn_solver_precond": (C.c_int, [VP, DP, DP, C.c_int]),
+        "bn_solver_last_info": (C.c_int, [VP, C.POINTER(SolveInfo)]),
+        "bn_solver_set_mode": (C.c_int, [VP, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int]),
+        "bn_solver_level_dims": (C.c_int, [VP, C.c_int, IP]),
+        "bn_solver_level_csr": (C.c_int, [VP, C.c_int, C.c_int, IP, IP, DP]),
+        "bn_solver_level_split": (C.c_int, [VP, C.c_int, IP, IP, DP]),
+        "bn_solver_ichol": (C.c_int, [VP, C.c_int, IP, IP, DP]),
+        "bn_solver_schedule": (C.c_int, [VP, C.c_int, IP, IP, IP]),
+        "bn_solver_ic_levels": (C.c_int, [VP, IP, IP]),
+        "bn_device_available": (C.c_int, []),
+        "bn_last_error": (C.c_char_p, []),
+        "bn_solver_adjoint_grad": (C.c_int, [VP, DP, DP, C.c_int, IP, IP, DP, DP, C.c_double, C.c_int, DP, DP, DP]),
+        "bn_solver_solve_batch": (C.c_int, [VP, C.c_int, DP, DP, C.c_double, C.c_int, IP, C.POINTER(SolveInfo)]),
+        "bn_netgen_default_params": (None, [C.POINTER(NetParams)]),
+        "bn_netgen_build": (C.c_int, [C.POINTER(NetParams), C.POINTER(Network)]),
+        "bn_netgen_assemble": (None, [C.POINTER(Network), DP, DP, DP, DP]),
+        "bn_netgen_conductance": (None, [C.POINTER(Network), DP, C.c_double, DP]),
+        "bn_netgen_free": (None, [C.POINTER(Network)]),
+        "amg_build_hierarchy": (None, [VP, C.POINTER(CSR)]),
+        "amg_free": (None, [VP]),
+        "ichol_compute": (None, [C.POINTER(CSR), VP]),
+        "ichol_free": (None, [VP]),
+        "ichol_solve": (None, [VP, DP, DP]),
+        "amg_backslash": (None, [VP, C.c_int, DP, DP, C.c_int]),
+        "amg_fwdslash": (None, [VP, C.c_int, DP, DP, C.c_int]),
+        "amg_vcycle_precond": (None, [VP, DP, DP]),
+        "amg_ichol_precond": (None, [VP, C.POINTER(CSR), VP, DP, DP]),
+        # kernel launchers (device pointers, stream)
+        "bn_csr_apply": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
+        "bn_trsv_level": (None, [C.c_char, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
+        "bn_gs_color": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
+        "bn_jacobi_rows": (None, [C.c_char, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP]),
+        "bn_dense_gemv": (None, [C.c_char, C.c_int, VP, VP, VP, VP]),
+        "bn_dot_partial": (None, [C.c_int, VP, VP, VP, VP, VP, VP, VP]),
+        "bn_scalar_finish": (None, [C.c_int, VP, VP, VP, VP, C.c_int, VP]),
+        "bn_edge_grad": (None, [C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+def ip(a):
+    return a.ctypes.data_as(IP)
+
+
+def dp(a):
+    return a.ctypes.data_as(DP)
+
+
+def last_error():
+    e = lib.bn_last_error()
+    return e.decode() if e else ""
+
+
+def device_available():
+    return bool(lib.bn_device_available())

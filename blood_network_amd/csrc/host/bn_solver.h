@@ -1,0 +1,113 @@
+/*
+ * bn_solver.h - internal solver state: host hierarchy plus its device
+ * mirror (relabelled per level in smoother-colour order).
+ */
+#ifndef BN_SOLVER_H
+#define BN_SOLVER_H
+
+#include <hip/hip_runtime_api.h>
+
+#include "amg.h"
+#include "bn_amg_ext.h"
+#include "bn_kernels.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* a CSR operator on the device, FP64 and/or FP32 values */
+typedef struct {
+  int rows, cols, nnz, lanes;
+  int *ia, *ja;
+  double *vd;
+  float *vs;
+} bn_dcsr;
+
+/* level-scheduled triangular factor L (forward) and L^T (backward) */
+typedef struct {
+  int n, nlev_f, nlev_b;
+  bn_dcsr lo, up;  /* L rows (cols <= i) and L^T rows (cols >= i) */
+  double *dinv_d;
+  float *dinv_s;
+  int *rows_f, *rows_b;   /* device: rows grouped by level */
+  int *ptr_f, *ptr_b;     /* host: level boundaries */
+} bn_dtri;
+
+typedef struct {
+  int n, nc;
+  bn_dcsr A;            /* level operator (relabelled) */
+  bn_dcsr P;            /* n x nc prolongation (relabelled both sides) */
+  bn_dcsr R;            /* nc x n restriction = P^T */
+  double *dinv_d;       /* 1/a_ii (1.0 when the diagonal is absent) */
+  float *dinv_s;
+  double *l1inv_d;      /* 1/sum_j |a_ij| */
+  float *l1inv_s;
+  int ncol;             /* colours of the multicolour sweep */
+  int *col_ptr;         /* host: colour boundaries (relabelled rows) */
+  int use_gs;           /* colour count allows the multicolour sweep */
+  int *perm_h;          /* host perm[new] = old */
+  int *perm_d;
+  void *b, *x, *r;      /* level work vectors in V-cycle precision */
+} bn_dlevel;
+
+struct bn_solver {
+  /* host side */
+  dcsr_mat A_own;       /* symmetrised solve matrix (handle API)     */
+  amg_data amg_own;     /* hierarchy (levels[0].A is the AMG matrix) */
+  ichol_factor L_own;   /* IC(0) of the solve matrix                 */
+  const dcsr_mat *A;    /* -> A_own, or a caller's matrix (cache)    */
+  amg_data *amg;        /* -> amg_own, or a caller's hierarchy       */
+  const ichol_factor *L;/* -> L_own, caller's factor, or NULL        */
+  int borrowed;         /* host structures belong to the caller      */
+  bn_options opt;
+  int device;           /* HIP device ordinal */
+  int dev_ready;        /* device mirror uploaded */
+  /* device side */
+  hipStream_t stream;
+  char vt;              /* V-cycle precision: 'd' or 's' */
+  int nlev;
+  bn_dlevel lev[AMG_MAX_LEVELS];
+  bn_dcsr Asolve;       /* FP64 solve matrix in level-0 numbering */
+  int asolve_shared;    /* Asolve aliases lev[0].A's FP64 arrays */
+  bn_dtri fine_ic;      /* IC(0) of the solve matrix, level-0 numbering */
+  int coarse_dense;     /* coarsest solve: 1 dense inverse, 0 IC-PCG */
+  double *cinv_d;       /* dense inverse of the coarsest operator */
+  float *cinv_s;
+  bn_dtri coarse_ic;    /* IC(0) of the coarsest operator (IC-PCG mode) */
+  double *cvec[6];      /* FP64 coarsest work vectors (IC-PCG, FP32 mode) */
+  /* PCG state (FP64, level-0 numbering) */
+  double *x, *r, *z, *p, *q, *bb, *rprev;
+  /* preconditioner work (V-cycle precision) */
+  void *g_in, *u1, *r1, *icy, *icx, *r2, *corr, *zout;
+  double *scal;         /* BN_S_COUNT scalars */
+  double *part1, *part2, *part3;
+  double *cpart1, *cpart2;  /* partials of the coarsest IC-PCG */
+  double *host_scal;    /* pinned mirror of scal */
+  double *hbuf;         /* host staging (n doubles, pinned) */
+  hipEvent_t ev0, ev1;
+  bn_solve_info last;
+  /* device allocations owned by this solver */
+  void **dptr;
+  int ndptr, capdptr;
+};
+
+/* device.c */
+int bn_device_upload(struct bn_solver *s);
+void bn_device_release(struct bn_solver *s);
+int bn_device_ok(void);
+
+/* vcycle.c */
+void bn_vc_backslash(struct bn_solver *s, int lev, const void *b, void *x, int x_zero);
+void bn_vc_fwdslash(struct bn_solver *s, int lev, const void *b, void *x, int x_zero);
+void bn_vc_ichol_precond(struct bn_solver *s, const double *g, double *z);
+void bn_vc_vcycle_precond(struct bn_solver *s, const double *g, double *z);
+void bn_vc_fine_ic(struct bn_solver *s, const void *b, void *x);
+
+/* pcg.c */
+int bn_pcg_run(struct bn_solver *s, const double *d_b, double *d_x, double tol, int maxiter,
+               int print_level, bn_solve_info *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

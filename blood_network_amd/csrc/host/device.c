@@ -1,0 +1,331 @@
+/*
+ * device.c - upload of the host hierarchy to the GPU.
+ *
+ * Each level is relabelled once in its smoother-colour order (C colours,
+ * then F colours), so every colour of the multicolour sweep is a
+ * contiguous row range of the level operator and all kernels of the level
+ * stream contiguous CSR segments.  P and R = P^T are relabelled on both
+ * sides, the solve matrix and its IC(0) factor in level-0 numbering.
+ * Operators are stored in FP64 and FP32 so the V-cycle precision can be
+ * switched without re-uploading.
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
+#include "bn_solver.h"
+#include "csr.h"
+#include "schedule.h"
+
+int bn_dense_inverse(const dcsr_mat *A, double *inv);
+void bn_set_error(const char *fmt, ...);
+void bn_track_device(struct bn_solver *s, void *p);
+
+static int g_device_state = -1; /* -1 unknown, 0 none, 1 usable */
+
+int bn_device_ok(void) {
+  if (g_device_state < 0) {
+    int n = 0;
+    hipError_t e = hipGetDeviceCount(&n);
+    g_device_state = (e == hipSuccess && n > 0) ? 1 : 0;
+  }
+  return g_device_state;
+}
+
+#define CK(call)                                                                   \
+  do {                                                                             \
+    hipError_t e_ = (call);                                                        \
+    if (e_ != hipSuccess) {                                                        \
+      bn_set_error("HIP error %d (%s) at %s:%d", (int)e_, hipGetErrorString(e_),   \
+                   __FILE__, __LINE__);                                            \
+      return BN_ERR_HIP;                                                           \
+    }                                                                              \
+  } while (0)
+
+static void *dmalloc(struct bn_solver *s, size_t bytes) {
+  void *p = NULL;
+  if (hipMalloc(&p, bytes ? bytes : 8) != hipSuccess) return NULL;
+  bn_track_device(s, p);
+  return p;
+}
+
+static int lanes_for(long long nnz, long long rows) {
+  double avg = rows > 0 ? (double)nnz / (double)rows : 1.0;
+  int l = 1;
+  while (l * 2 <= avg && l < 64) l *= 2;
+  return l;
+}
+
+static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
+  d->rows = m->row;
+  d->cols = m->col;
+  d->nnz = m->nnz;
+  d->lanes = lanes_for(m->nnz, m->row);
+  d->ia = (int *)dmalloc(s, ((size_t)m->row + 1) * sizeof(int));
+  d->ja = (int *)dmalloc(s, (size_t)m->nnz * sizeof(int));
+  d->vd = (double *)dmalloc(s, (size_t)m->nnz * sizeof(double));
+  d->vs = (float *)dmalloc(s, (size_t)m->nnz * sizeof(float));
+  if (!d->ia || !d->ja || !d->vd || !d->vs) {
+    bn_set_error("device out of memory uploading a %d x %d CSR (%d nnz)", m->row, m->col, m->nnz);
+    return BN_ERR_HIP;
+  }
+  float *tmp = (float *)bn_xmalloc((size_t)m->nnz * sizeof(float));
+  for (int k = 0; k < m->nnz; k++) tmp[k] = (float)m->val[k];
+  CK(hipMemcpy(d->ia, m->ia, ((size_t)m->row + 1) * sizeof(int), hipMemcpyHostToDevice));
+  CK(hipMemcpy(d->ja, m->ja, (size_t)m->nnz * sizeof(int), hipMemcpyHostToDevice));
+  CK(hipMemcpy(d->vd, m->val, (size_t)m->nnz * sizeof(double), hipMemcpyHostToDevice));
+  CK(hipMemcpy(d->vs, tmp, (size_t)m->nnz * sizeof(float), hipMemcpyHostToDevice));
+  free(tmp);
+  return 0;
+}
+
+static int upload_vec2(struct bn_solver *s, const double *h, int n, double **vd, float **vs) {
+  *vd = (double *)dmalloc(s, (size_t)n * sizeof(double));
+  *vs = (float *)dmalloc(s, (size_t)n * sizeof(float));
+  if (!*vd || !*vs) {
+    bn_set_error("device out of memory (vector of %d)", n);
+    return BN_ERR_HIP;
+  }
+  float *tmp = (float *)bn_xmalloc((size_t)n * sizeof(float));
+  for (int i = 0; i < n; i++) tmp[i] = (float)h[i];
+  CK(hipMemcpy(*vd, h, (size_t)n * sizeof(double), hipMemcpyHostToDevice));
+  CK(hipMemcpy(*vs, tmp, (size_t)n * sizeof(float), hipMemcpyHostToDevice));
+  free(tmp);
+  return 0;
+}
+
+/* diagonal inverse as the GS row update uses it: 1/a_ii, 1.0 if absent
+ * (C_amg1/src/amg_cycle.c:57); l1 inverse 1/sum|a_ij| */
+static void diag_inverses(const dcsr_mat *A, double *dinv, double *l1inv) {
+  for (int i = 0; i < A->row; i++) {
+    double d = 1.0, l1 = 0.0;
+    for (int k = A->ia[i]; k < A->ia[i + 1]; k++) {
+      if (A->ja[k] == i) d = A->val[k];
+      l1 += fabs(A->val[k]);
+    }
+    dinv[i] = 1.0 / d;
+    if (l1inv) l1inv[i] = l1 > 0.0 ? 1.0 / l1 : 1.0;
+  }
+}
+
+/* Upload a triangular factor L (rows hold columns <= i, diagonal
+ * included) given in original numbering, relabelled by perm/inv. */
+static int upload_tri(struct bn_solver *s, const ichol_factor *F, const int *perm, const int *inv,
+                      bn_dtri *t) {
+  const int n = F->n;
+  dcsr_mat Lh = {n, n, F->nnz, F->ia, F->ja, F->val};
+  dcsr_mat Uh;
+  bn_csr_transpose(&Lh, &Uh);
+  int *rows_f = (int *)bn_xmalloc((size_t)n * sizeof(int));
+  int *rows_b = (int *)bn_xmalloc((size_t)n * sizeof(int));
+  t->nlev_f = bn_level_sets(Lh.ia, Lh.ja, n, 1, rows_f, &t->ptr_f);
+  t->nlev_b = bn_level_sets(Uh.ia, Uh.ja, n, 0, rows_b, &t->ptr_b);
+  double *dinv = (double *)bn_xmalloc((size_t)n * sizeof(double));
+  for (int i = 0; i < n; i++) {
+    double d = 1.0;
+    for (int k = Lh.ia[i]; k < Lh.ia[i + 1]; k++)
+      if (Lh.ja[k] == i) { d = Lh.val[k]; break; }
+    dinv[i] = 1.0 / d;
+  }
+  dcsr_mat Lp, Up;
+  double *dinv_p = (double *)bn_xmalloc((size_t)n * sizeof(double));
+  if (perm) {
+    bn_csr_permute(&Lh, perm, inv, &Lp);
+    bn_csr_permute(&Uh, perm, inv, &Up);
+    for (int k = 0; k < n; k++) dinv_p[k] = dinv[perm[k]];
+    for (int k = 0; k < n; k++) {
+      rows_f[k] = inv[rows_f[k]];
+      rows_b[k] = inv[rows_b[k]];
+    }
+  } else {
+    bn_csr_copy(&Lh, &Lp);
+    bn_csr_copy(&Uh, &Up);
+    memcpy(dinv_p, dinv, (size_t)n * sizeof(double));
+  }
+  int rc = upload_csr(s, &Lp, &t->lo);
+  if (!rc) rc = upload_csr(s, &Up, &t->up);
+  if (!rc) rc = upload_vec2(s, dinv_p, n, &t->dinv_d, &t->dinv_s);
+  if (!rc) {
+    t->rows_f = (int *)dmalloc(s, (size_t)n * sizeof(int));
+    t->rows_b = (int *)dmalloc(s, (size_t)n * sizeof(int));
+    if (!t->rows_f || !t->rows_b) rc = BN_ERR_HIP;
+  }
+  if (!rc) {
+    CK(hipMemcpy(t->rows_f, rows_f, (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    CK(hipMemcpy(t->rows_b, rows_b, (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+  }
+  t->n = n;
+  bn_csr_free(&Uh);
+  bn_csr_free(&Lp);
+  bn_csr_free(&Up);
+  free(dinv);
+  free(dinv_p);
+  free(rows_f);
+  free(rows_b);
+  return rc;
+}
+
+static int alloc_vec(struct bn_solver *s, void **p, int n) {
+  *p = dmalloc(s, (size_t)(n > 0 ? n : 1) * sizeof(double));
+  if (!*p) {
+    bn_set_error("device out of memory (work vector of %d)", n);
+    return BN_ERR_HIP;
+  }
+  return 0;
+}
+
+static double now_ms(void) {
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return 1e3 * (double)t.tv_sec + 1e-6 * (double)t.tv_nsec;
+}
+
+int bn_tri_upload(struct bn_solver *s, const ichol_factor *F, bn_dtri *t) {
+  return upload_tri(s, F, NULL, NULL, t);
+}
+
+int bn_csr_upload(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) { return upload_csr(s, m, d); }
+
+int bn_device_upload(struct bn_solver *s) {
+  const double t0 = now_ms();
+  if (!bn_device_ok()) {
+    bn_set_error("no HIP device available: the solve runs only on the GPU");
+    return BN_ERR_NO_DEVICE;
+  }
+  if (s->opt.device >= 0) CK(hipSetDevice(s->opt.device));
+  CK(hipGetDevice(&s->device));
+  if (!s->stream) CK(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+  if (!s->ev0) {
+    CK(hipEventCreate(&s->ev0));
+    CK(hipEventCreate(&s->ev1));
+  }
+  s->vt = (s->opt.precision == BN_PREC_FP32) ? 's' : 'd';
+  const amg_data *amg = s->amg;
+  const int nlev = amg->num_levels;
+  s->nlev = nlev;
+  const int split = 1; /* C-then-F colours serve both GS smoothers */
+  int *inv[AMG_MAX_LEVELS];
+  int rc = 0;
+
+  /* colour orders and inverse permutations */
+  for (int l = 0; l < nlev; l++) {
+    const amg_level *L = &amg->levels[l];
+    bn_dlevel *d = &s->lev[l];
+    const int n = L->A.row;
+    d->n = n;
+    d->nc = (l < nlev - 1) ? L->n_coarse : 0;
+    d->perm_h = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+    if (l < nlev - 1) {
+      d->ncol = bn_colour_order(&L->A, L->mis, split, d->perm_h, &d->col_ptr);
+      d->use_gs = d->ncol <= s->opt.max_colours;
+    } else {
+      for (int i = 0; i < n; i++) d->perm_h[i] = i;
+      d->ncol = 0;
+      d->col_ptr = NULL;
+      d->use_gs = 0;
+    }
+    inv[l] = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+    for (int k = 0; k < n; k++) inv[l][d->perm_h[k]] = k;
+  }
+
+  for (int l = 0; l < nlev && !rc; l++) {
+    const amg_level *L = &amg->levels[l];
+    bn_dlevel *d = &s->lev[l];
+    const int n = d->n;
+    dcsr_mat Ap;
+    bn_csr_permute(&L->A, d->perm_h, inv[l], &Ap);
+    rc = upload_csr(s, &Ap, &d->A);
+    double *dinv = (double *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    double *l1inv = (double *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+    diag_inverses(&Ap, dinv, l1inv);
+    if (!rc) rc = upload_vec2(s, dinv, n, &d->dinv_d, &d->dinv_s);
+    if (!rc) rc = upload_vec2(s, l1inv, n, &d->l1inv_d, &d->l1inv_s);
+    free(dinv);
+    free(l1inv);
+    bn_csr_free(&Ap);
+    if (!rc && l < nlev - 1) {
+      dcsr_mat Pp, Rp;
+      bn_csr_permute(&L->P, d->perm_h, inv[l + 1], &Pp);
+      bn_csr_transpose(&Pp, &Rp);
+      rc = upload_csr(s, &Pp, &d->P);
+      if (!rc) rc = upload_csr(s, &Rp, &d->R);
+      bn_csr_free(&Pp);
+      bn_csr_free(&Rp);
+    }
+    if (!rc) {
+      d->perm_d = (int *)dmalloc(s, (size_t)(n > 0 ? n : 1) * sizeof(int));
+      if (!d->perm_d) rc = BN_ERR_HIP;
+      else CK(hipMemcpy(d->perm_d, d->perm_h, (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+    }
+    if (!rc) rc = alloc_vec(s, &d->b, n);
+    if (!rc) rc = alloc_vec(s, &d->x, n);
+    if (!rc) rc = alloc_vec(s, &d->r, n);
+  }
+
+  /* solve matrix: shares level 0 unless set_matrix replaced it */
+  if (!rc) {
+    const dcsr_mat *A0 = &amg->levels[0].A;
+    const dcsr_mat *As = s->A ? s->A : A0;
+    int same = (As == A0) ||
+               (As->row == A0->row && As->nnz == A0->nnz &&
+                !memcmp(As->ia, A0->ia, ((size_t)A0->row + 1) * sizeof(int)) &&
+                !memcmp(As->ja, A0->ja, (size_t)A0->nnz * sizeof(int)) &&
+                !memcmp(As->val, A0->val, (size_t)A0->nnz * sizeof(double)));
+    s->asolve_shared = same;
+    if (same) {
+      s->Asolve = s->lev[0].A;
+    } else {
+      dcsr_mat Ap;
+      bn_csr_permute(As, s->lev[0].perm_h, inv[0], &Ap);
+      rc = upload_csr(s, &Ap, &s->Asolve);
+      bn_csr_free(&Ap);
+    }
+  }
+  /* fine IC(0) (absent for hierarchy-only device mirrors) */
+  if (!rc && s->L) rc = upload_tri(s, s->L, s->lev[0].perm_h, inv[0], &s->fine_ic);
+
+  /* coarsest solver */
+  if (!rc) {
+    const amg_level *C = &amg->levels[nlev - 1];
+    const int nc = C->A.row;
+    int dense = (s->opt.coarse_solver == BN_COARSE_DENSE) ||
+                (s->opt.coarse_solver == BN_COARSE_AUTO && nc <= BN_DENSE_MAX);
+    if (nlev == 1 && s->opt.coarse_solver == BN_COARSE_AUTO) dense = (nc <= BN_DENSE_MAX);
+    s->coarse_dense = dense;
+    if (dense) {
+      double *inv_h = (double *)bn_xmalloc((size_t)nc * (size_t)nc * sizeof(double));
+      if (bn_dense_inverse(&C->A, inv_h) && s->opt.verbose)
+        fprintf(stderr, "bnamg: coarsest operator not numerically SPD (pivot clamped)\n");
+      rc = upload_vec2(s, inv_h, nc * nc, &s->cinv_d, &s->cinv_s);
+      free(inv_h);
+    } else {
+      rc = upload_tri(s, &C->L_ichol, NULL, NULL, &s->coarse_ic);
+    }
+    for (int k = 0; k < 6 && !rc; k++) rc = alloc_vec(s, (void **)&s->cvec[k], nc);
+  }
+
+  /* PCG and preconditioner work space */
+  if (!rc) {
+    const int n = s->lev[0].n;
+    void **pv[] = {(void **)&s->x, (void **)&s->r, (void **)&s->z, (void **)&s->p, (void **)&s->q,
+                   (void **)&s->bb, (void **)&s->rprev, &s->g_in, &s->u1, &s->r1, &s->icy,
+                   &s->icx, &s->r2, &s->corr, &s->zout};
+    for (size_t k = 0; k < sizeof(pv) / sizeof(pv[0]) && !rc; k++) rc = alloc_vec(s, pv[k], n);
+    if (!rc) rc = alloc_vec(s, (void **)&s->scal, BN_S_COUNT * 2);
+    if (!rc) rc = alloc_vec(s, (void **)&s->part1, BN_RED_BLOCKS);
+    if (!rc) rc = alloc_vec(s, (void **)&s->part2, BN_RED_BLOCKS);
+    if (!rc) rc = alloc_vec(s, (void **)&s->part3, BN_RED_BLOCKS);
+    if (!rc) rc = alloc_vec(s, (void **)&s->cpart1, BN_RED_BLOCKS);
+    if (!rc) rc = alloc_vec(s, (void **)&s->cpart2, BN_RED_BLOCKS);
+    if (!rc && !s->host_scal) CK(hipHostMalloc((void **)&s->host_scal, BN_S_COUNT * 2 * sizeof(double), 0));
+  }
+  for (int l = 0; l < nlev; l++) free(inv[l]);
+  if (!rc) {
+    CK(hipDeviceSynchronize());
+    s->dev_ready = 1;
+  }
+  s->last.upload_ms = now_ms() - t0;
+  return rc;
+}

@@ -1,0 +1,86 @@
+/*
+ * extras.c - entry points that reuse one hierarchy for more than the
+ * plain solve: the adjoint gradient and multi-right-hand-side solves.
+ */
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "bn_solver.h"
+
+void bn_set_error(const char *fmt, ...);
+void bn_track_device(struct bn_solver *s, void *p);
+
+/* A is symmetric, so the adjoint system A lam = dL/dp is solved with the
+ * same hierarchy and PCG; the edge-scatter kernel then forms
+ *   dL/dg_e = -(lam_u - lam_v)(p_u - p_v)   (Dirichlet ends: lam = 0,
+ *                                            p = pdir)
+ *   dL/dlog r_e = 4 g_e dL/dg_e             (g = pi r^4 / (8 mu len)). */
+int bn_solver_adjoint_grad(void *h, const double *p, const double *dLdp, int m, const int *eu, const int *ev,
+                           const double *g, const double *pdir, double tol, int maxiter, double *lam,
+                           double *dLdg, double *dLdlogr) {
+  struct bn_solver *s = (struct bn_solver *)h;
+  if (!s || !p || !dLdp || !lam || m < 0 || (m > 0 && (!eu || !ev || !g))) return BN_ERR_ARG;
+  const int n = s->A->row;
+  memset(lam, 0, (size_t)n * sizeof(double));
+  int it = bn_solver_solve_info(h, dLdp, lam, tol, maxiter, 0, NULL);
+  if (it == BN_ERR_HIP || it == BN_ERR_NO_DEVICE || it == BN_ERR_ARG) return it;
+  if (m == 0 || (!dLdg && !dLdlogr)) return it;
+  int nd = 0;
+  for (int e = 0; e < m; e++) {
+    if (eu[e] < -nd) nd = -eu[e];
+    if (ev[e] < -nd) nd = -ev[e];
+  }
+  if (nd > 0 && !pdir) return BN_ERR_ARG;
+  void *buf[8] = {0};
+  size_t sz[8] = {(size_t)m * 4, (size_t)m * 4, (size_t)m * 8, (size_t)n * 8, (size_t)n * 8,
+                  (size_t)(nd > 0 ? nd : 1) * 8, (size_t)m * 8, (size_t)m * 8};
+  for (int k = 0; k < 8; k++)
+    if (hipMalloc(&buf[k], sz[k] ? sz[k] : 8) != hipSuccess) {
+      for (int q = 0; q < k; q++) hipFree(buf[q]);
+      bn_set_error("device out of memory in the adjoint gradient");
+      return BN_ERR_HIP;
+    }
+  hipStream_t st = s->stream;
+  hipMemcpyAsync(buf[0], eu, sz[0], hipMemcpyHostToDevice, st);
+  hipMemcpyAsync(buf[1], ev, sz[1], hipMemcpyHostToDevice, st);
+  hipMemcpyAsync(buf[2], g, sz[2], hipMemcpyHostToDevice, st);
+  hipMemcpyAsync(buf[3], p, sz[3], hipMemcpyHostToDevice, st);
+  hipMemcpyAsync(buf[4], lam, sz[4], hipMemcpyHostToDevice, st);
+  if (nd > 0) hipMemcpyAsync(buf[5], pdir, (size_t)nd * 8, hipMemcpyHostToDevice, st);
+  bn_edge_grad(m, (const int *)buf[0], (const int *)buf[1], (const double *)buf[2], (const double *)buf[3],
+               (const double *)buf[4], (const double *)buf[5], (double *)buf[6], (double *)buf[7], st);
+  if (dLdg) hipMemcpyAsync(dLdg, buf[6], sz[6], hipMemcpyDeviceToHost, st);
+  if (dLdlogr) hipMemcpyAsync(dLdlogr, buf[7], sz[7], hipMemcpyDeviceToHost, st);
+  hipError_t e = hipStreamSynchronize(st);
+  for (int k = 0; k < 8; k++) hipFree(buf[k]);
+  return e == hipSuccess ? it : BN_ERR_HIP;
+}
+
+/* k right-hand sides, column-major; every column is solved to tol with
+ * the shared hierarchy.  its[j] gets column j's iteration count; the
+ * return value is the largest count (negative if any column failed). */
+int bn_solver_solve_batch(void *h, int k, const double *b, double *x, double tol, int maxiter, int *its,
+                          bn_solve_info *info) {
+  struct bn_solver *s = (struct bn_solver *)h;
+  if (!s || k < 0 || (k > 0 && (!b || !x))) return BN_ERR_ARG;
+  const size_t n = (size_t)s->A->row;
+  int worst = 0, failed = 0;
+  double ms = 0.0, wall = 0.0;
+  for (int j = 0; j < k; j++) {
+    bn_solve_info inf;
+    int it = bn_solver_solve_info(h, b + j * n, x + j * n, tol, maxiter, 0, &inf);
+    if (it == BN_ERR_HIP || it == BN_ERR_NO_DEVICE || it == BN_ERR_ARG) return it;
+    if (its) its[j] = it;
+    if (it < 0) failed = 1;
+    if ((it < 0 ? -it : it) > worst) worst = it < 0 ? -it : it;
+    ms += inf.solve_ms;
+    wall += inf.wall_ms;
+  }
+  if (info) {
+    *info = s->last;
+    info->solve_ms = ms;
+    info->wall_ms = wall;
+  }
+  return failed ? -worst : worst;
+}

@@ -1,0 +1,133 @@
+/*
+ * schedule.c - host analysis that turns sequential sweeps into parallel
+ * schedules for the device:
+ *   - greedy multicolouring of the C set and of the F set of a level
+ *     (C-then-F multicolour Gauss-Seidel), or of all points (plain GS);
+ *   - level sets of a triangular factor (dependency depth per row) for
+ *     level-scheduled triangular solves.
+ */
+#include <stdlib.h>
+#include <string.h>
+
+#include "schedule.h"
+#include "csr.h"
+
+/* Greedy colouring of the nodes with cls[i] == want (visited ascending):
+ * each takes the smallest colour unused by already coloured neighbours
+ * of the same class.  Returns the number of colours; col[i] set for the
+ * class members. */
+static int greedy_class(const dcsr_mat *A, const int *cls, int want, int *col) {
+  const int n = A->row;
+  int cap = 64, ncol = 0;
+  int *stamp = (int *)bn_xmalloc((size_t)cap * sizeof(int));
+  for (int c = 0; c < cap; c++) stamp[c] = -1;
+  for (int i = 0; i < n; i++) {
+    if (cls && cls[i] != want) continue;
+    for (int k = A->ia[i]; k < A->ia[i + 1]; k++) {
+      const int j = A->ja[k];
+      if (j == i || (cls && cls[j] != want) || col[j] < 0) continue;
+      stamp[col[j]] = i;
+    }
+    int c = 0;
+    while (c < cap && stamp[c] == i) c++;
+    if (c == cap) {
+      int nc = cap * 2;
+      stamp = (int *)realloc(stamp, (size_t)nc * sizeof(int));
+      for (int q = cap; q < nc; q++) stamp[q] = -1;
+      cap = nc;
+    }
+    col[i] = c;
+    if (c + 1 > ncol) ncol = c + 1;
+  }
+  free(stamp);
+  return ncol;
+}
+
+int bn_colour_order(const dcsr_mat *A, const int *mis, int split, int *perm, int **col_ptr) {
+  const int n = A->row;
+  int *col = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  for (int i = 0; i < n; i++) col[i] = -1;
+  int *cls = NULL;
+  int nclass = 1;
+  if (split && mis) {
+    cls = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+    for (int i = 0; i < n; i++) cls[i] = mis[i] ? 0 : 1; /* C first */
+    nclass = 2;
+  }
+  int ncol_cls[2] = {0, 0};
+  for (int c = 0; c < nclass; c++) ncol_cls[c] = greedy_class(A, cls, c, col);
+  const int ncol = ncol_cls[0] + ncol_cls[1];
+  /* global colour id: C colours first, then F colours */
+  int *cnt = (int *)bn_xcalloc((size_t)ncol + 1, sizeof(int));
+  for (int i = 0; i < n; i++) {
+    int g = col[i] + ((cls && cls[i] == 1) ? ncol_cls[0] : 0);
+    col[i] = g;
+    cnt[g + 1]++;
+  }
+  for (int c = 0; c < ncol; c++) cnt[c + 1] += cnt[c];
+  int *ptr = (int *)bn_xmalloc(((size_t)ncol + 1) * sizeof(int));
+  memcpy(ptr, cnt, ((size_t)ncol + 1) * sizeof(int));
+  for (int i = 0; i < n; i++) perm[cnt[col[i]]++] = i; /* ascending within a colour */
+  free(cnt);
+  free(col);
+  free(cls);
+  *col_ptr = ptr;
+  return ncol;
+}
+
+int bn_level_sets(const int *ia, const int *ja, int n, int lower, int *rows, int **lev_ptr) {
+  int *depth = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  int nlev = 0;
+  if (lower) {
+    for (int i = 0; i < n; i++) {
+      int d = 0;
+      for (int k = ia[i]; k < ia[i + 1]; k++) {
+        const int j = ja[k];
+        if (j < i && depth[j] + 1 > d) d = depth[j] + 1;
+      }
+      depth[i] = d;
+      if (d + 1 > nlev) nlev = d + 1;
+    }
+  } else {
+    for (int i = n - 1; i >= 0; i--) {
+      int d = 0;
+      for (int k = ia[i]; k < ia[i + 1]; k++) {
+        const int j = ja[k];
+        if (j > i && depth[j] + 1 > d) d = depth[j] + 1;
+      }
+      depth[i] = d;
+      if (d + 1 > nlev) nlev = d + 1;
+    }
+  }
+  int *cnt = (int *)bn_xcalloc((size_t)nlev + 1, sizeof(int));
+  for (int i = 0; i < n; i++) cnt[depth[i] + 1]++;
+  for (int l = 0; l < nlev; l++) cnt[l + 1] += cnt[l];
+  int *ptr = (int *)bn_xmalloc(((size_t)nlev + 1) * sizeof(int));
+  memcpy(ptr, cnt, ((size_t)nlev + 1) * sizeof(int));
+  for (int i = 0; i < n; i++) rows[cnt[depth[i]]++] = i;
+  free(cnt);
+  free(depth);
+  *lev_ptr = ptr;
+  return nlev;
+}
+
+void bn_csr_permute(const dcsr_mat *A, const int *perm_r, const int *inv_c, dcsr_mat *B) {
+  const int n = A->row;
+  bn_csr_alloc(B, A->row, A->col, A->nnz);
+  int *cnt = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  for (int k = 0; k < n; k++) {
+    const int i = perm_r ? perm_r[k] : k;
+    cnt[k] = A->ia[i + 1] - A->ia[i];
+  }
+  bn_prefix(cnt, n, B->ia);
+  free(cnt);
+#pragma omp parallel for schedule(static)
+  for (int k = 0; k < n; k++) {
+    const int i = perm_r ? perm_r[k] : k;
+    int p = B->ia[k];
+    for (int q = A->ia[i]; q < A->ia[i + 1]; q++, p++) {
+      B->ja[p] = inv_c ? inv_c[A->ja[q]] : A->ja[q];
+      B->val[p] = A->val[q];
+    }
+  }
+}

@@ -1,0 +1,291 @@
+/*
+ * vcycle.c - device V-cycle halves and the AMG + IC(0) preconditioner.
+ *
+ * Host C that only enqueues kernels on the solver's stream; all vectors
+ * stay on the device.  Structure follows the reference:
+ *   backslash  C_amg1/src/amg_cycle.c:180-217 (pre-smooth, residual,
+ *              restrict, recurse from zero, prolong)
+ *   fwdslash   C_amg1/src/amg_cycle.c:222-258 (residual, restrict,
+ *              recurse, prolong, post-smooth in reverse order)
+ *   AMG+IC     C_amg1/src/amg_ichol_precond.c:22-59
+ *   V-cycle    C_amg1/src/amg_cycle.c:263-284
+ * Differences by design: the CF-ordered Gauss-Seidel sweep runs as a
+ * C-then-F multicolour sweep (levels needing more than max_colours
+ * colours fall back to l1-Jacobi), restriction gathers over a stored
+ * R = P^T instead of scattering, the coarsest solve is a dense inverse
+ * (or the reference IC-PCG), and zero initial guesses skip the work they
+ * make redundant (a residual equal to b, the first colour's row sums).
+ */
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+
+#include "bn_solver.h"
+
+static inline const void *opv(const bn_dcsr *m, char vt) {
+  return vt == 's' ? (const void *)m->vs : (const void *)m->vd;
+}
+static inline const void *vecv(const double *vd, const float *vs, char vt) {
+  return vt == 's' ? (const void *)vs : (const void *)vd;
+}
+
+/* y = op(M, x [, b]) in precision vt */
+static void apply(struct bn_solver *s, const bn_dcsr *M, int op, const void *x, void *y, const void *b) {
+  bn_csr_apply(s->vt, op, M->rows, M->ia, M->ja, opv(M, s->vt), x, y, b, M->lanes, NULL, NULL, s->stream);
+}
+
+/* nu smoothing sweeps on level `lev`; backward runs colours in reverse.
+ * x_zero: x is zero on entry (first colour then needs no row sums).
+ * zsum != NULL: the final sweep also writes zsum = zbase + x. */
+static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int backward, int x_zero,
+                   const void *zbase, void *zsum) {
+  bn_dlevel *d = &s->lev[lev];
+  const char vt = s->vt;
+  const int nu = s->opt.nu > 0 ? s->opt.nu : 1;
+  const int n = d->n;
+  if (x_zero) bn_fill(vt, n, x, 0.0, s->stream);
+  const int gs = d->use_gs && (s->opt.smoother == SMOOTHER_CF_GS || s->opt.smoother == SMOOTHER_GS);
+  if (gs) {
+    for (int sw = 0; sw < nu; sw++) {
+      const int last = (sw == nu - 1);
+      for (int t = 0; t < d->ncol; t++) {
+        const int c = backward ? d->ncol - 1 - t : t;
+        const int lo = d->col_ptr[c], hi = d->col_ptr[c + 1];
+        if (sw == 0 && t == 0 && x_zero && !(last && zsum)) {
+          bn_gs_color_zero(vt, lo, hi, vecv(d->dinv_d, d->dinv_s, vt), b, x, s->stream);
+          continue;
+        }
+        bn_gs_color(vt, lo, hi, d->A.ia, d->A.ja, opv(&d->A, vt), vecv(d->dinv_d, d->dinv_s, vt), b, x,
+                    d->A.lanes, last ? zbase : NULL, last ? zsum : NULL, s->stream);
+      }
+    }
+  } else {
+    const int jac = (s->opt.smoother == SMOOTHER_JACOBI);
+    const void *w = jac ? vecv(d->dinv_d, d->dinv_s, vt) : vecv(d->l1inv_d, d->l1inv_s, vt);
+    const double om = jac ? s->opt.omega : 1.0;
+    for (int sw = 0; sw < nu; sw++) {
+      if (sw == 0 && x_zero) {
+        /* x = om * w * b */
+        bn_copy(vt, n, b, d->r, s->stream);
+      } else {
+        apply(s, &d->A, BN_OP_RESID, x, d->r, b);
+      }
+      bn_jacobi_update(vt, n, d->r, w, om, x, s->stream);
+    }
+    if (zsum) {
+      bn_copy(vt, n, zbase, zsum, s->stream);
+      bn_axpy1(vt, n, x, zsum, s->stream);
+    }
+  }
+}
+
+/* x = A_c^{-1} b on the coarsest level */
+static void coarse_solve(struct bn_solver *s, const void *b, void *x);
+
+void bn_vc_backslash(struct bn_solver *s, int lev, const void *b, void *x, int x_zero) {
+  bn_dlevel *d = &s->lev[lev];
+  if (lev == s->nlev - 1) {
+    coarse_solve(s, b, x);
+    return;
+  }
+  bn_dlevel *c = &s->lev[lev + 1];
+  smooth(s, lev, b, x, 0, x_zero, NULL, NULL);
+  apply(s, &d->A, BN_OP_RESID, x, d->r, b);
+  apply(s, &d->R, BN_OP_SET, d->r, c->b, NULL);
+  bn_vc_backslash(s, lev + 1, c->b, c->x, 1);
+  apply(s, &d->P, BN_OP_ADD, c->x, x, NULL);
+}
+
+static void fwdslash_rec(struct bn_solver *s, int lev, const void *b, void *x, int x_zero,
+                         const void *zbase, void *zsum) {
+  bn_dlevel *d = &s->lev[lev];
+  if (lev == s->nlev - 1) {
+    coarse_solve(s, b, x);
+    if (zsum) {
+      bn_copy(s->vt, d->n, zbase, zsum, s->stream);
+      bn_axpy1(s->vt, d->n, x, zsum, s->stream);
+    }
+    return;
+  }
+  bn_dlevel *c = &s->lev[lev + 1];
+  if (x_zero) {
+    apply(s, &d->R, BN_OP_SET, b, c->b, NULL);
+  } else {
+    apply(s, &d->A, BN_OP_RESID, x, d->r, b);
+    apply(s, &d->R, BN_OP_SET, d->r, c->b, NULL);
+  }
+  fwdslash_rec(s, lev + 1, c->b, c->x, 1, NULL, NULL);
+  apply(s, &d->P, x_zero ? BN_OP_SET : BN_OP_ADD, c->x, x, NULL);
+  smooth(s, lev, b, x, 1, 0, zbase, zsum);
+}
+
+void bn_vc_fwdslash(struct bn_solver *s, int lev, const void *b, void *x, int x_zero) {
+  fwdslash_rec(s, lev, b, x, x_zero, NULL, NULL);
+}
+
+/* ---------------------------------------------------------------------
+ * coarsest level
+ * ------------------------------------------------------------------- */
+static int coarse_icpcg(struct bn_solver *s, const double *b, double *x);
+
+/* x = A_c^{-1} b.  The dense path ignores x; the IC-PCG path starts from
+ * x, which every internal caller has zeroed (the reference recursion
+ * calloc's it, C_amg1/src/amg_cycle.c:205,243). */
+static void coarse_solve(struct bn_solver *s, const void *b, void *x) {
+  const int n = s->lev[s->nlev - 1].n;
+  if (s->vt == 'd') {
+    if (s->coarse_dense) {
+      bn_dense_gemv('d', n, s->cinv_d, b, x, s->stream);
+    } else {
+      bn_fill('d', n, x, 0.0, s->stream);
+      coarse_icpcg(s, (const double *)b, (double *)x);
+    }
+    return;
+  }
+  /* FP32 V-cycle: the coarsest solve stays FP64 */
+  double *bd = s->cvec[4], *xd = s->cvec[5];
+  bn_convert('s', 'd', n, b, bd, s->stream);
+  if (s->coarse_dense) {
+    bn_dense_gemv('d', n, s->cinv_d, bd, xd, s->stream);
+  } else {
+    bn_fill('d', n, xd, 0.0, s->stream);
+    coarse_icpcg(s, bd, xd);
+  }
+  bn_convert('d', 's', n, xd, x, s->stream);
+}
+
+/* IC(0) application with level-scheduled forward and backward sweeps:
+ * x = (L L^T)^{-1} b  (C_amg1/src/ichol.c:121-165).  If zsum, also
+ * zsum = zbase + x. */
+static void tri_solve(struct bn_solver *s, bn_dtri *t, char vt, const void *b, void *y, void *x,
+                      const void *zbase, void *zsum) {
+  const void *dinv = vt == 's' ? (const void *)t->dinv_s : (const void *)t->dinv_d;
+  for (int l = 0; l < t->nlev_f; l++)
+    bn_trsv_level(vt, t->ptr_f[l + 1] - t->ptr_f[l], t->rows_f + t->ptr_f[l], t->lo.ia, t->lo.ja,
+                  opv(&t->lo, vt), dinv, b, y, t->lo.lanes, NULL, NULL, s->stream);
+  for (int l = 0; l < t->nlev_b; l++)
+    bn_trsv_level(vt, t->ptr_b[l + 1] - t->ptr_b[l], t->rows_b + t->ptr_b[l], t->up.ia, t->up.ja,
+                  opv(&t->up, vt), dinv, y, x, t->up.lanes, zbase, zsum, s->stream);
+}
+
+/* k Jacobi sweeps per triangle (approximate IC application; only used in
+ * the flexible outer iteration) */
+static void tri_jacobi(struct bn_solver *s, bn_dtri *t, char vt, const void *b, void *y, void *x,
+                       void *tmp, int sweeps, const void *zbase, void *zsum) {
+  const void *dinv = vt == 's' ? (const void *)t->dinv_s : (const void *)t->dinv_d;
+  const int n = t->n;
+  void *bufs[2] = {y, tmp};
+  int cur = 0;
+  bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, NULL, bufs[cur], t->lo.lanes,
+                 s->stream);
+  for (int k = 1; k < sweeps; k++) {
+    bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, bufs[cur], bufs[1 - cur],
+                   t->lo.lanes, s->stream);
+    cur = 1 - cur;
+  }
+  const void *yy = bufs[cur];
+  void *xb[2] = {x, bufs[1 - cur]};
+  int cx = 0;
+  bn_jacobi_rows(vt, n, t->up.ia, t->up.ja, opv(&t->up, vt), dinv, yy, NULL, xb[cx], t->up.lanes,
+                 s->stream);
+  for (int k = 1; k < sweeps; k++) {
+    bn_jacobi_rows(vt, n, t->up.ia, t->up.ja, opv(&t->up, vt), dinv, yy, xb[cx], xb[1 - cx],
+                   t->up.lanes, s->stream);
+    cx = 1 - cx;
+  }
+  if (xb[cx] != x) bn_copy(vt, n, xb[cx], x, s->stream);
+  if (zsum) {
+    bn_copy(vt, n, zbase, zsum, s->stream);
+    bn_axpy1(vt, n, x, zsum, s->stream);
+  }
+}
+
+void bn_vc_tri_apply(struct bn_solver *s, bn_dtri *t, char vt, const void *b, void *y, void *x) {
+  tri_solve(s, t, vt, b, y, x, NULL, NULL);
+}
+
+void bn_vc_fine_ic(struct bn_solver *s, const void *b, void *x) {
+  if (s->opt.fine_ic == BN_IC_JACOBI)
+    tri_jacobi(s, &s->fine_ic, s->vt, b, s->icy, x, s->lev[0].r, s->opt.ic_sweeps, NULL, NULL);
+  else
+    tri_solve(s, &s->fine_ic, s->vt, b, s->icy, x, NULL, NULL);
+}
+
+/* IC-PCG on the coarsest level, FP64, to 1e-12 with at most n_c steps
+ * (C_amg1/src/amg_cycle.c:186-190, amg_pcg.c:12-101).  Host-driven loop
+ * with its own scalars and partial buffers (it runs inside the outer
+ * PCG's preconditioner). */
+static int coarse_icpcg(struct bn_solver *s, const double *b, double *x) {
+  const bn_dlevel *C = &s->lev[s->nlev - 1];
+  const int n = C->n;
+  double *r = s->cvec[0], *z = s->cvec[1], *p = s->cvec[2], *q = s->cvec[3];
+  double *y = (double *)C->r; /* forward-sweep buffer */
+  double *sc = s->scal + BN_S_COUNT;
+  hipStream_t st = s->stream;
+  const double *Av = C->A.vd;
+  /* x is the initial guess: r = b - A x */
+  bn_csr_apply('d', BN_OP_RESID, n, C->A.ia, C->A.ja, Av, x, r, b, C->A.lanes, NULL, NULL, st);
+  tri_solve(s, &s->coarse_ic, 'd', r, y, z, NULL, NULL);
+  bn_copy('d', n, z, p, st);
+  double *pa = s->cpart1, *pb = s->cpart2;
+  bn_dot_partial(n, r, z, r, r, pa, pb, st);
+  bn_scalar_finish(0, pa, pb, NULL, sc, 0, st);
+  int it;
+  for (it = 1; it <= n; it++) {
+    bn_csr_apply('d', BN_OP_SET, n, C->A.ia, C->A.ja, Av, p, q, NULL, C->A.lanes, p, pa, st);
+    bn_scalar_finish(1, pa, NULL, NULL, sc, 0, st);
+    bn_pcg_update_xr(n, x, p, r, q, sc, pb, st);
+    tri_solve(s, &s->coarse_ic, 'd', r, y, z, NULL, NULL);
+    bn_dot_partial(n, r, z, NULL, NULL, pa, NULL, st);
+    bn_scalar_finish(2, pa, pb, NULL, sc, 0, st);
+    hipMemcpyAsync(s->host_scal + BN_S_COUNT, sc, BN_S_COUNT * sizeof(double), hipMemcpyDeviceToHost, st);
+    hipStreamSynchronize(st);
+    const double rel = s->host_scal[BN_S_COUNT + BN_S_REL];
+    if (rel < 1e-12 || !(rel == rel)) break;
+    bn_pcg_update_p(n, p, z, sc, st);
+  }
+  return it;
+}
+
+/* ---------------------------------------------------------------------
+ * preconditioners (input g and output z are FP64, level-0 numbering)
+ * ------------------------------------------------------------------- */
+void bn_vc_ichol_precond(struct bn_solver *s, const double *g, double *z) {
+  const char vt = s->vt;
+  const int n = s->lev[0].n;
+  const void *gv = g;
+  if (vt == 's') {
+    bn_convert('d', 's', n, g, s->g_in, s->stream);
+    gv = s->g_in;
+  }
+  const bn_dcsr *A = &s->Asolve;
+  /* u1 = backslash(g) from zero */
+  bn_vc_backslash(s, 0, gv, s->u1, 1);
+  /* r1 = g - A u1; u2 = u1 + (L L^T)^{-1} r1 (u2 kept in icx's partner) */
+  apply(s, A, BN_OP_RESID, s->u1, s->r1, gv);
+  if (s->opt.fine_ic == BN_IC_JACOBI)
+    tri_jacobi(s, &s->fine_ic, vt, s->r1, s->icy, s->icx, s->lev[0].r, s->opt.ic_sweeps, s->u1, s->r2);
+  else
+    tri_solve(s, &s->fine_ic, vt, s->r1, s->icy, s->icx, s->u1, s->r2);
+  /* s->r2 now holds u2; r1 <- g - A u2 */
+  apply(s, A, BN_OP_RESID, s->r2, s->r1, gv);
+  /* z = u2 + fwdslash(g - A u2) from zero, final update fused */
+  void *zv = (vt == 's') ? s->zout : (void *)z;
+  fwdslash_rec(s, 0, s->r1, s->corr, 1, s->r2, zv);
+  if (vt == 's') bn_convert('s', 'd', n, s->zout, z, s->stream);
+}
+
+void bn_vc_vcycle_precond(struct bn_solver *s, const double *g, double *z) {
+  const char vt = s->vt;
+  const int n = s->lev[0].n;
+  const void *gv = g;
+  if (vt == 's') {
+    bn_convert('d', 's', n, g, s->g_in, s->stream);
+    gv = s->g_in;
+  }
+  bn_vc_backslash(s, 0, gv, s->u1, 1);
+  apply(s, &s->lev[0].A, BN_OP_RESID, s->u1, s->r1, gv);
+  void *zv = (vt == 's') ? s->zout : (void *)z;
+  fwdslash_rec(s, 0, s->r1, s->corr, 1, s->u1, zv);
+  if (vt == 's') bn_convert('s', 'd', n, s->zout, z, s->stream);
+}

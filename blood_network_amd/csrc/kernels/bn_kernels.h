@@ -1,0 +1,136 @@
+/*
+ * bn_kernels.h - C launchers for the gfx950 HIP kernels (internal ABI
+ * between the C host code and kernels.hip).  All pointers are device
+ * pointers; every launcher enqueues on `stream` and returns at once.
+ * Value type codes: 'd' = FP64 storage and arithmetic, 's' = FP32.
+ */
+#ifndef BN_KERNELS_H
+#define BN_KERNELS_H
+
+#include <hip/hip_runtime_api.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* scalar slots in the per-solver device scalar block */
+enum {
+  BN_S_RZ = 0,     /* r^T z of the current iteration            */
+  BN_S_RZ0 = 1,    /* r0^T z0                                   */
+  BN_S_PAP = 2,    /* p^T A p                                   */
+  BN_S_ALPHA = 3,
+  BN_S_BETA = 4,
+  BN_S_RR = 5,     /* r^T r                                     */
+  BN_S_RZNEW = 6,
+  BN_S_REL = 7,    /* sqrt(rz_new / rz0)                        */
+  BN_S_BB = 8,     /* b^T b                                     */
+  BN_S_RZOLD = 9,  /* previous r^T z (flexible beta)            */
+  BN_S_ZRPREV = 10,/* z^T r_prev (flexible beta)                */
+  BN_S_COUNT = 16
+};
+
+/* reduction partial buffer holds at most this many blocks per dot */
+#define BN_RED_BLOCKS 1024
+
+/* CSR operator kinds for bn_csr_apply */
+enum {
+  BN_OP_SET = 0,    /* y = A x                 */
+  BN_OP_RESID = 1,  /* y = b - A x             */
+  BN_OP_ADD = 2     /* y = y + A x             */
+};
+
+/* y = op(A, x [, b]); lanes = threads per row (1..64, power of two).
+ * If partial != NULL also writes per-block partial sums of w_i * y_i
+ * (w = dotw) into partial[0..BN_RED_BLOCKS). */
+void bn_csr_apply(char vt, int op, int n, const int *ia, const int *ja,
+                  const void *val, const void *x, void *y, const void *b,
+                  int lanes, const void *dotw, double *partial,
+                  hipStream_t stream);
+
+/* One multicolour Gauss-Seidel colour: rows lo..hi-1 of a level whose
+ * rows are stored in colour order:
+ *   x_i <- (b_i - sum_{j != i} a_ij x_j) * dinv_i.
+ * If zsum != NULL, also writes z_i = zbase_i + x_i (fused final update). */
+void bn_gs_color(char vt, int lo, int hi, const int *ia, const int *ja,
+                 const void *val, const void *dinv, const void *b, void *x,
+                 int lanes, const void *zbase, void *zsum,
+                 hipStream_t stream);
+/* first colour of a sweep that starts from x = 0: x_i = b_i * dinv_i */
+void bn_gs_color_zero(char vt, int lo, int hi, const void *dinv,
+                      const void *b, void *x, hipStream_t stream);
+
+/* Jacobi-type sweep pieces: x_i += w_i * (b_i - (A x)_i), w_i = scale/d_i;
+ * the residual must come from a separate bn_csr_apply into r. */
+void bn_jacobi_update(char vt, int n, const void *r, const void *dscale,
+                      double omega, void *x, hipStream_t stream);
+
+/* One level of a level-scheduled triangular solve: for the `cnt` rows in
+ * rows[], y_i = (b_i - sum_{j != i} T_ij y_j) * dinv_i.  If zsum != NULL,
+ * also z_i = zbase_i + y_i. */
+void bn_trsv_level(char vt, int cnt, const int *rows, const int *ia,
+                   const int *ja, const void *val, const void *dinv,
+                   const void *b, void *y, int lanes, const void *zbase,
+                   void *zsum, hipStream_t stream);
+
+/* One Jacobi sweep of a triangular (or any) system over all n rows:
+ * xout_i = (b_i - sum_{j != i} T_ij xin_j) * dinv_i.  xin == NULL means
+ * xin = 0 (xout = b * dinv). */
+void bn_jacobi_rows(char vt, int n, const int *ia, const int *ja, const void *val,
+                    const void *dinv, const void *b, const void *xin, void *xout,
+                    int lanes, hipStream_t stream);
+
+/* dense y = M x (row-major n x n), M in vt, x,y in vt */
+void bn_dense_gemv(char vt, int n, const void *M, const void *x, void *y,
+                   hipStream_t stream);
+
+/* ---- BLAS-1 (FP64 unless noted) ---- */
+void bn_fill(char vt, int n, void *x, double v, hipStream_t stream);
+void bn_copy(char vt, int n, const void *x, void *y, hipStream_t stream);
+/* y(vt_out) = x(vt_in) elementwise conversion */
+void bn_convert(char vt_in, char vt_out, int n, const void *x, void *y,
+                hipStream_t stream);
+/* partial sums of x.y (and of u.v if u != NULL into partial2) */
+void bn_dot_partial(int n, const double *x, const double *y,
+                    const double *u, const double *v, double *partial,
+                    double *partial2, hipStream_t stream);
+/* x += alpha p; r -= alpha q; partial of r.r.  alpha = s[BN_S_ALPHA] */
+void bn_pcg_update_xr(int n, double *x, const double *p, double *r,
+                      const double *q, const double *s, double *partial,
+                      hipStream_t stream);
+/* p = z + beta p, beta = s[BN_S_BETA] */
+void bn_pcg_update_p(int n, double *p, const double *z, const double *s,
+                     hipStream_t stream);
+/* y = y + x (vt) */
+void bn_axpy1(char vt, int n, const void *x, void *y, hipStream_t stream);
+
+/* Scalar-finish kernels (single block): reduce partials and update s[].
+ * which:
+ *   0 start:   s[RZ]=s[RZ0]=sum(p1), s[RR]=sum(p2), s[BB] kept
+ *   1 alpha:   s[PAP]=sum(p1), s[ALPHA]=s[RZ]/s[PAP]
+ *   2 iterate: s[RZNEW]=sum(p1), s[RR]=sum(p2), s[REL]=sqrt(RZNEW/RZ0),
+ *              s[BETA]=RZNEW/RZ, s[RZ]=RZNEW
+ *   3 bnorm:   s[BB]=sum(p1)
+ *   4 iterate-flexible: like 2 but with the Polak-Ribiere beta
+ *              -ALPHA * sum(p3) / RZ, where p3 holds z_new . (A p), i.e.
+ *              z_new . (r_new - r_old) / (r_old . z_old)
+ *   5 plain:   s[slot]=sum(p1) with slot passed in `slot` */
+void bn_scalar_finish(int which, const double *p1, const double *p2,
+                      const double *p3, double *s, int slot,
+                      hipStream_t stream);
+
+/* gather/scatter for permuted level numbering: y[k] = x[perm[k]] (gather)
+ * or y[perm[k]] = x[k] (scatter), FP64 */
+void bn_permute(int n, const int *perm, const double *x, double *y,
+                int scatter, hipStream_t stream);
+
+/* edge-scatter adjoint gradient: for each segment e with endpoints
+ * (u,v) (index or -1-d for Dirichlet node d):
+ *   dg_e = -(lam_u - lam_v)(p_u - p_v),  grad_e = 4 g_e dg_e */
+void bn_edge_grad(int m, const int *eu, const int *ev, const double *g,
+                  const double *p, const double *lam, const double *pdir,
+                  double *dg, double *grad_logr, hipStream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,585 @@
+// kernels.hip - gfx950 kernels for the AMG-preconditioned PCG solve.
+//
+// Every kernel here is HBM-bandwidth or latency bound (CSR gathers,
+// triangular sweeps, BLAS-1): there is no dense contraction to put on
+// MFMA.  Design rules used throughout:
+//  * 256-thread blocks (4 wave64s); `LANES` consecutive lanes share one
+//    CSR row so the ja/val streams of a row are read as one contiguous
+//    segment, and the row sum is finished with DPP/xor shuffles.
+//  * Streaming kernels use a capped grid (<= BN_RED_BLOCKS blocks, a
+//    multiple of 8) whose logical block id is remapped so that each of the
+//    8 XCDs owns one contiguous slice of rows: x-gathers of neighbouring
+//    rows then hit the same XCD's L2.
+//  * Reductions are deterministic: per-block partials in fixed order,
+//    finished by one block in fixed order (no floating-point atomics).
+#include <hip/hip_runtime.h>
+#include <math.h>
+
+#include "bn_kernels.h"
+
+#define BLOCK 256
+
+namespace {
+
+__device__ __forceinline__ int logical_block() {
+  const int g = gridDim.x, b = blockIdx.x;
+  if ((g & 7) != 0) return b;
+  return (b & 7) * (g >> 3) + (b >> 3);
+}
+
+template <int W, typename T>
+__device__ __forceinline__ T group_sum(T v) {
+#pragma unroll
+  for (int o = W / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, W);
+  return v;
+}
+
+// block-wide sum of one double per thread; result valid in thread 0
+__device__ __forceinline__ double block_sum(double v) {
+  __shared__ double wsum[BLOCK / 64];
+  v = group_sum<64>(v);
+  const int lane = threadIdx.x & 63, wid = threadIdx.x >> 6;
+  if (lane == 0) wsum[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (threadIdx.x == 0) {
+#pragma unroll
+    for (int k = 0; k < BLOCK / 64; k++) t += wsum[k];
+  }
+  __syncthreads();
+  return t;
+}
+
+// Partial-sum buffers always hold BN_RED_BLOCKS slots; the finishing
+// kernel sums all of them, so logical block 0 zeroes the slots past the
+// grid (instead of a memset node, which is unsafe inside captured graphs).
+__device__ __forceinline__ void clear_tail(double *partial) {
+  for (int k = gridDim.x + threadIdx.x; k < BN_RED_BLOCKS; k += BLOCK) partial[k] = 0.0;
+}
+
+inline int grid_for(long long nrb) {
+  long long g = nrb;
+  if (g > BN_RED_BLOCKS) g = BN_RED_BLOCKS;
+  if (g >= 8) g &= ~7LL;
+  if (g < 1) g = 1;
+  return (int)g;
+}
+
+// ---------------------------------------------------------------------
+// CSR apply: y = A x | y = b - A x | y += A x, optional fused w.y dot
+// ---------------------------------------------------------------------
+template <typename T, int LANES, int OP, bool DOT>
+__global__ __launch_bounds__(BLOCK) void k_csr(int n, int chunk, const int *__restrict__ ia,
+                                               const int *__restrict__ ja, const T *__restrict__ val,
+                                               const T *__restrict__ x, T *__restrict__ y,
+                                               const T *__restrict__ b, const T *__restrict__ w,
+                                               double *__restrict__ partial) {
+  constexpr int RPB = BLOCK / LANES;
+  const int lb = logical_block();
+  const int sub = threadIdx.x % LANES;
+  const int nrb = (n + RPB - 1) / RPB;
+  const int rb0 = lb * chunk;
+  int rb1 = rb0 + chunk;
+  if (rb1 > nrb) rb1 = nrb;
+  double acc = 0.0;
+  for (int rb = rb0; rb < rb1; rb++) {
+    const int row = rb * RPB + threadIdx.x / LANES;
+    if (row < n) {
+      T s = 0;
+      const int e = ia[row + 1];
+      for (int q = ia[row] + sub; q < e; q += LANES) s += val[q] * x[ja[q]];
+      if (LANES > 1) s = group_sum<LANES>(s);
+      if (sub == 0) {
+        T out;
+        if (OP == BN_OP_SET) out = s;
+        else if (OP == BN_OP_RESID) out = b[row] - s;
+        else out = y[row] + s;
+        y[row] = out;
+        if (DOT) acc += (double)w[row] * (double)out;
+      }
+    }
+  }
+  if (DOT) {
+    double t = block_sum(acc);
+    if (threadIdx.x == 0) partial[lb] = t;
+    if (lb == 0) clear_tail(partial);
+  }
+}
+
+template <typename T, int LANES>
+void launch_csr(int op, int n, const int *ia, const int *ja, const T *val, const T *x, T *y,
+                const T *b, const T *w, double *partial, hipStream_t st) {
+  constexpr int RPB = BLOCK / LANES;
+  const long long nrb = ((long long)n + RPB - 1) / RPB;
+  const int g = grid_for(nrb);
+  const int chunk = (int)((nrb + g - 1) / g);
+  if (partial) {
+    // partial slots of blocks beyond the grid must read as zero
+    if (op == BN_OP_SET)
+      hipLaunchKernelGGL((k_csr<T, LANES, BN_OP_SET, true>), dim3(g), dim3(BLOCK), 0, st, n, chunk, ia, ja, val, x, y, b, w, partial);
+    else if (op == BN_OP_RESID)
+      hipLaunchKernelGGL((k_csr<T, LANES, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, st, n, chunk, ia, ja, val, x, y, b, w, partial);
+    else
+      hipLaunchKernelGGL((k_csr<T, LANES, BN_OP_ADD, true>), dim3(g), dim3(BLOCK), 0, st, n, chunk, ia, ja, val, x, y, b, w, partial);
+  } else {
+    if (op == BN_OP_SET)
+      hipLaunchKernelGGL((k_csr<T, LANES, BN_OP_SET, false>), dim3(g), dim3(BLOCK), 0, st, n, chunk, ia, ja, val, x, y, b, w, partial);
+    else if (op == BN_OP_RESID)
+      hipLaunchKernelGGL((k_csr<T, LANES, BN_OP_RESID, false>), dim3(g), dim3(BLOCK), 0, st, n, chunk, ia, ja, val, x, y, b, w, partial);
+    else
+      hipLaunchKernelGGL((k_csr<T, LANES, BN_OP_ADD, false>), dim3(g), dim3(BLOCK), 0, st, n, chunk, ia, ja, val, x, y, b, w, partial);
+  }
+}
+
+template <typename T>
+void dispatch_csr(int op, int n, const int *ia, const int *ja, const T *val, const T *x, T *y,
+                  const T *b, int lanes, const T *w, double *partial, hipStream_t st) {
+  switch (lanes) {
+    case 1: launch_csr<T, 1>(op, n, ia, ja, val, x, y, b, w, partial, st); break;
+    case 2: launch_csr<T, 2>(op, n, ia, ja, val, x, y, b, w, partial, st); break;
+    case 4: launch_csr<T, 4>(op, n, ia, ja, val, x, y, b, w, partial, st); break;
+    case 8: launch_csr<T, 8>(op, n, ia, ja, val, x, y, b, w, partial, st); break;
+    case 16: launch_csr<T, 16>(op, n, ia, ja, val, x, y, b, w, partial, st); break;
+    case 32: launch_csr<T, 32>(op, n, ia, ja, val, x, y, b, w, partial, st); break;
+    default: launch_csr<T, 64>(op, n, ia, ja, val, x, y, b, w, partial, st); break;
+  }
+}
+
+// ---------------------------------------------------------------------
+// Gauss-Seidel colour / triangular level: rows either a contiguous range
+// [lo, hi) or listed in rows[].  y_i = (b_i - sum_{j != i} a_ij y_j) dinv_i
+// ---------------------------------------------------------------------
+template <typename T, int LANES, bool LIST>
+__global__ __launch_bounds__(BLOCK) void k_rowsolve(int lo, int cnt, const int *__restrict__ rows,
+                                                    const int *__restrict__ ia, const int *__restrict__ ja,
+                                                    const T *__restrict__ val, const T *__restrict__ dinv,
+                                                    const T *__restrict__ b, T *__restrict__ y,
+                                                    const T *__restrict__ zbase, T *__restrict__ zsum) {
+  constexpr int RPB = BLOCK / LANES;
+  const int k = blockIdx.x * RPB + threadIdx.x / LANES;
+  const int sub = threadIdx.x % LANES;
+  if (k >= cnt) return;
+  const int i = LIST ? rows[k] : lo + k;
+  T s = 0;
+  const int e = ia[i + 1];
+  for (int q = ia[i] + sub; q < e; q += LANES) {
+    const int j = ja[q];
+    if (j != i) s += val[q] * y[j];
+  }
+  if (LANES > 1) s = group_sum<LANES>(s);
+  if (sub == 0) {
+    const T v = (b[i] - s) * dinv[i];
+    y[i] = v;
+    if (zsum) zsum[i] = zbase[i] + v;
+  }
+}
+
+template <typename T, bool LIST>
+void dispatch_rowsolve(int lo, int cnt, const int *rows, const int *ia, const int *ja, const T *val,
+                       const T *dinv, const T *b, T *y, int lanes, const T *zb, T *zs, hipStream_t st) {
+  if (cnt <= 0) return;
+#define BN_RS(L)                                                                                   \
+  hipLaunchKernelGGL((k_rowsolve<T, L, LIST>), dim3((cnt + BLOCK / L - 1) / (BLOCK / L)), dim3(BLOCK), \
+                     0, st, lo, cnt, rows, ia, ja, val, dinv, b, y, zb, zs)
+  switch (lanes) {
+    case 1: BN_RS(1); break;
+    case 2: BN_RS(2); break;
+    case 4: BN_RS(4); break;
+    case 8: BN_RS(8); break;
+    case 16: BN_RS(16); break;
+    case 32: BN_RS(32); break;
+    default: BN_RS(64); break;
+  }
+#undef BN_RS
+}
+
+template <typename T, int LANES>
+__global__ __launch_bounds__(BLOCK) void k_jacobi_rows(int n, const int *__restrict__ ia,
+                                                       const int *__restrict__ ja, const T *__restrict__ val,
+                                                       const T *__restrict__ dinv, const T *__restrict__ b,
+                                                       const T *__restrict__ xin, T *__restrict__ xout) {
+  constexpr int RPB = BLOCK / LANES;
+  const int i = blockIdx.x * RPB + threadIdx.x / LANES;
+  const int sub = threadIdx.x % LANES;
+  if (i >= n) return;
+  T s = 0;
+  const int e = ia[i + 1];
+  for (int q = ia[i] + sub; q < e; q += LANES) {
+    const int j = ja[q];
+    if (j != i) s += val[q] * xin[j];
+  }
+  if (LANES > 1) s = group_sum<LANES>(s);
+  if (sub == 0) xout[i] = (b[i] - s) * dinv[i];
+}
+
+template <typename T>
+void dispatch_jacobi_rows(int n, const int *ia, const int *ja, const T *val, const T *dinv, const T *b,
+                          const T *xin, T *xout, int lanes, hipStream_t st) {
+#define BN_JR(L)                                                                                       \
+  hipLaunchKernelGGL((k_jacobi_rows<T, L>), dim3((n + BLOCK / L - 1) / (BLOCK / L)), dim3(BLOCK), 0, st, \
+                     n, ia, ja, val, dinv, b, xin, xout)
+  switch (lanes) {
+    case 1: BN_JR(1); break;
+    case 2: BN_JR(2); break;
+    case 4: BN_JR(4); break;
+    case 8: BN_JR(8); break;
+    case 16: BN_JR(16); break;
+    case 32: BN_JR(32); break;
+    default: BN_JR(64); break;
+  }
+#undef BN_JR
+}
+
+template <typename T>
+__global__ void k_diag_scale(int lo, int cnt, const T *__restrict__ dinv, const T *__restrict__ b,
+                             T *__restrict__ x) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k < cnt) x[lo + k] = b[lo + k] * dinv[lo + k];
+}
+
+template <typename T>
+__global__ void k_jacobi_update(int n, const T *__restrict__ r, const T *__restrict__ d, T omega,
+                                T *__restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] += omega * r[i] * d[i];
+}
+
+// ---------------------------------------------------------------------
+// dense y = M x: one block per row, coalesced row read
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_gemv(int n, const T *__restrict__ M, const T *__restrict__ x,
+                                                T *__restrict__ y) {
+  const int row = blockIdx.x;
+  const T *m = M + (size_t)row * (size_t)n;
+  double s = 0.0;
+  for (int j = threadIdx.x; j < n; j += BLOCK) s += (double)m[j] * (double)x[j];
+  s = block_sum(s);
+  if (threadIdx.x == 0) y[row] = (T)s;
+}
+
+// ---------------------------------------------------------------------
+// BLAS-1
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ void k_fill(int n, T *x, T v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
+}
+template <typename T>
+__global__ void k_copy(int n, const T *__restrict__ x, T *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = x[i];
+}
+template <typename A, typename B>
+__global__ void k_convert(int n, const A *__restrict__ x, B *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] = (B)x[i];
+}
+template <typename T>
+__global__ void k_axpy1(int n, const T *__restrict__ x, T *__restrict__ y) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) y[i] += x[i];
+}
+
+__global__ __launch_bounds__(BLOCK) void k_dot2(int n, int chunk, const double *__restrict__ x,
+                                                const double *__restrict__ y, const double *__restrict__ u,
+                                                const double *__restrict__ v, double *__restrict__ p1,
+                                                double *__restrict__ p2) {
+  const int lb = logical_block();
+  const long long i0 = (long long)lb * chunk * BLOCK;
+  long long i1 = i0 + (long long)chunk * BLOCK;
+  if (i1 > n) i1 = n;
+  double a = 0.0, c = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+    a += x[i] * y[i];
+    if (u) c += u[i] * v[i];
+  }
+  a = block_sum(a);
+  if (u) c = block_sum(c);
+  if (threadIdx.x == 0) {
+    p1[lb] = a;
+    if (u) p2[lb] = c;
+  }
+  if (lb == 0) {
+    clear_tail(p1);
+    if (u) clear_tail(p2);
+  }
+}
+
+__global__ __launch_bounds__(BLOCK) void k_update_xr(int n, int chunk, double *__restrict__ x,
+                                                     const double *__restrict__ p, double *__restrict__ r,
+                                                     const double *__restrict__ q, const double *__restrict__ s,
+                                                     double *__restrict__ part) {
+  const int lb = logical_block();
+  const double alpha = s[BN_S_ALPHA];
+  const long long i0 = (long long)lb * chunk * BLOCK;
+  long long i1 = i0 + (long long)chunk * BLOCK;
+  if (i1 > n) i1 = n;
+  double a = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+    x[i] = x[i] + p[i] * alpha;
+    const double rn = r[i] + q[i] * (-alpha);
+    r[i] = rn;
+    a += rn * rn;
+  }
+  a = block_sum(a);
+  if (threadIdx.x == 0) part[lb] = a;
+  if (lb == 0) clear_tail(part);
+}
+
+__global__ void k_update_p(int n, double *__restrict__ p, const double *__restrict__ z,
+                           const double *__restrict__ s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double beta = s[BN_S_BETA];
+  if (i < n) p[i] = beta * p[i] + z[i];
+}
+
+// fixed-order finish of BN_RED_BLOCKS partials by one block
+__device__ double finish_sum(const double *p) {
+  double v = 0.0;
+  for (int k = threadIdx.x; k < BN_RED_BLOCKS; k += BLOCK) v += p[k];
+  return block_sum(v);
+}
+
+__global__ __launch_bounds__(BLOCK) void k_scalar_finish(int which, const double *p1, const double *p2,
+                                                         const double *p3, double *s, int slot) {
+  const double a = p1 ? finish_sum(p1) : 0.0;
+  const double b = p2 ? finish_sum(p2) : 0.0;
+  const double c = p3 ? finish_sum(p3) : 0.0;
+  if (threadIdx.x != 0) return;
+  switch (which) {
+    case 0:
+      s[BN_S_RZ] = a;
+      s[BN_S_RZ0] = a;
+      s[BN_S_RR] = b;
+      s[BN_S_REL] = 1.0;
+      break;
+    case 1:
+      s[BN_S_PAP] = a;
+      s[BN_S_ALPHA] = s[BN_S_RZ] / a;
+      break;
+    case 2:
+    case 4: {
+      s[BN_S_RZNEW] = a;
+      s[BN_S_RR] = b;
+      s[BN_S_REL] = sqrt(a / s[BN_S_RZ0]);
+      s[BN_S_BETA] = (which == 4) ? -s[BN_S_ALPHA] * c / s[BN_S_RZ] : a / s[BN_S_RZ];
+      s[BN_S_RZOLD] = s[BN_S_RZ];
+      s[BN_S_RZ] = a;
+      break;
+    }
+    case 3:
+      s[BN_S_BB] = a;
+      break;
+    default:
+      s[slot] = a;
+      break;
+  }
+}
+
+__global__ void k_permute(int n, const int *__restrict__ perm, const double *__restrict__ x,
+                          double *__restrict__ y, int scatter) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= n) return;
+  if (scatter) y[perm[k]] = x[k];
+  else y[k] = x[perm[k]];
+}
+
+__global__ void k_edge_grad(int m, const int *__restrict__ eu, const int *__restrict__ ev,
+                            const double *__restrict__ g, const double *__restrict__ p,
+                            const double *__restrict__ lam, const double *__restrict__ pdir,
+                            double *__restrict__ dg, double *__restrict__ gr) {
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= m) return;
+  const int u = eu[e], v = ev[e];
+  const double pu = u >= 0 ? p[u] : pdir[-1 - u];
+  const double pv = v >= 0 ? p[v] : pdir[-1 - v];
+  const double lu = u >= 0 ? lam[u] : 0.0;
+  const double lv = v >= 0 ? lam[v] : 0.0;
+  const double d = -(lu - lv) * (pu - pv);
+  if (dg) dg[e] = d;
+  if (gr) gr[e] = 4.0 * g[e] * d;
+}
+
+inline dim3 g1(long long n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
+
+}  // namespace
+
+// =====================================================================
+// C launchers
+// =====================================================================
+extern "C" {
+
+void bn_csr_apply(char vt, int op, int n, const int *ia, const int *ja, const void *val, const void *x,
+                  void *y, const void *b, int lanes, const void *dotw, double *partial,
+                  hipStream_t stream) {
+  if (n <= 0) {
+    if (partial) bn_fill('d', BN_RED_BLOCKS, partial, 0.0, stream);
+    return;
+  }
+  if (vt == 's')
+    dispatch_csr<float>(op, n, ia, ja, (const float *)val, (const float *)x, (float *)y,
+                        (const float *)b, lanes, (const float *)dotw, partial, stream);
+  else
+    dispatch_csr<double>(op, n, ia, ja, (const double *)val, (const double *)x, (double *)y,
+                         (const double *)b, lanes, (const double *)dotw, partial, stream);
+}
+
+void bn_gs_color(char vt, int lo, int hi, const int *ia, const int *ja, const void *val,
+                 const void *dinv, const void *b, void *x, int lanes, const void *zbase, void *zsum,
+                 hipStream_t stream) {
+  if (vt == 's')
+    dispatch_rowsolve<float, false>(lo, hi - lo, nullptr, ia, ja, (const float *)val, (const float *)dinv,
+                                    (const float *)b, (float *)x, lanes, (const float *)zbase,
+                                    (float *)zsum, stream);
+  else
+    dispatch_rowsolve<double, false>(lo, hi - lo, nullptr, ia, ja, (const double *)val,
+                                     (const double *)dinv, (const double *)b, (double *)x, lanes,
+                                     (const double *)zbase, (double *)zsum, stream);
+}
+
+void bn_gs_color_zero(char vt, int lo, int hi, const void *dinv, const void *b, void *x,
+                      hipStream_t stream) {
+  const int cnt = hi - lo;
+  if (cnt <= 0) return;
+  if (vt == 's')
+    hipLaunchKernelGGL(k_diag_scale<float>, g1(cnt), dim3(BLOCK), 0, stream, lo, cnt, (const float *)dinv,
+                       (const float *)b, (float *)x);
+  else
+    hipLaunchKernelGGL(k_diag_scale<double>, g1(cnt), dim3(BLOCK), 0, stream, lo, cnt,
+                       (const double *)dinv, (const double *)b, (double *)x);
+}
+
+void bn_jacobi_rows(char vt, int n, const int *ia, const int *ja, const void *val, const void *dinv,
+                    const void *b, const void *xin, void *xout, int lanes, hipStream_t stream) {
+  if (n <= 0) return;
+  if (!xin) {
+    bn_gs_color_zero(vt, 0, n, dinv, b, xout, stream);
+    return;
+  }
+  if (vt == 's')
+    dispatch_jacobi_rows<float>(n, ia, ja, (const float *)val, (const float *)dinv, (const float *)b,
+                                (const float *)xin, (float *)xout, lanes, stream);
+  else
+    dispatch_jacobi_rows<double>(n, ia, ja, (const double *)val, (const double *)dinv, (const double *)b,
+                                 (const double *)xin, (double *)xout, lanes, stream);
+}
+
+void bn_jacobi_update(char vt, int n, const void *r, const void *dscale, double omega, void *x,
+                      hipStream_t stream) {
+  if (n <= 0) return;
+  if (vt == 's')
+    hipLaunchKernelGGL(k_jacobi_update<float>, g1(n), dim3(BLOCK), 0, stream, n, (const float *)r,
+                       (const float *)dscale, (float)omega, (float *)x);
+  else
+    hipLaunchKernelGGL(k_jacobi_update<double>, g1(n), dim3(BLOCK), 0, stream, n, (const double *)r,
+                       (const double *)dscale, omega, (double *)x);
+}
+
+void bn_trsv_level(char vt, int cnt, const int *rows, const int *ia, const int *ja, const void *val,
+                   const void *dinv, const void *b, void *y, int lanes, const void *zbase, void *zsum,
+                   hipStream_t stream) {
+  if (vt == 's')
+    dispatch_rowsolve<float, true>(0, cnt, rows, ia, ja, (const float *)val, (const float *)dinv,
+                                   (const float *)b, (float *)y, lanes, (const float *)zbase,
+                                   (float *)zsum, stream);
+  else
+    dispatch_rowsolve<double, true>(0, cnt, rows, ia, ja, (const double *)val, (const double *)dinv,
+                                    (const double *)b, (double *)y, lanes, (const double *)zbase,
+                                    (double *)zsum, stream);
+}
+
+void bn_dense_gemv(char vt, int n, const void *M, const void *x, void *y, hipStream_t stream) {
+  if (n <= 0) return;
+  if (vt == 's')
+    hipLaunchKernelGGL(k_gemv<float>, dim3(n), dim3(BLOCK), 0, stream, n, (const float *)M,
+                       (const float *)x, (float *)y);
+  else
+    hipLaunchKernelGGL(k_gemv<double>, dim3(n), dim3(BLOCK), 0, stream, n, (const double *)M,
+                       (const double *)x, (double *)y);
+}
+
+void bn_fill(char vt, int n, void *x, double v, hipStream_t stream) {
+  if (n <= 0) return;
+  if (vt == 's')
+    hipLaunchKernelGGL(k_fill<float>, g1(n), dim3(BLOCK), 0, stream, n, (float *)x, (float)v);
+  else
+    hipLaunchKernelGGL(k_fill<double>, g1(n), dim3(BLOCK), 0, stream, n, (double *)x, v);
+}
+
+void bn_copy(char vt, int n, const void *x, void *y, hipStream_t stream) {
+  if (n <= 0) return;
+  if (vt == 's')
+    hipLaunchKernelGGL(k_copy<float>, g1(n), dim3(BLOCK), 0, stream, n, (const float *)x, (float *)y);
+  else
+    hipLaunchKernelGGL(k_copy<double>, g1(n), dim3(BLOCK), 0, stream, n, (const double *)x, (double *)y);
+}
+
+void bn_convert(char vt_in, char vt_out, int n, const void *x, void *y, hipStream_t stream) {
+  if (n <= 0) return;
+  if (vt_in == 'd' && vt_out == 's')
+    hipLaunchKernelGGL((k_convert<double, float>), g1(n), dim3(BLOCK), 0, stream, n, (const double *)x,
+                       (float *)y);
+  else if (vt_in == 's' && vt_out == 'd')
+    hipLaunchKernelGGL((k_convert<float, double>), g1(n), dim3(BLOCK), 0, stream, n, (const float *)x,
+                       (double *)y);
+  else
+    bn_copy(vt_in, n, x, y, stream);
+}
+
+void bn_axpy1(char vt, int n, const void *x, void *y, hipStream_t stream) {
+  if (n <= 0) return;
+  if (vt == 's')
+    hipLaunchKernelGGL(k_axpy1<float>, g1(n), dim3(BLOCK), 0, stream, n, (const float *)x, (float *)y);
+  else
+    hipLaunchKernelGGL(k_axpy1<double>, g1(n), dim3(BLOCK), 0, stream, n, (const double *)x, (double *)y);
+}
+
+void bn_dot_partial(int n, const double *x, const double *y, const double *u, const double *v,
+                    double *partial, double *partial2, hipStream_t stream) {
+  if (n <= 0) {
+    bn_fill('d', BN_RED_BLOCKS, partial, 0.0, stream);
+    if (u) bn_fill('d', BN_RED_BLOCKS, partial2, 0.0, stream);
+    return;
+  }
+  const long long nb = ((long long)n + BLOCK - 1) / BLOCK;
+  const int g = grid_for(nb);
+  const int chunk = (int)((nb + g - 1) / g);
+  hipLaunchKernelGGL(k_dot2, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, y, u, v, partial, partial2);
+}
+
+void bn_pcg_update_xr(int n, double *x, const double *p, double *r, const double *q, const double *s,
+                      double *partial, hipStream_t stream) {
+  if (n <= 0) {
+    bn_fill('d', BN_RED_BLOCKS, partial, 0.0, stream);
+    return;
+  }
+  const long long nb = ((long long)n + BLOCK - 1) / BLOCK;
+  const int g = grid_for(nb);
+  const int chunk = (int)((nb + g - 1) / g);
+  hipLaunchKernelGGL(k_update_xr, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, p, r, q, s, partial);
+}
+
+void bn_pcg_update_p(int n, double *p, const double *z, const double *s, hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_update_p, g1(n), dim3(BLOCK), 0, stream, n, p, z, s);
+}
+
+void bn_scalar_finish(int which, const double *p1, const double *p2, const double *p3, double *s, int slot,
+                      hipStream_t stream) {
+  hipLaunchKernelGGL(k_scalar_finish, dim3(1), dim3(BLOCK), 0, stream, which, p1, p2, p3, s, slot);
+}
+
+void bn_permute(int n, const int *perm, const double *x, double *y, int scatter, hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_permute, g1(n), dim3(BLOCK), 0, stream, n, perm, x, y, scatter);
+}
+
+void bn_edge_grad(int m, const int *eu, const int *ev, const double *g, const double *p, const double *lam,
+                  const double *pdir, double *dg, double *grad_logr, hipStream_t stream) {
+  if (m <= 0) return;
+  hipLaunchKernelGGL(k_edge_grad, g1(m), dim3(BLOCK), 0, stream, m, eu, ev, g, p, lam, pdir, dg, grad_logr);
+}
+
+}  // extern "C"

@@ -1,0 +1,107 @@
+"""Synthetic vessel networks (wrapper over bn_netgen_* in libbnamg.so).
+
+``make_network(L, depth, n_trees=1, seed=0)`` returns a ``VesselNetwork``
+with the SPD pressure system A p = b (Dirichlet roots eliminated) and the
+per-segment data the adjoint needs.  The named workloads below are the
+four sizes the benchmarks and parity tests use.
+"""
+import ctypes as C
+import hashlib
+
+import numpy as np
+
+from . import _lib as L
+
+__all__ = ["VesselNetwork", "make_network", "WORKLOADS"]
+
+# name -> (lattice edge, tree depth, trees per kind, coarsening)
+WORKLOADS = {
+    "lattice16_trees9": (16, 9, 1, "rs"),        # 5,116 unknowns
+    "lattice100_trees15": (100, 15, 1, "pmis"),  # 1,065,532 unknowns
+    "lattice160_trees17": (160, 17, 1, "hmis"),  # 4,358,140 unknowns
+    "lattice100_16x16trees11": (100, 11, 16, "pmis"),  # 1,065,472 unknowns
+}
+
+
+class VesselNetwork:
+    def __init__(self, net, params):
+        self.params = params
+        self.n = net.n
+        self.nnz = net.nnz
+        self.n_edges = net.n_edges
+        self.n_nodes = net.n_nodes
+        self.n_dirichlet = net.n_dirichlet
+        arr = np.ctypeslib.as_array
+        self.ia = arr(net.ia, (net.n + 1,)).copy()
+        self.ja = arr(net.ja, (net.nnz,)).copy()
+        self.val = arr(net.val, (net.nnz,)).copy()
+        self.b = arr(net.b, (net.n,)).copy()
+        m = net.n_edges
+        self.eu = arr(net.eu, (m,)).copy()
+        self.ev = arr(net.ev, (m,)).copy()
+        self.radius = arr(net.radius, (m,)).copy()
+        self.length = arr(net.length, (m,)).copy()
+        self.g = arr(net.g, (m,)).copy()
+        self.pos_uv = arr(net.pos_uv, (m,)).copy()
+        self.pos_vu = arr(net.pos_vu, (m,)).copy()
+        self.diag_pos = arr(net.diag_pos, (net.n,)).copy()
+        self.p_dirichlet = arr(net.p_dirichlet, (net.n_dirichlet,)).copy()
+        self.dirichlet_is_art = arr(net.dirichlet_is_art, (net.n_dirichlet,)).copy()
+        self.mu = params.mu
+
+    def conductance(self, radius):
+        """g = pi r^4 / (8 mu len) per segment."""
+        r = np.asarray(radius, dtype=np.float64)
+        return np.pi / (8.0 * self.mu) * (r * r) * (r * r) / self.length
+
+    def assemble(self, g, p_dir=None):
+        """(val, b) of A and b for segment conductances g on the fixed pattern."""
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        val = np.zeros(self.nnz)
+        b = np.zeros(self.n)
+        net = self._struct()
+        pd = None if p_dir is None else np.ascontiguousarray(p_dir, dtype=np.float64)
+        L.lib.bn_netgen_assemble(C.byref(net), L.dp(g), L.dp(pd) if pd is not None else None,
+                                 L.dp(val), L.dp(b))
+        return val, b
+
+    def rhs_for(self, p_dir):
+        """b for other Dirichlet pressures (A does not change)."""
+        return self.assemble(self.g, p_dir)[1]
+
+    def sha256(self):
+        h = hashlib.sha256()
+        for a in (self.ia, self.ja, self.val, self.b):
+            h.update(np.ascontiguousarray(a).tobytes())
+        return h.hexdigest()
+
+    def _struct(self):
+        # a bn_network view over this object's arrays (assemble only reads)
+        s = L.Network()
+        s.n_nodes, s.n, s.n_edges, s.nnz, s.n_dirichlet = (self.n_nodes, self.n, self.n_edges,
+                                                           self.nnz, self.n_dirichlet)
+        s.ia, s.ja, s.val, s.b = L.ip(self.ia), L.ip(self.ja), L.dp(self.val), L.dp(self.b)
+        s.eu, s.ev = L.ip(self.eu), L.ip(self.ev)
+        s.radius, s.length, s.g = L.dp(self.radius), L.dp(self.length), L.dp(self.g)
+        s.pos_uv, s.pos_vu, s.diag_pos = L.ip(self.pos_uv), L.ip(self.pos_vu), L.ip(self.diag_pos)
+        s.p_dirichlet = L.dp(self.p_dirichlet)
+        s.dirichlet_is_art = L.ip(self.dirichlet_is_art)
+        s.node_unknown = None
+        self._keep = s
+        return s
+
+
+def make_network(L_edge, depth, n_trees=1, seed=0, **overrides):
+    p = L.NetParams()
+    L.lib.bn_netgen_default_params(C.byref(p))
+    p.L, p.depth, p.n_trees, p.seed = L_edge, depth, n_trees, seed
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    net = L.Network()
+    rc = L.lib.bn_netgen_build(C.byref(p), C.byref(net))
+    if rc != 0:
+        raise ValueError(f"bad network parameters L={L_edge} depth={depth} n_trees={n_trees}")
+    try:
+        return VesselNetwork(net, p)
+    finally:
+        L.lib.bn_netgen_free(C.byref(net))

@@ -1,0 +1,127 @@
+/*
+ * bn_amg_ext.h - extensions of libbnamg.so beyond the C_amg1 interface.
+ * None of these has a counterpart in the reference; they add what a GPU
+ * library needs at the boundary: explicit options (coarsening, V-cycle
+ * precision, coarsest solver), device-pointer solves, a solve report,
+ * hierarchy introspection for parity checks, the synthetic network
+ * generator, the adjoint gradient and batched solves.
+ */
+#ifndef BN_AMG_EXT_H
+#define BN_AMG_EXT_H
+
+#include <stdint.h>
+
+#include "amg.h"
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* error codes (negative, never a valid iteration count) */
+#define BN_ERR_NO_DEVICE (-1000001)
+#define BN_ERR_HIP (-1000002)
+#define BN_ERR_ARG (-1000003)
+
+/* V-cycle storage/arithmetic precision */
+#define BN_PREC_FP64 0
+#define BN_PREC_FP32 1
+
+/* coarsest-level solver */
+#define BN_COARSE_AUTO 0   /* dense when n_c <= BN_DENSE_MAX, else IC-PCG */
+#define BN_COARSE_DENSE 1  /* explicit inverse of the coarsest operator   */
+#define BN_COARSE_ICPCG 2  /* reference: IC-PCG to 1e-12, <= n_c steps    */
+#define BN_DENSE_MAX 8192
+
+/* fine-level IC(0) correction inside the preconditioner */
+#define BN_IC_EXACT 0   /* level-scheduled forward/backward solves          */
+#define BN_IC_JACOBI 1  /* ic_sweeps Jacobi sweeps per triangle (flexible)   */
+
+/* outer stopping test */
+#define BN_STOP_BNORM 0  /* sqrt(r'z / r0'z0) < tol (reference, amg_pcg.c:75) */
+#define BN_STOP_RELRES 1 /* ||r||_2 / ||b||_2 < tol                          */
+
+typedef struct {
+  int coarsening;     /* COARSEN_RS / COARSEN_PMIS / COARSEN_HMIS          */
+  double threshold;   /* strength threshold theta                         */
+  int min_size;       /* coarsen while n >= min_size                      */
+  int max_levels;     /* <= AMG_MAX_LEVELS                                */
+  int nu;             /* smoothing sweeps per half cycle                  */
+  int smoother;       /* SMOOTHER_*                                       */
+  double omega;       /* damped-Jacobi weight                             */
+  int precision;      /* BN_PREC_*                                        */
+  int coarse_solver;  /* BN_COARSE_*                                      */
+  int fine_ic;        /* BN_IC_*                                          */
+  int ic_sweeps;      /* sweeps per triangle for BN_IC_JACOBI             */
+  int max_colours;    /* colours above which a level smooths l1-Jacobi    */
+  int flexible;       /* 1: flexible (Polak-Ribiere) beta                 */
+  int stop;           /* BN_STOP_*                                        */
+  int symmetrize;     /* 1: use (A + A^T)/2 as the reference wrapper does */
+  int device;         /* HIP device ordinal, -1 = current                 */
+  int verbose;        /* 1: hierarchy report on stderr                    */
+} bn_options;
+
+typedef struct {
+  int iterations;     /* iteration count, or -maxiter                      */
+  double rel_bnorm;   /* final sqrt(r'z / r0'z0)                           */
+  double relres;      /* final ||r|| / ||b|| from the recurrence           */
+  double solve_ms;    /* device time from first to last kernel (hipEvent)  */
+  double wall_ms;     /* host wall time of the call                        */
+  double setup_ms;    /* host hierarchy + IC set-up                        */
+  double upload_ms;   /* host analysis + device upload                     */
+  int num_levels;
+} bn_solve_info;
+
+void bn_options_default(bn_options *o);
+
+/* Create with explicit options (amg_solver_create = these defaults with
+ * PMIS, CF-GS, FP64, symmetrise).  Returns NULL on bad arguments. */
+void *bn_solver_create(int n, int nnz, const int *ia, const int *ja, const double *val,
+                       const bn_options *opt);
+/* Solve with device pointers (level-0 numbering = caller numbering);
+ * runs on the solver's stream, synchronous on return. */
+int bn_solver_solve_device(void *h, const double *d_b, double *d_x, double tol, int maxiter,
+                           int print_level, bn_solve_info *info);
+/* Solve with host pointers, also filling `info`. */
+int bn_solver_solve_info(void *h, const double *b, double *x, double tol, int maxiter,
+                         int print_level, bn_solve_info *info);
+/* One preconditioner application z = B g (host pointers); which = 0
+ * AMG+IC (amg_ichol_precond), 1 plain V-cycle (amg_vcycle_precond). */
+int bn_solver_precond(void *h, const double *g, double *z, int which);
+/* Device timing of the last solve etc. */
+int bn_solver_last_info(void *h, bn_solve_info *info);
+/* Change the V-cycle precision / IC mode / stop rule after creation. */
+int bn_solver_set_mode(void *h, int precision, int fine_ic, int ic_sweeps, int flexible, int stop);
+
+/* Hierarchy introspection (host copies, reference layout).
+ * dims[0..7] = n, nnz(A), n_coarse, nnz(P), nnz(S), nnz(coarsest L),
+ *              nnz(fine L), has_split;  what: 0 A, 1 P, 2 S */
+int bn_solver_level_dims(void *h, int lev, int *dims);
+int bn_solver_level_csr(void *h, int lev, int what, int *ia, int *ja, double *val);
+int bn_solver_level_split(void *h, int lev, int *mis, int *cf_order, double *l1);
+int bn_solver_ichol(void *h, int which, int *ia, int *ja, double *val);
+/* device schedule: colours / levels used on each level (for reports) */
+int bn_solver_schedule(void *h, int lev, int *ncolours, int *use_gs, int *lanes);
+int bn_solver_ic_levels(void *h, int *fwd, int *bwd);
+
+/* 1 if a HIP device is usable */
+int bn_device_available(void);
+/* message of the last error on this thread */
+const char *bn_last_error(void);
+
+/* ---- adjoint: lambda = A^{-1} dLdp with the same hierarchy, then the
+ * edge-scatter gradient dL/dg_e = -(lam_u - lam_v)(p_u - p_v) and
+ * dL/dlog r_e = 4 g_e dL/dg_e.  Host pointers; m segments with endpoints
+ * eu/ev (unknown index or -1-d for Dirichlet node d, pressure pdir[d]). */
+int bn_solver_adjoint_grad(void *h, const double *p, const double *dLdp, int m, const int *eu,
+                           const int *ev, const double *g, const double *pdir, double tol,
+                           int maxiter, double *lam, double *dLdg, double *dLdlogr);
+
+/* ---- batched solves: k right-hand sides stored column-major (b[j*n+i]),
+ * each solved to tol; iterations per column into its[k]. */
+int bn_solver_solve_batch(void *h, int k, const double *b, double *x, double tol, int maxiter,
+                          int *its, bn_solve_info *info);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,74 @@
+/*
+ * netgen.h - deterministic synthetic vessel-network generator.
+ *
+ * A vessel network is a 3-D capillary lattice (L^3 nodes, 6-neighbour
+ * segments) plus binary arterial and venous trees whose leaves attach to
+ * distinct random lattice nodes.  Every segment carries a Poiseuille
+ * conductance g = pi r^4 / (8 mu len).  The tree roots are Dirichlet nodes;
+ * eliminating them leaves the SPD weighted graph Laplacian
+ *     A = C_I^T W C_I,   b = -C_I^T W C_B p_B
+ * which is the same algebra as Matlab_amg/read_nwk.m:23-37 (there C is the
+ * face-node incidence, W = diag(g)).  Unknowns are the non-Dirichlet nodes
+ * in ascending original id; CSR rows are sorted by column.
+ */
+#ifndef BN_NETGEN_H
+#define BN_NETGEN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct {
+  int      L;          /* lattice edge (nodes per axis)                    */
+  int      depth;      /* binary-tree depth: 2^depth-1 nodes per tree      */
+  int      n_trees;    /* trees per kind (arterial and venous each)        */
+  uint64_t seed;       /* master seed; every random quantity has a stream  */
+  double   p_art;      /* arterial root pressure [Pa]                      */
+  double   p_ven;      /* venous root pressure [Pa]                        */
+  double   mu;         /* viscosity [Pa s]                                 */
+  double   spacing;    /* lattice segment length [m]                       */
+  double   cap_r;      /* capillary median radius [m]                      */
+  double   cap_sigma;  /* lognormal sigma of capillary radius              */
+  double   root_r;     /* tree root radius [m]                             */
+} bn_netgen_params;
+
+typedef struct {
+  int n_nodes;      /* all nodes, Dirichlet included                      */
+  int n;            /* unknowns                                           */
+  int n_edges;      /* segments                                           */
+  int nnz;          /* nnz of A                                           */
+  int n_dirichlet;  /* Dirichlet (root) nodes                             */
+  /* CSR of A (rows sorted), and b for the stored Dirichlet pressures */
+  int    *ia, *ja;
+  double *val, *b;
+  /* per segment: endpoints as unknown index, or -1-d for Dirichlet node d */
+  int    *eu, *ev;
+  double *radius, *length, *g;
+  /* position in val[] of the (eu,ev) and (ev,eu) entries, -1 if an
+   * endpoint is Dirichlet; diag position of each unknown */
+  int    *pos_uv, *pos_vu, *diag_pos;
+  double *p_dirichlet;   /* pressure of each Dirichlet node               */
+  int    *dirichlet_is_art; /* 1 arterial root, 0 venous root            */
+  int    *node_unknown;  /* original node id -> unknown index or -1       */
+} bn_network;
+
+void bn_netgen_default_params(bn_netgen_params *p);
+/* Builds the network.  Returns 0, or -1 on bad parameters (more leaves than
+ * lattice nodes, L < 2, depth < 2) or allocation failure. */
+int  bn_netgen_build(const bn_netgen_params *p, bn_network *net);
+/* Re-assembles val[] and b[] from per-segment conductances g[] on the same
+ * pattern (g may alias net->g).  p_dir overrides the Dirichlet pressures if
+ * non-NULL. */
+void bn_netgen_assemble(const bn_network *net, const double *g,
+                        const double *p_dir, double *val, double *b);
+/* g = pi r^4 / (8 mu len) for every segment */
+void bn_netgen_conductance(const bn_network *net, const double *radius,
+                           double mu, double *g);
+void bn_netgen_free(bn_network *net);
+
+#ifdef __cplusplus
+}
+#endif
+#endif
