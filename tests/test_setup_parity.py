@@ -1,0 +1,88 @@
+"""The product's host set-up (libbnamg.so, OpenMP) against the oracle:
+C/F splits, S, P, Galerkin operators, l1 norms and both IC(0) factors bit
+for bit.  Runs without a GPU: hierarchy construction is host code and the
+solver handle defers the device upload when no device is present."""
+import numpy as np
+import pytest
+
+from blood_network_amd import AMGSolver, make_network
+from test_oracle_pins import random_graph_laplacian
+
+
+def compare(s, h):
+    assert s.num_levels() == h.num_levels()
+    for lev in range(s.num_levels()):
+        a, o = s.level(lev), h.level(lev)
+        assert a.keys() == o.keys(), lev
+        for k in o:
+            if isinstance(o[k], tuple):
+                for p, q in zip(a[k], o[k]):
+                    assert np.array_equal(p, q), (lev, k)
+            else:
+                assert np.array_equal(a[k], o[k]), (lev, k)
+    for p, q in zip(s.ichol(0), h.fine_ichol()):
+        assert np.array_equal(p, q)
+
+
+@pytest.mark.parametrize("coarsening", ["rs", "pmis", "hmis"])
+def test_setup_bit_exact_lattice16(net16, oracle, coarsening):
+    s = AMGSolver(net16.ia, net16.ja, net16.val, coarsening=coarsening)
+    compare(s, oracle.build(net16.ia, net16.ja, net16.val, coarsening=coarsening))
+
+
+@pytest.mark.parametrize("coarsening", ["rs", "pmis", "hmis"])
+def test_setup_bit_exact_lattice30_deep(net30, oracle, coarsening):
+    s = AMGSolver(net30.ia, net30.ja, net30.val, coarsening=coarsening, min_size=100)
+    h = oracle.build(net30.ia, net30.ja, net30.val, coarsening=coarsening, min_size=100)
+    assert s.num_levels() >= 4
+    compare(s, h)
+
+
+@pytest.mark.parametrize("seed", [0, 1, 2])
+def test_setup_bit_exact_random_graphs(oracle, seed):
+    A = random_graph_laplacian(700, 6, seed=seed)
+    for c in ("rs", "pmis", "hmis"):
+        s = AMGSolver(A.indptr, A.indices, A.data, coarsening=c, min_size=50)
+        compare(s, oracle.build(A.indptr, A.indices, A.data, coarsening=c, min_size=50))
+
+
+def test_update_rebuilds_values_bit_exact(oracle):
+    net = make_network(14, 7, seed=9)
+    s = AMGSolver(net.ia, net.ja, net.val, coarsening="pmis", min_size=300)
+    h = oracle.build(net.ia, net.ja, net.val, coarsening="pmis", min_size=300)
+    g2 = net.g * np.exp(np.random.default_rng(2).normal(0, 0.2, net.n_edges))
+    v2, _ = net.assemble(g2)
+    s.update(net.ia, net.ja, v2)
+    h.update(v2)
+    compare(s, h)
+
+
+def test_wrapper_defaults_match_reference_wrapper(net16, oracle):
+    # amg_solver_create hard-codes PMIS + CF-GS (C_amg1/src/amg_wrapper.c:80-89)
+    s = AMGSolver(net16.ia, net16.ja, net16.val, 0.25, 1000, 20, 1)
+    compare(s, oracle.build(net16.ia, net16.ja, net16.val, coarsening="pmis", min_size=1000))
+    assert s.size() == net16.n
+
+
+def test_stalled_and_single_level(oracle):
+    # min_size > n: one level, coarsest = fine (build_hierarchy.c:59)
+    net = make_network(8, 4)
+    s = AMGSolver(net.ia, net.ja, net.val, min_size=10**6)
+    assert s.num_levels() == 1
+    compare(s, oracle.build(net.ia, net.ja, net.val, coarsening="pmis", min_size=10**6))
+    # diagonal matrix: S empty, every point C -> coarsening stalls
+    n = 50
+    ia = np.arange(n + 1, dtype=np.int32)
+    ja = np.arange(n, dtype=np.int32)
+    v = np.linspace(1, 2, n)
+    s = AMGSolver(ia, ja, v, min_size=10)
+    assert s.num_levels() == 1
+    compare(s, oracle.build(ia, ja, v, coarsening="pmis", min_size=10))
+
+
+@pytest.mark.slow
+def test_setup_bit_exact_lattice60(oracle):
+    net = make_network(60, 12, seed=4)
+    for c in ("pmis", "hmis"):
+        s = AMGSolver(net.ia, net.ja, net.val, coarsening=c)
+        compare(s, oracle.build(net.ia, net.ja, net.val, coarsening=c))
