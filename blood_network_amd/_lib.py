@@ -32,7 +32,7 @@ class Options(C.Structure):
                 ("fine_ic", C.c_int), ("ic_sweeps", C.c_int),
                 ("max_colours", C.c_int), ("flexible", C.c_int),
                 ("stop", C.c_int), ("symmetrize", C.c_int), ("device", C.c_int),
-                ("verbose", C.c_int)]
+                ("verbose", C.c_int), ("graph", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -101,6 +101,11 @@ def _load():
         "bn_solver_ichol": (C.c_int, [VP, C.c_int, IP, IP, DP]),
         "bn_solver_schedule": (C.c_int, [VP, C.c_int, IP, IP, IP]),
         "bn_solver_ic_levels": (C.c_int, [VP, IP, IP]),
+        "bn_solver_time_kernel": (C.c_int, [VP, C.c_int, C.c_int, DP, DP]),
+        "bn_dev_alloc": (C.c_int, [C.POINTER(VP), C.c_size_t]),
+        "bn_dev_free": (C.c_int, [VP]),
+        "bn_dev_copy": (C.c_int, [VP, VP, C.c_size_t, C.c_int]),
+        "bn_dev_sync": (C.c_int, []),
         "bn_device_available": (C.c_int, []),
         "bn_last_error": (C.c_char_p, []),
         "bn_solver_adjoint_grad": (C.c_int, [VP, DP, DP, C.c_int, IP, IP, DP, DP, C.c_double, C.c_int, DP, DP, DP]),
@@ -150,6 +155,43 @@ def dp(a):
 def last_error():
     e = lib.bn_last_error()
     return e.decode() if e else ""
+
+
+class DeviceArray:
+    """A device buffer owned through libbnamg's HIP runtime (FP64 or int32
+    numpy mirror).  Lets ctypes callers (tests, benches) pass device
+    pointers without loading a second HIP runtime."""
+
+    def __init__(self, host):
+        host = np.ascontiguousarray(host)
+        self.dtype, self.shape, self.nbytes = host.dtype, host.shape, host.nbytes
+        p = C.c_void_p()
+        if lib.bn_dev_alloc(C.byref(p), max(self.nbytes, 8)):
+            raise RuntimeError(f"device allocation failed: {last_error()}")
+        self.ptr = p.value
+        self.upload(host)
+
+    def upload(self, host):
+        host = np.ascontiguousarray(host, dtype=self.dtype)
+        if lib.bn_dev_copy(C.c_void_p(self.ptr), host.ctypes.data_as(VP), self.nbytes, 1):
+            raise RuntimeError("host-to-device copy failed")
+
+    def numpy(self):
+        out = np.empty(self.shape, dtype=self.dtype)
+        if lib.bn_dev_copy(out.ctypes.data_as(VP), C.c_void_p(self.ptr), self.nbytes, 2):
+            raise RuntimeError("device-to-host copy failed")
+        return out
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            lib.bn_dev_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def device_available():

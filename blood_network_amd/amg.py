@@ -60,12 +60,13 @@ class AMGSolver:
     def __init__(self, ia, ja, val, threshold=0.25, min_size=4096, max_levels=20, nu=1, *,
                  coarsening=None, precision="fp64", coarse="auto", fine_ic="exact",
                  ic_sweeps=3, flexible=None, stop="bnorm", smoother="cf_gs", max_colours=32,
-                 symmetrize=True, verbose=False):
+                 symmetrize=True, verbose=False, graph=True):
         self.ia, self.ja, self.val = _csr(ia, ja, val)
         self.n = len(self.ia) - 1
         plain = (coarsening is None and precision == "fp64" and coarse == "auto"
                  and fine_ic == "exact" and stop == "bnorm" and smoother == "cf_gs"
-                 and flexible is None and symmetrize and not verbose and max_colours == 32)
+                 and flexible is None and symmetrize and not verbose and max_colours == 32
+                 and graph)
         if plain:
             h = L.lib.amg_solver_create(self.n, len(self.ja), L.ip(self.ia), L.ip(self.ja),
                                         L.dp(self.val), threshold, min_size, max_levels, nu)
@@ -89,6 +90,7 @@ class AMGSolver:
             o.stop = L.STOP_RELRES if stop == "relres" else L.STOP_BNORM
             o.symmetrize = int(bool(symmetrize))
             o.verbose = int(bool(verbose))
+            o.graph = int(bool(graph))
             h = L.lib.bn_solver_create(self.n, len(self.ja), L.ip(self.ia), L.ip(self.ja),
                                        L.dp(self.val), C.byref(o))
         if not h:

@@ -55,6 +55,13 @@ static void free_tri_host(bn_dtri *t) {
 
 void bn_device_release(struct bn_solver *s) {
   if (s->stream) hipStreamSynchronize(s->stream);
+  for (int k = 0; k < s->ngraph; k++) {
+    if (s->gexec[k]) hipGraphExecDestroy(s->gexec[k]);
+    if (s->graph[k]) hipGraphDestroy(s->graph[k]);
+    s->gexec[k] = NULL;
+    s->graph[k] = NULL;
+  }
+  s->ngraph = 0;
   for (int k = 0; k < s->ndptr; k++) hipFree(s->dptr[k]);
   free(s->dptr);
   s->dptr = NULL;
@@ -83,7 +90,10 @@ static void release_all(struct bn_solver *s) {
   s->stream = NULL;
 }
 
-static int ensure_device(struct bn_solver *s) {
+int bn_ensure_device(struct bn_solver *s);
+static int ensure_device(struct bn_solver *s) { return bn_ensure_device(s); }
+
+int bn_ensure_device(struct bn_solver *s) {
   if (s->dev_ready) return 0;
   if (!bn_device_ok()) {
     bn_set_error("no HIP device available: libbnamg solves run only on the GPU");
@@ -116,6 +126,7 @@ void bn_options_default(bn_options *o) {
   o->symmetrize = 1;
   o->device = -1;
   o->verbose = 0;
+  o->graph = 1;
 }
 
 static void copy_input(int n, int nnz, const int *ia, const int *ja, const double *val, int sym,

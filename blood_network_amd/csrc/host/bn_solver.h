@@ -21,6 +21,9 @@ typedef struct {
   int *ia, *ja;
   double *vd;
   float *vs;
+  int long_len;    /* rows longer than this go to the block-per-row bin */
+  int nlong;       /* number of such rows                              */
+  int *long_rows;  /* device list of those rows                        */
 } bn_dcsr;
 
 /* level-scheduled triangular factor L (forward) and L^T (backward) */
@@ -85,6 +88,11 @@ struct bn_solver {
   double *host_scal;    /* pinned mirror of scal */
   double *hbuf;         /* host staging (n doubles, pinned) */
   hipEvent_t ev0, ev1;
+  /* captured PCG steps, one per mode (never re-captured in place) */
+  hipGraph_t graph[8];
+  hipGraphExec_t gexec[8];
+  int graph_key[8];
+  int ngraph;
   bn_solve_info last;
   /* device allocations owned by this solver */
   void **dptr;

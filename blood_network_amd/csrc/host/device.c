@@ -51,10 +51,16 @@ static void *dmalloc(struct bn_solver *s, size_t bytes) {
   return p;
 }
 
+/* Threads per CSR row: the smallest power of two that covers the mean
+ * row in one round of four strides (about four entries per lane, so a
+ * wave keeps several rows' loads in flight), raised until the launch has
+ * about 2^17 threads (small coarse operators otherwise leave CUs idle),
+ * at most one wave64 per row. */
 static int lanes_for(long long nnz, long long rows) {
   double avg = rows > 0 ? (double)nnz / (double)rows : 1.0;
   int l = 1;
-  while (l * 2 <= avg && l < 64) l *= 2;
+  while (4 * l < avg && l < 64) l *= 2;
+  while (l < 64 && (long long)l * rows < (1LL << 17) && l < avg) l *= 2;
   return l;
 }
 
@@ -70,6 +76,25 @@ static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
   if (!d->ia || !d->ja || !d->vd || !d->vs) {
     bn_set_error("device out of memory uploading a %d x %d CSR (%d nnz)", m->row, m->col, m->nnz);
     return BN_ERR_HIP;
+  }
+  /* row-length bins: rows over 16 strides of the lane group go to the
+   * block-per-row kernel (coarse Galerkin rows are heavy-tailed) */
+  d->long_len = 16 * d->lanes;
+  d->nlong = 0;
+  d->long_rows = NULL;
+  for (int i = 0; i < m->row; i++) d->nlong += (m->ia[i + 1] - m->ia[i] > d->long_len);
+  if (d->nlong > 0) {
+    int *lr = (int *)bn_xmalloc((size_t)d->nlong * sizeof(int));
+    int q = 0;
+    for (int i = 0; i < m->row; i++)
+      if (m->ia[i + 1] - m->ia[i] > d->long_len) lr[q++] = i;
+    d->long_rows = (int *)dmalloc(s, (size_t)d->nlong * sizeof(int));
+    if (!d->long_rows) {
+      free(lr);
+      return BN_ERR_HIP;
+    }
+    CK(hipMemcpy(d->long_rows, lr, (size_t)d->nlong * sizeof(int), hipMemcpyHostToDevice));
+    free(lr);
   }
   float *tmp = (float *)bn_xmalloc((size_t)m->nnz * sizeof(float));
   for (int k = 0; k < m->nnz; k++) tmp[k] = (float)m->val[k];
@@ -220,6 +245,11 @@ int bn_device_upload(struct bn_solver *s) {
     if (l < nlev - 1) {
       d->ncol = bn_colour_order(&L->A, L->mis, split, d->perm_h, &d->col_ptr);
       d->use_gs = d->ncol <= s->opt.max_colours;
+      if (!d->use_gs) {
+        /* l1-Jacobi level: keep the natural order, whose neighbouring
+         * rows share x cache lines (colour order would scatter them) */
+        for (int i = 0; i < n; i++) d->perm_h[i] = i;
+      }
     } else {
       for (int i = 0; i < n; i++) d->perm_h[i] = i;
       d->ncol = 0;

@@ -10,6 +10,7 @@
 
 void bn_set_error(const char *fmt, ...);
 void bn_track_device(struct bn_solver *s, void *p);
+int bn_ensure_device(struct bn_solver *s);
 
 /* A is symmetric, so the adjoint system A lam = dL/dp is solved with the
  * same hierarchy and PCG; the edge-scatter kernel then forms
@@ -84,3 +85,80 @@ int bn_solver_solve_batch(void *h, int k, const double *b, double *x, double tol
   }
   return failed ? -worst : worst;
 }
+
+/* ---------------------------------------------------------------------
+ * live kernel timing for the roofline report
+ * ------------------------------------------------------------------- */
+int bn_solver_time_kernel(void *h, int which, int reps, double *ms_avg, double *bytes) {
+  struct bn_solver *s = (struct bn_solver *)h;
+  if (!s || reps <= 0) return BN_ERR_ARG;
+  int rc = bn_ensure_device(s);
+  if (rc) return rc;
+  const bn_dlevel *d = &s->lev[0];
+  const int n = d->n;
+  const char vt = s->vt;
+  const double v = (vt == 's') ? 4.0 : 8.0;
+  const bn_dcsr *A = &s->Asolve;
+  hipStream_t st = s->stream;
+  double b = 0.0;
+  for (int r = 0; r <= reps; r++) {
+    if (r == 1) hipEventRecord(s->ev0, st);
+    switch (which) {
+      case 0:
+        bn_csr_apply('d', BN_OP_SET, n, A->ia, A->ja, A->vd, s->p, s->q, NULL, A->lanes, NULL, NULL, st);
+        b = (double)A->nnz * 12.0 + (n + 1.0) * 4.0 + 2.0 * n * 8.0;
+        break;
+      case 1: {
+        const void *val = vt == 's' ? (const void *)d->A.vs : (const void *)d->A.vd;
+        const void *dinv = vt == 's' ? (const void *)d->dinv_s : (const void *)d->dinv_d;
+        for (int c = 0; c < d->ncol; c++)
+          bn_gs_color(vt, d->col_ptr[c], d->col_ptr[c + 1], d->A.ia, d->A.ja, val, dinv, s->r1, s->u1,
+                      d->A.lanes, NULL, NULL, st);
+        b = (double)d->A.nnz * (4.0 + v) + (n + 1.0) * 4.0 + 3.0 * n * v + n * v; /* + dinv */
+        break;
+      }
+      case 2: {
+        const void *val = vt == 's' ? (const void *)d->A.vs : (const void *)d->A.vd;
+        bn_csr_apply(vt, BN_OP_RESID, n, d->A.ia, d->A.ja, val, s->u1, s->r1, s->g_in, d->A.lanes, NULL,
+                     NULL, st);
+        b = (double)d->A.nnz * (4.0 + v) + (n + 1.0) * 4.0 + 3.0 * n * v;
+        break;
+      }
+      case 3:
+        bn_vc_fine_ic(s, s->r1, s->icx);
+        b = 2.0 * ((double)s->fine_ic.lo.nnz * (4.0 + v) + (n + 1.0) * 4.0 + 3.0 * n * v + 4.0 * n);
+        break;
+      case 4:
+        bn_vc_ichol_precond(s, s->r, s->z);
+        b = 0.0;
+        break;
+      default:
+        return BN_ERR_ARG;
+    }
+  }
+  hipEventRecord(s->ev1, st);
+  if (hipEventSynchronize(s->ev1) != hipSuccess) return BN_ERR_HIP;
+  float ms = 0.0f;
+  hipEventElapsedTime(&ms, s->ev0, s->ev1);
+  if (ms_avg) *ms_avg = ms / reps;
+  if (bytes) *bytes = b;
+  return 0;
+}
+
+/* ---------------------------------------------------------------------
+ * device memory helpers (one HIP runtime per process: the library's)
+ * ------------------------------------------------------------------- */
+int bn_dev_alloc(void **ptr, size_t bytes) {
+  if (!ptr) return BN_ERR_ARG;
+  return hipMalloc(ptr, bytes ? bytes : 8) == hipSuccess ? 0 : BN_ERR_HIP;
+}
+
+int bn_dev_free(void *ptr) { return hipFree(ptr) == hipSuccess ? 0 : BN_ERR_HIP; }
+
+int bn_dev_copy(void *dst, const void *src, size_t bytes, int kind) {
+  hipMemcpyKind k = kind == 1 ? hipMemcpyHostToDevice : kind == 2 ? hipMemcpyDeviceToHost
+                                                                  : hipMemcpyDeviceToDevice;
+  return hipMemcpy(dst, src, bytes, k) == hipSuccess ? 0 : BN_ERR_HIP;
+}
+
+int bn_dev_sync(void) { return hipDeviceSynchronize() == hipSuccess ? 0 : BN_ERR_HIP; }

@@ -5,8 +5,11 @@
  * each step q = A p, alpha = r'z / p'q, x += alpha p, r -= alpha q,
  * z = B r, stop when sqrt(r'z / r0'z0) < tol, beta = r'z_new / r'z_old,
  * p = z + beta p; returns the step count or -maxiter.
- * All vectors and scalars live on the device; the only device-to-host
- * traffic per step is the scalar block (to decide whether to stop).
+ * All vectors and scalars live on the device.  A step is enqueued as one
+ * unit that starts with the p update (beta = 0 and p = 0 before the first
+ * step, so p = z then), which makes every step the same kernel sequence:
+ * it is captured once as a hipGraph and replayed.  The only device-to-host
+ * traffic per step is the scalar block that decides whether to stop.
  * Options: the ||r||/||b|| stop test and the flexible (Polak-Ribiere)
  * beta = z_new'(r_new - r_old) / r_old'z_old, needed when the
  * preconditioner is not a fixed symmetric operator (FP32 V-cycle,
@@ -27,9 +30,54 @@ static double now_ms(void) {
   return 1e3 * (double)t.tv_sec + 1e-6 * (double)t.tv_nsec;
 }
 
-static void precond(struct bn_solver *s, const double *g, double *z, int which) {
-  if (which == 1) bn_vc_vcycle_precond(s, g, z);
-  else bn_vc_ichol_precond(s, g, z);
+/* one PCG step, enqueued on the solver stream */
+static void enqueue_step(struct bn_solver *s) {
+  hipStream_t st = s->stream;
+  const int n = s->lev[0].n;
+  const bn_dcsr *A = &s->Asolve;
+  double *sc = s->scal;
+  const int flex = s->opt.flexible;
+  bn_pcg_update_p(n, s->p, s->z, sc, st);
+  /* q = A p, p'q -> alpha */
+  bn_csr_apply('d', BN_OP_SET, n, A->ia, A->ja, A->vd, s->p, s->q, NULL, A->lanes, s->p, s->part1, st);
+  bn_scalar_finish(1, s->part1, NULL, NULL, sc, 0, st);
+  bn_pcg_update_xr(n, s->x, s->p, s->r, s->q, sc, s->part2, st);
+  bn_vc_ichol_precond(s, s->r, s->z);
+  bn_dot_partial(n, s->r, s->z, flex ? s->z : NULL, flex ? s->q : NULL, s->part1, s->part3, st);
+  bn_scalar_finish(flex ? 4 : 2, s->part1, s->part2, flex ? s->part3 : NULL, sc, 0, st);
+}
+
+static int mode_key(const struct bn_solver *s) {
+  return (s->vt == 's') | (s->opt.fine_ic << 1) | (s->opt.flexible << 3) | (s->opt.smoother << 4) |
+         (s->opt.nu << 8) | (s->opt.ic_sweeps << 16);
+}
+
+/* The captured step for the current mode, capturing it on first use.
+ * Graphs are kept per mode until the device state is released.  Returns
+ * NULL when graphs are off (option, BN_NO_GRAPH in the environment, or a
+ * host-driven coarsest solve). */
+static hipGraphExec_t step_graph(struct bn_solver *s) {
+  if (!s->opt.graph || !s->coarse_dense) return NULL;
+  const char *env = getenv("BN_NO_GRAPH");
+  if (env && env[0] && env[0] != '0') return NULL;
+  const int key = mode_key(s);
+  for (int k = 0; k < s->ngraph; k++)
+    if (s->graph_key[k] == key) return s->gexec[k];
+  if (s->ngraph == (int)(sizeof(s->gexec) / sizeof(s->gexec[0]))) return NULL;
+  hipGraph_t g = NULL;
+  hipGraphExec_t e = NULL;
+  if (hipStreamBeginCapture(s->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) return NULL;
+  enqueue_step(s);
+  if (hipStreamEndCapture(s->stream, &g) != hipSuccess || !g) return NULL;
+  if (hipGraphInstantiate(&e, g, NULL, NULL, 0) != hipSuccess) {
+    hipGraphDestroy(g);
+    return NULL;
+  }
+  s->graph[s->ngraph] = g;
+  s->gexec[s->ngraph] = e;
+  s->graph_key[s->ngraph] = key;
+  s->ngraph++;
+  return e;
 }
 
 /* d_b, d_x in caller (original) numbering; x is the initial guess. */
@@ -39,11 +87,10 @@ int bn_pcg_run(struct bn_solver *s, const double *d_b, double *d_x, double tol, 
   hipStream_t st = s->stream;
   const int n = s->lev[0].n;
   const bn_dcsr *A = &s->Asolve;
-  const int which = 0;
   double *sc = s->scal;
   double *hs = s->host_scal;
-  const int flex = s->opt.flexible;
   const int relres_stop = (s->opt.stop == BN_STOP_RELRES);
+  hipGraphExec_t gexec = step_graph(s);
 
   hipEventRecord(s->ev0, st);
   bn_permute(n, s->lev[0].perm_d, d_b, s->bb, 0, st);
@@ -52,8 +99,8 @@ int bn_pcg_run(struct bn_solver *s, const double *d_b, double *d_x, double tol, 
   bn_csr_apply('d', BN_OP_RESID, n, A->ia, A->ja, A->vd, s->x, s->r, s->bb, A->lanes, NULL, NULL, st);
   bn_dot_partial(n, s->bb, s->bb, NULL, NULL, s->part1, NULL, st);
   bn_scalar_finish(3, s->part1, NULL, NULL, sc, 0, st);
-  precond(s, s->r, s->z, which);
-  bn_copy('d', n, s->z, s->p, st);
+  bn_vc_ichol_precond(s, s->r, s->z);
+  bn_fill('d', n, s->p, 0.0, st);
   bn_dot_partial(n, s->r, s->z, s->r, s->r, s->part1, s->part2, st);
   bn_scalar_finish(0, s->part1, s->part2, NULL, sc, 0, st);
   hipMemcpyAsync(hs, sc, BN_S_COUNT * sizeof(double), hipMemcpyDeviceToHost, st);
@@ -76,13 +123,8 @@ int bn_pcg_run(struct bn_solver *s, const double *d_b, double *d_x, double tol, 
     rel = 0.0;
   } else {
     for (it = 1; it <= maxiter; it++) {
-      /* q = A p, p'q */
-      bn_csr_apply('d', BN_OP_SET, n, A->ia, A->ja, A->vd, s->p, s->q, NULL, A->lanes, s->p, s->part1, st);
-      bn_scalar_finish(1, s->part1, NULL, NULL, sc, 0, st);
-      bn_pcg_update_xr(n, s->x, s->p, s->r, s->q, sc, s->part2, st);
-      precond(s, s->r, s->z, which);
-      bn_dot_partial(n, s->r, s->z, flex ? s->z : NULL, flex ? s->q : NULL, s->part1, s->part3, st);
-      bn_scalar_finish(flex ? 4 : 2, s->part1, s->part2, flex ? s->part3 : NULL, sc, 0, st);
+      if (gexec) hipGraphLaunch(gexec, st);
+      else enqueue_step(s);
       hipMemcpyAsync(hs, sc, BN_S_COUNT * sizeof(double), hipMemcpyDeviceToHost, st);
       if (hipStreamSynchronize(st) != hipSuccess) {
         bn_set_error("device failure in PCG step %d", it);
@@ -95,7 +137,6 @@ int bn_pcg_run(struct bn_solver *s, const double *d_b, double *d_x, double tol, 
                 hs[BN_S_RZNEW] / hs[BN_S_RZOLD]);
       const double test = relres_stop ? rnorm / bnorm : rel;
       if (test < tol) break;
-      bn_pcg_update_p(n, s->p, s->z, sc, st);
     }
   }
   if (it > maxiter) {

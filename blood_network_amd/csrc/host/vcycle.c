@@ -31,7 +31,8 @@ static inline const void *vecv(const double *vd, const float *vs, char vt) {
 
 /* y = op(M, x [, b]) in precision vt */
 static void apply(struct bn_solver *s, const bn_dcsr *M, int op, const void *x, void *y, const void *b) {
-  bn_csr_apply(s->vt, op, M->rows, M->ia, M->ja, opv(M, s->vt), x, y, b, M->lanes, NULL, NULL, s->stream);
+  bn_csr_apply_binned(s->vt, op, M->rows, M->ia, M->ja, opv(M, s->vt), x, y, b, M->lanes, M->long_len,
+                      M->nlong, M->long_rows, s->stream);
 }
 
 /* nu smoothing sweeps on level `lev`; backward runs colours in reverse.

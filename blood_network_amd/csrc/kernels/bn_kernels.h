@@ -30,7 +30,7 @@ enum {
 };
 
 /* reduction partial buffer holds at most this many blocks per dot */
-#define BN_RED_BLOCKS 1024
+#define BN_RED_BLOCKS 2048
 
 /* CSR operator kinds for bn_csr_apply */
 enum {
@@ -46,6 +46,14 @@ void bn_csr_apply(char vt, int op, int n, const int *ia, const int *ja,
                   const void *val, const void *x, void *y, const void *b,
                   int lanes, const void *dotw, double *partial,
                   hipStream_t stream);
+
+/* Same, split by row length: rows longer than long_len (listed in
+ * long_rows, nlong of them) are computed by a block-per-row kernel, the
+ * rest by the lanes-per-row kernel.  No fused dot. */
+void bn_csr_apply_binned(char vt, int op, int n, const int *ia, const int *ja,
+                         const void *val, const void *x, void *y, const void *b,
+                         int lanes, int long_len, int nlong, const int *long_rows,
+                         hipStream_t stream);
 
 /* One multicolour Gauss-Seidel colour: rows lo..hi-1 of a level whose
  * rows are stored in colour order:
@@ -105,7 +113,7 @@ void bn_axpy1(char vt, int n, const void *x, void *y, hipStream_t stream);
 
 /* Scalar-finish kernels (single block): reduce partials and update s[].
  * which:
- *   0 start:   s[RZ]=s[RZ0]=sum(p1), s[RR]=sum(p2), s[BB] kept
+ *   0 start:   s[RZ]=s[RZ0]=sum(p1), s[RR]=sum(p2), s[BETA]=0, s[BB] kept
  *   1 alpha:   s[PAP]=sum(p1), s[ALPHA]=s[RZ]/s[PAP]
  *   2 iterate: s[RZNEW]=sum(p1), s[RR]=sum(p2), s[REL]=sqrt(RZNEW/RZ0),
  *              s[BETA]=RZNEW/RZ, s[RZ]=RZNEW

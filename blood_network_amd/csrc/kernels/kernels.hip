@@ -57,6 +57,50 @@ __device__ __forceinline__ void clear_tail(double *partial) {
   for (int k = gridDim.x + threadIdx.x; k < BN_RED_BLOCKS; k += BLOCK) partial[k] = 0.0;
 }
 
+// Sum of val[q] * x[col[q]] over one CSR row segment, `LANES` lanes
+// striding through it.  Every round issues four strides of loads before
+// using any (the last, partial round with predicated loads), so a lane
+// keeps its gathers in flight together instead of forming a chain of
+// dependent load rounds.  SKIP_DIAG drops the entry whose column equals
+// `self`.
+template <typename T, int LANES, bool SKIP_DIAG>
+__device__ __forceinline__ T row_dot(int q, const int e, const int *__restrict__ ja,
+                                     const T *__restrict__ val, const T *__restrict__ x, int self) {
+  T s0 = 0, s1 = 0;
+  for (; q + 3 * LANES < e; q += 4 * LANES) {
+    const int j0 = ja[q], j1 = ja[q + LANES], j2 = ja[q + 2 * LANES], j3 = ja[q + 3 * LANES];
+    const T a0 = val[q], a1 = val[q + LANES], a2 = val[q + 2 * LANES], a3 = val[q + 3 * LANES];
+    const T x0 = (SKIP_DIAG && j0 == self) ? T(0) : x[j0];
+    const T x1 = (SKIP_DIAG && j1 == self) ? T(0) : x[j1];
+    const T x2 = (SKIP_DIAG && j2 == self) ? T(0) : x[j2];
+    const T x3 = (SKIP_DIAG && j3 == self) ? T(0) : x[j3];
+    s0 += a0 * x0 + a1 * x1;
+    s1 += a2 * x2 + a3 * x3;
+  }
+  if (q < e) {
+    const bool h1 = q + LANES < e, h2 = q + 2 * LANES < e;
+    const int j0 = ja[q];
+    const int j1 = h1 ? ja[q + LANES] : self;
+    const int j2 = h2 ? ja[q + 2 * LANES] : self;
+    const T a0 = val[q];
+    const T a1 = h1 ? val[q + LANES] : T(0);
+    const T a2 = h2 ? val[q + 2 * LANES] : T(0);
+    const T x0 = (SKIP_DIAG && j0 == self) ? T(0) : x[j0];
+    const T x1 = (!h1 || (SKIP_DIAG && j1 == self)) ? T(0) : x[j1];
+    const T x2 = (!h2 || (SKIP_DIAG && j2 == self)) ? T(0) : x[j2];
+    s0 += a0 * x0 + a1 * x1;
+    s1 += a2 * x2;
+  }
+  return s0 + s1;
+}
+
+// Grid for one-row-block-per-block kernels: rounded up to a multiple of
+// 8 (when large) so logical_block() can give each XCD a contiguous slice.
+inline int grid_rows(long long nrb) {
+  if (nrb >= 64) return (int)((nrb + 7) & ~7LL);
+  return (int)(nrb > 0 ? nrb : 1);
+}
+
 inline int grid_for(long long nrb) {
   long long g = nrb;
   if (g > BN_RED_BLOCKS) g = BN_RED_BLOCKS;
@@ -73,7 +117,7 @@ __global__ __launch_bounds__(BLOCK) void k_csr(int n, int chunk, const int *__re
                                                const int *__restrict__ ja, const T *__restrict__ val,
                                                const T *__restrict__ x, T *__restrict__ y,
                                                const T *__restrict__ b, const T *__restrict__ w,
-                                               double *__restrict__ partial) {
+                                               double *__restrict__ partial, int long_len) {
   constexpr int RPB = BLOCK / LANES;
   const int lb = logical_block();
   const int sub = threadIdx.x % LANES;
@@ -85,9 +129,9 @@ __global__ __launch_bounds__(BLOCK) void k_csr(int n, int chunk, const int *__re
   for (int rb = rb0; rb < rb1; rb++) {
     const int row = rb * RPB + threadIdx.x / LANES;
     if (row < n) {
-      T s = 0;
-      const int e = ia[row + 1];
-      for (int q = ia[row] + sub; q < e; q += LANES) s += val[q] * x[ja[q]];
+      const int q0 = ia[row], q1 = ia[row + 1];
+      if (q1 - q0 > long_len) continue; /* handled by k_csr_long */
+      T s = row_dot<T, LANES, false>(q0 + sub, q1, ja, val, x, row);
       if (LANES > 1) s = group_sum<LANES>(s);
       if (sub == 0) {
         T out;
@@ -108,41 +152,69 @@ __global__ __launch_bounds__(BLOCK) void k_csr(int n, int chunk, const int *__re
 
 template <typename T, int LANES>
 void launch_csr(int op, int n, const int *ia, const int *ja, const T *val, const T *x, T *y,
-                const T *b, const T *w, double *partial, hipStream_t st) {
+                const T *b, const T *w, double *partial, int long_len, hipStream_t st) {
   constexpr int RPB = BLOCK / LANES;
   const long long nrb = ((long long)n + RPB - 1) / RPB;
-  const int g = grid_for(nrb);
-  const int chunk = (int)((nrb + g - 1) / g);
+#define BN_CSR_LAUNCH(OPV, DOTV, G, CH) \
+  hipLaunchKernelGGL((k_csr<T, LANES, OPV, DOTV>), dim3(G), dim3(BLOCK), 0, st, n, CH, ia, ja, val, x, y, b, w, partial, long_len)
   if (partial) {
-    // partial slots of blocks beyond the grid must read as zero
-    if (op == BN_OP_SET)
-      hipLaunchKernelGGL((k_csr<T, LANES, BN_OP_SET, true>), dim3(g), dim3(BLOCK), 0, st, n, chunk, ia, ja, val, x, y, b, w, partial);
-    else if (op == BN_OP_RESID)
-      hipLaunchKernelGGL((k_csr<T, LANES, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, st, n, chunk, ia, ja, val, x, y, b, w, partial);
-    else
-      hipLaunchKernelGGL((k_csr<T, LANES, BN_OP_ADD, true>), dim3(g), dim3(BLOCK), 0, st, n, chunk, ia, ja, val, x, y, b, w, partial);
+    // fused dot: capped grid so the partials fit BN_RED_BLOCKS slots
+    const int g = grid_for(nrb);
+    const int chunk = (int)((nrb + g - 1) / g);
+    if (op == BN_OP_SET) BN_CSR_LAUNCH(BN_OP_SET, true, g, chunk);
+    else if (op == BN_OP_RESID) BN_CSR_LAUNCH(BN_OP_RESID, true, g, chunk);
+    else BN_CSR_LAUNCH(BN_OP_ADD, true, g, chunk);
   } else {
-    if (op == BN_OP_SET)
-      hipLaunchKernelGGL((k_csr<T, LANES, BN_OP_SET, false>), dim3(g), dim3(BLOCK), 0, st, n, chunk, ia, ja, val, x, y, b, w, partial);
-    else if (op == BN_OP_RESID)
-      hipLaunchKernelGGL((k_csr<T, LANES, BN_OP_RESID, false>), dim3(g), dim3(BLOCK), 0, st, n, chunk, ia, ja, val, x, y, b, w, partial);
-    else
-      hipLaunchKernelGGL((k_csr<T, LANES, BN_OP_ADD, false>), dim3(g), dim3(BLOCK), 0, st, n, chunk, ia, ja, val, x, y, b, w, partial);
+    // one row-block per block: the hardware keeps every CU full
+    const int g = grid_rows(nrb);
+    if (op == BN_OP_SET) BN_CSR_LAUNCH(BN_OP_SET, false, g, 1);
+    else if (op == BN_OP_RESID) BN_CSR_LAUNCH(BN_OP_RESID, false, g, 1);
+    else BN_CSR_LAUNCH(BN_OP_ADD, false, g, 1);
   }
+#undef BN_CSR_LAUNCH
 }
 
 template <typename T>
 void dispatch_csr(int op, int n, const int *ia, const int *ja, const T *val, const T *x, T *y,
-                  const T *b, int lanes, const T *w, double *partial, hipStream_t st) {
+                  const T *b, int lanes, const T *w, double *partial, int long_len, hipStream_t st) {
   switch (lanes) {
-    case 1: launch_csr<T, 1>(op, n, ia, ja, val, x, y, b, w, partial, st); break;
-    case 2: launch_csr<T, 2>(op, n, ia, ja, val, x, y, b, w, partial, st); break;
-    case 4: launch_csr<T, 4>(op, n, ia, ja, val, x, y, b, w, partial, st); break;
-    case 8: launch_csr<T, 8>(op, n, ia, ja, val, x, y, b, w, partial, st); break;
-    case 16: launch_csr<T, 16>(op, n, ia, ja, val, x, y, b, w, partial, st); break;
-    case 32: launch_csr<T, 32>(op, n, ia, ja, val, x, y, b, w, partial, st); break;
-    default: launch_csr<T, 64>(op, n, ia, ja, val, x, y, b, w, partial, st); break;
+    case 1: launch_csr<T, 1>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
+    case 2: launch_csr<T, 2>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
+    case 4: launch_csr<T, 4>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
+    case 8: launch_csr<T, 8>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
+    case 16: launch_csr<T, 16>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
+    case 32: launch_csr<T, 32>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
+    default: launch_csr<T, 64>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
   }
+}
+
+// Long-row bin: one 256-thread block per listed row (rows longer than
+// the main kernel's limit), finished with a block-wide reduction.
+template <typename T, int OP>
+__global__ __launch_bounds__(BLOCK) void k_csr_long(const int *__restrict__ rows, const int *__restrict__ ia,
+                                                    const int *__restrict__ ja, const T *__restrict__ val,
+                                                    const T *__restrict__ x, T *__restrict__ y,
+                                                    const T *__restrict__ b) {
+  const int row = rows[blockIdx.x];
+  const T s = row_dot<T, BLOCK, false>(ia[row] + threadIdx.x, ia[row + 1], ja, val, x, row);
+  const double t = block_sum((double)s);
+  if (threadIdx.x == 0) {
+    if (OP == BN_OP_SET) y[row] = (T)t;
+    else if (OP == BN_OP_RESID) y[row] = b[row] - (T)t;
+    else y[row] = y[row] + (T)t;
+  }
+}
+
+template <typename T>
+void launch_long(int op, int nlong, const int *rows, const int *ia, const int *ja, const T *val, const T *x,
+                 T *y, const T *b, hipStream_t st) {
+  if (nlong <= 0) return;
+  if (op == BN_OP_SET)
+    hipLaunchKernelGGL((k_csr_long<T, BN_OP_SET>), dim3(nlong), dim3(BLOCK), 0, st, rows, ia, ja, val, x, y, b);
+  else if (op == BN_OP_RESID)
+    hipLaunchKernelGGL((k_csr_long<T, BN_OP_RESID>), dim3(nlong), dim3(BLOCK), 0, st, rows, ia, ja, val, x, y, b);
+  else
+    hipLaunchKernelGGL((k_csr_long<T, BN_OP_ADD>), dim3(nlong), dim3(BLOCK), 0, st, rows, ia, ja, val, x, y, b);
 }
 
 // ---------------------------------------------------------------------
@@ -156,16 +228,11 @@ __global__ __launch_bounds__(BLOCK) void k_rowsolve(int lo, int cnt, const int *
                                                     const T *__restrict__ b, T *__restrict__ y,
                                                     const T *__restrict__ zbase, T *__restrict__ zsum) {
   constexpr int RPB = BLOCK / LANES;
-  const int k = blockIdx.x * RPB + threadIdx.x / LANES;
+  const int k = logical_block() * RPB + threadIdx.x / LANES;
   const int sub = threadIdx.x % LANES;
   if (k >= cnt) return;
   const int i = LIST ? rows[k] : lo + k;
-  T s = 0;
-  const int e = ia[i + 1];
-  for (int q = ia[i] + sub; q < e; q += LANES) {
-    const int j = ja[q];
-    if (j != i) s += val[q] * y[j];
-  }
+  T s = row_dot<T, LANES, true>(ia[i] + sub, ia[i + 1], ja, val, y, i);
   if (LANES > 1) s = group_sum<LANES>(s);
   if (sub == 0) {
     const T v = (b[i] - s) * dinv[i];
@@ -179,7 +246,7 @@ void dispatch_rowsolve(int lo, int cnt, const int *rows, const int *ia, const in
                        const T *dinv, const T *b, T *y, int lanes, const T *zb, T *zs, hipStream_t st) {
   if (cnt <= 0) return;
 #define BN_RS(L)                                                                                   \
-  hipLaunchKernelGGL((k_rowsolve<T, L, LIST>), dim3((cnt + BLOCK / L - 1) / (BLOCK / L)), dim3(BLOCK), \
+  hipLaunchKernelGGL((k_rowsolve<T, L, LIST>), dim3(grid_rows((cnt + BLOCK / L - 1) / (BLOCK / L))), dim3(BLOCK), \
                      0, st, lo, cnt, rows, ia, ja, val, dinv, b, y, zb, zs)
   switch (lanes) {
     case 1: BN_RS(1); break;
@@ -199,15 +266,10 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_rows(int n, const int *__restr
                                                        const T *__restrict__ dinv, const T *__restrict__ b,
                                                        const T *__restrict__ xin, T *__restrict__ xout) {
   constexpr int RPB = BLOCK / LANES;
-  const int i = blockIdx.x * RPB + threadIdx.x / LANES;
+  const int i = logical_block() * RPB + threadIdx.x / LANES;
   const int sub = threadIdx.x % LANES;
   if (i >= n) return;
-  T s = 0;
-  const int e = ia[i + 1];
-  for (int q = ia[i] + sub; q < e; q += LANES) {
-    const int j = ja[q];
-    if (j != i) s += val[q] * xin[j];
-  }
+  T s = row_dot<T, LANES, true>(ia[i] + sub, ia[i + 1], ja, val, xin, i);
   if (LANES > 1) s = group_sum<LANES>(s);
   if (sub == 0) xout[i] = (b[i] - s) * dinv[i];
 }
@@ -216,7 +278,7 @@ template <typename T>
 void dispatch_jacobi_rows(int n, const int *ia, const int *ja, const T *val, const T *dinv, const T *b,
                           const T *xin, T *xout, int lanes, hipStream_t st) {
 #define BN_JR(L)                                                                                       \
-  hipLaunchKernelGGL((k_jacobi_rows<T, L>), dim3((n + BLOCK / L - 1) / (BLOCK / L)), dim3(BLOCK), 0, st, \
+  hipLaunchKernelGGL((k_jacobi_rows<T, L>), dim3(grid_rows((n + BLOCK / L - 1) / (BLOCK / L))), dim3(BLOCK), 0, st, \
                      n, ia, ja, val, dinv, b, xin, xout)
   switch (lanes) {
     case 1: BN_JR(1); break;
@@ -354,6 +416,7 @@ __global__ __launch_bounds__(BLOCK) void k_scalar_finish(int which, const double
       s[BN_S_RZ0] = a;
       s[BN_S_RR] = b;
       s[BN_S_REL] = 1.0;
+      s[BN_S_BETA] = 0.0; /* first p = z + 0 p with p zeroed */
       break;
     case 1:
       s[BN_S_PAP] = a;
@@ -420,10 +483,28 @@ void bn_csr_apply(char vt, int op, int n, const int *ia, const int *ja, const vo
   }
   if (vt == 's')
     dispatch_csr<float>(op, n, ia, ja, (const float *)val, (const float *)x, (float *)y,
-                        (const float *)b, lanes, (const float *)dotw, partial, stream);
+                        (const float *)b, lanes, (const float *)dotw, partial, 0x7fffffff, stream);
   else
     dispatch_csr<double>(op, n, ia, ja, (const double *)val, (const double *)x, (double *)y,
-                         (const double *)b, lanes, (const double *)dotw, partial, stream);
+                         (const double *)b, lanes, (const double *)dotw, partial, 0x7fffffff, stream);
+}
+
+void bn_csr_apply_binned(char vt, int op, int n, const int *ia, const int *ja, const void *val,
+                         const void *x, void *y, const void *b, int lanes, int long_len, int nlong,
+                         const int *long_rows, hipStream_t stream) {
+  if (n <= 0) return;
+  if (nlong <= 0) long_len = 0x7fffffff;
+  if (vt == 's') {
+    dispatch_csr<float>(op, n, ia, ja, (const float *)val, (const float *)x, (float *)y, (const float *)b,
+                        lanes, nullptr, nullptr, long_len, stream);
+    launch_long<float>(op, nlong, long_rows, ia, ja, (const float *)val, (const float *)x, (float *)y,
+                       (const float *)b, stream);
+  } else {
+    dispatch_csr<double>(op, n, ia, ja, (const double *)val, (const double *)x, (double *)y,
+                         (const double *)b, lanes, nullptr, nullptr, long_len, stream);
+    launch_long<double>(op, nlong, long_rows, ia, ja, (const double *)val, (const double *)x, (double *)y,
+                        (const double *)b, stream);
+  }
 }
 
 void bn_gs_color(char vt, int lo, int hi, const int *ia, const int *ja, const void *val,

@@ -9,6 +9,7 @@
 #ifndef BN_AMG_EXT_H
 #define BN_AMG_EXT_H
 
+#include <stddef.h>
 #include <stdint.h>
 
 #include "amg.h"
@@ -58,6 +59,7 @@ typedef struct {
   int symmetrize;     /* 1: use (A + A^T)/2 as the reference wrapper does */
   int device;         /* HIP device ordinal, -1 = current                 */
   int verbose;        /* 1: hierarchy report on stderr                    */
+  int graph;          /* 1: replay each PCG step as one captured hipGraph */
 } bn_options;
 
 typedef struct {
@@ -102,6 +104,25 @@ int bn_solver_ichol(void *h, int which, int *ia, int *ja, double *val);
 /* device schedule: colours / levels used on each level (for reports) */
 int bn_solver_schedule(void *h, int lev, int *ncolours, int *use_gs, int *lanes);
 int bn_solver_ic_levels(void *h, int *fwd, int *bwd);
+
+/* Time one kernel of the solve on the solver's stream with HIP events
+ * (average over `reps` launches after one warm-up launch), and give its
+ * algorithmic (compulsory) HBM bytes per launch.  which:
+ *   0  fine-level CSR SpMV in FP64 (the outer PCG's q = A p)
+ *   1  one full fine-level multicolour Gauss-Seidel sweep, V-cycle precision
+ *   2  fine-level residual r = b - A x, V-cycle precision
+ *   3  fine-level IC(0) forward + backward application (current IC mode)
+ *   4  one preconditioner application (AMG + IC)
+ * Returns 0 or a BN_ERR_* code. */
+int bn_solver_time_kernel(void *h, int which, int reps, double *ms_avg, double *bytes);
+
+/* Device memory helpers for callers that hold no HIP runtime of their
+ * own (e.g. ctypes users): plain hipMalloc/hipMemcpy/hipFree on the
+ * current device.  Return 0 or BN_ERR_HIP. */
+int bn_dev_alloc(void **ptr, size_t bytes);
+int bn_dev_free(void *ptr);
+int bn_dev_copy(void *dst, const void *src, size_t bytes, int kind); /* 1 H2D, 2 D2H, 3 D2D */
+int bn_dev_sync(void);
 
 /* 1 if a HIP device is usable */
 int bn_device_available(void);

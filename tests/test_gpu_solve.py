@@ -114,3 +114,20 @@ def test_solve_is_deterministic(net30):
     x1, i1 = s.solve(net30.b, tol=1e-10)
     x2, i2 = s.solve(net30.b, tol=1e-10)
     assert i1 == i2 and np.array_equal(x1, x2)
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_graph_replay_equals_eager(net30, precision):
+    # every PCG step replays one captured hipGraph; eager launches of the
+    # same kernels must give bit-identical iterates, across several solves
+    # with other work in between
+    _case(f"lattice30 graph vs eager {precision}")
+    kw = dict(coarsening="pmis", min_size=500, precision=precision,
+              fine_ic="jacobi" if precision == "fp32" else "exact")
+    sg = AMGSolver(net30.ia, net30.ja, net30.val, graph=True, **kw)
+    se = AMGSolver(net30.ia, net30.ja, net30.val, graph=False, **kw)
+    xe, ie = se.solve(net30.b, tol=1e-10)
+    for rep in range(3):
+        xg, ig = sg.solve(net30.b, tol=1e-10)
+        assert ig == ie and np.array_equal(xg, xe)
+        se.solve(net30.b * (rep + 2.0), tol=1e-6)  # eager work between replays
