@@ -17,7 +17,9 @@ extern "C" {
 
 /* a CSR operator on the device, FP64 and/or FP32 values */
 typedef struct {
-  int rows, cols, nnz, lanes;
+  int rows, cols, nnz;
+  int lanes;       /* FP64 lanes per row (autotuned at upload)          */
+  int lanes_s;     /* FP32 lanes per row                                */
   int *ia, *ja;
   double *vd;
   float *vs;
@@ -98,6 +100,9 @@ struct bn_solver {
   void **dptr;
   int ndptr, capdptr;
 };
+
+/* lanes per row for precision vt */
+static inline int bn_lanes(const bn_dcsr *m, char vt) { return vt == 's' ? m->lanes_s : m->lanes; }
 
 /* device.c */
 int bn_device_upload(struct bn_solver *s);

@@ -68,7 +68,7 @@ static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
   d->rows = m->row;
   d->cols = m->col;
   d->nnz = m->nnz;
-  d->lanes = lanes_for(m->nnz, m->row);
+  d->lanes = d->lanes_s = lanes_for(m->nnz, m->row);
   d->ia = (int *)dmalloc(s, ((size_t)m->row + 1) * sizeof(int));
   d->ja = (int *)dmalloc(s, (size_t)m->nnz * sizeof(int));
   d->vd = (double *)dmalloc(s, (size_t)m->nnz * sizeof(double));
@@ -190,6 +190,65 @@ static int upload_tri(struct bn_solver *s, const ichol_factor *F, const int *per
   free(rows_f);
   free(rows_b);
   return rc;
+}
+
+/* Autotuner: time the operator's SpMV for a few lanes-per-row choices
+ * around the heuristic, in both precisions, and keep the fastest.  x and
+ * y are scratch device buffers of at least cols / rows doubles. */
+static void autotune_csr(struct bn_solver *s, bn_dcsr *d, void *x, void *y) {
+  if (!d->ia || d->rows <= 0 || d->nnz <= 0) return;
+  const int base = d->lanes;
+  int cand[6], nc = 0;
+  const int opts[6] = {1, base / 4, base / 2, base, base * 2, base * 4};
+  for (int k = 0; k < 6; k++) {
+    int l = opts[k];
+    if (l < 1 || l > 64) continue;
+    int dup = 0;
+    for (int q = 0; q < nc; q++) dup |= (cand[q] == l);
+    if (!dup) cand[nc++] = l;
+  }
+  for (int pv = 0; pv < 2; pv++) {
+    const char vt = pv ? 's' : 'd';
+    const void *val = pv ? (const void *)d->vs : (const void *)d->vd;
+    float best = 1e30f;
+    int bl = base;
+    for (int k = 0; k < nc; k++) {
+      const int reps = 3;
+      bn_csr_apply_binned(vt, BN_OP_SET, d->rows, d->ia, d->ja, val, x, y, NULL, cand[k], d->long_len,
+                          d->nlong, d->long_rows, s->stream);
+      hipEventRecord(s->ev0, s->stream);
+      for (int r = 0; r < reps; r++)
+        bn_csr_apply_binned(vt, BN_OP_SET, d->rows, d->ia, d->ja, val, x, y, NULL, cand[k], d->long_len,
+                            d->nlong, d->long_rows, s->stream);
+      hipEventRecord(s->ev1, s->stream);
+      hipEventSynchronize(s->ev1);
+      float ms = 0.0f;
+      hipEventElapsedTime(&ms, s->ev0, s->ev1);
+      if (ms < best) {
+        best = ms;
+        bl = cand[k];
+      }
+    }
+    if (pv) d->lanes_s = bl;
+    else d->lanes = bl;
+  }
+}
+
+static void autotune_all(struct bn_solver *s) {
+  void *x = s->x, *y = s->q;
+  bn_fill('d', s->lev[0].n, x, 1.0, s->stream);
+  for (int l = 0; l < s->nlev; l++) {
+    autotune_csr(s, &s->lev[l].A, x, y);
+    autotune_csr(s, &s->lev[l].P, x, y);
+    autotune_csr(s, &s->lev[l].R, x, y);
+  }
+  if (!s->asolve_shared) autotune_csr(s, &s->Asolve, x, y);
+  else s->Asolve = s->lev[0].A;
+  autotune_csr(s, &s->fine_ic.lo, x, y);
+  autotune_csr(s, &s->fine_ic.up, x, y);
+  autotune_csr(s, &s->coarse_ic.lo, x, y);
+  autotune_csr(s, &s->coarse_ic.up, x, y);
+  hipStreamSynchronize(s->stream);
 }
 
 static int alloc_vec(struct bn_solver *s, void **p, int n) {
@@ -353,6 +412,8 @@ int bn_device_upload(struct bn_solver *s) {
   }
   for (int l = 0; l < nlev; l++) free(inv[l]);
   if (!rc) {
+    const char *env = getenv("BN_NO_AUTOTUNE");
+    if (!(env && env[0] && env[0] != '0')) autotune_all(s);
     CK(hipDeviceSynchronize());
     s->dev_ready = 1;
   }

@@ -113,14 +113,14 @@ int bn_solver_time_kernel(void *h, int which, int reps, double *ms_avg, double *
         const void *dinv = vt == 's' ? (const void *)d->dinv_s : (const void *)d->dinv_d;
         for (int c = 0; c < d->ncol; c++)
           bn_gs_color(vt, d->col_ptr[c], d->col_ptr[c + 1], d->A.ia, d->A.ja, val, dinv, s->r1, s->u1,
-                      d->A.lanes, NULL, NULL, st);
+                      bn_lanes(&d->A, vt), NULL, NULL, st);
         b = (double)d->A.nnz * (4.0 + v) + (n + 1.0) * 4.0 + 3.0 * n * v + n * v; /* + dinv */
         break;
       }
       case 2: {
         const void *val = vt == 's' ? (const void *)d->A.vs : (const void *)d->A.vd;
-        bn_csr_apply(vt, BN_OP_RESID, n, d->A.ia, d->A.ja, val, s->u1, s->r1, s->g_in, d->A.lanes, NULL,
-                     NULL, st);
+        bn_csr_apply(vt, BN_OP_RESID, n, d->A.ia, d->A.ja, val, s->u1, s->r1, s->g_in, bn_lanes(&d->A, vt),
+                     NULL, NULL, st);
         b = (double)d->A.nnz * (4.0 + v) + (n + 1.0) * 4.0 + 3.0 * n * v;
         break;
       }

@@ -31,7 +31,7 @@ static inline const void *vecv(const double *vd, const float *vs, char vt) {
 
 /* y = op(M, x [, b]) in precision vt */
 static void apply(struct bn_solver *s, const bn_dcsr *M, int op, const void *x, void *y, const void *b) {
-  bn_csr_apply_binned(s->vt, op, M->rows, M->ia, M->ja, opv(M, s->vt), x, y, b, M->lanes, M->long_len,
+  bn_csr_apply_binned(s->vt, op, M->rows, M->ia, M->ja, opv(M, s->vt), x, y, b, bn_lanes(M, s->vt), M->long_len,
                       M->nlong, M->long_rows, s->stream);
 }
 
@@ -57,7 +57,7 @@ static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int bac
           continue;
         }
         bn_gs_color(vt, lo, hi, d->A.ia, d->A.ja, opv(&d->A, vt), vecv(d->dinv_d, d->dinv_s, vt), b, x,
-                    d->A.lanes, last ? zbase : NULL, last ? zsum : NULL, s->stream);
+                    bn_lanes(&d->A, vt), last ? zbase : NULL, last ? zsum : NULL, s->stream);
       }
     }
   } else {
@@ -163,10 +163,10 @@ static void tri_solve(struct bn_solver *s, bn_dtri *t, char vt, const void *b, v
   const void *dinv = vt == 's' ? (const void *)t->dinv_s : (const void *)t->dinv_d;
   for (int l = 0; l < t->nlev_f; l++)
     bn_trsv_level(vt, t->ptr_f[l + 1] - t->ptr_f[l], t->rows_f + t->ptr_f[l], t->lo.ia, t->lo.ja,
-                  opv(&t->lo, vt), dinv, b, y, t->lo.lanes, NULL, NULL, s->stream);
+                  opv(&t->lo, vt), dinv, b, y, bn_lanes(&t->lo, vt), NULL, NULL, s->stream);
   for (int l = 0; l < t->nlev_b; l++)
     bn_trsv_level(vt, t->ptr_b[l + 1] - t->ptr_b[l], t->rows_b + t->ptr_b[l], t->up.ia, t->up.ja,
-                  opv(&t->up, vt), dinv, y, x, t->up.lanes, zbase, zsum, s->stream);
+                  opv(&t->up, vt), dinv, y, x, bn_lanes(&t->up, vt), zbase, zsum, s->stream);
 }
 
 /* k Jacobi sweeps per triangle (approximate IC application; only used in
@@ -177,21 +177,21 @@ static void tri_jacobi(struct bn_solver *s, bn_dtri *t, char vt, const void *b, 
   const int n = t->n;
   void *bufs[2] = {y, tmp};
   int cur = 0;
-  bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, NULL, bufs[cur], t->lo.lanes,
+  bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, NULL, bufs[cur], bn_lanes(&t->lo, vt),
                  s->stream);
   for (int k = 1; k < sweeps; k++) {
     bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, bufs[cur], bufs[1 - cur],
-                   t->lo.lanes, s->stream);
+                   bn_lanes(&t->lo, vt), s->stream);
     cur = 1 - cur;
   }
   const void *yy = bufs[cur];
   void *xb[2] = {x, bufs[1 - cur]};
   int cx = 0;
-  bn_jacobi_rows(vt, n, t->up.ia, t->up.ja, opv(&t->up, vt), dinv, yy, NULL, xb[cx], t->up.lanes,
+  bn_jacobi_rows(vt, n, t->up.ia, t->up.ja, opv(&t->up, vt), dinv, yy, NULL, xb[cx], bn_lanes(&t->up, vt),
                  s->stream);
   for (int k = 1; k < sweeps; k++) {
     bn_jacobi_rows(vt, n, t->up.ia, t->up.ja, opv(&t->up, vt), dinv, yy, xb[cx], xb[1 - cx],
-                   t->up.lanes, s->stream);
+                   bn_lanes(&t->up, vt), s->stream);
     cx = 1 - cx;
   }
   if (xb[cx] != x) bn_copy(vt, n, xb[cx], x, s->stream);

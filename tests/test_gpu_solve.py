@@ -117,17 +117,21 @@ def test_solve_is_deterministic(net30):
 
 
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_graph_replay_equals_eager(net30, precision):
+def test_graph_replay_equals_eager(net30, precision, monkeypatch):
     # every PCG step replays one captured hipGraph; eager launches of the
-    # same kernels must give bit-identical iterates, across several solves
-    # with other work in between
+    # same kernels (BN_NO_GRAPH=1, same solver, so the same autotuned
+    # launch shapes) must give bit-identical iterates, across several
+    # solves with other work in between
     _case(f"lattice30 graph vs eager {precision}")
     kw = dict(coarsening="pmis", min_size=500, precision=precision,
               fine_ic="jacobi" if precision == "fp32" else "exact")
-    sg = AMGSolver(net30.ia, net30.ja, net30.val, graph=True, **kw)
-    se = AMGSolver(net30.ia, net30.ja, net30.val, graph=False, **kw)
-    xe, ie = se.solve(net30.b, tol=1e-10)
+    s = AMGSolver(net30.ia, net30.ja, net30.val, **kw)
+    monkeypatch.setenv("BN_NO_GRAPH", "1")
+    xe, ie = s.solve(net30.b, tol=1e-10)
+    monkeypatch.delenv("BN_NO_GRAPH")
     for rep in range(3):
-        xg, ig = sg.solve(net30.b, tol=1e-10)
+        xg, ig = s.solve(net30.b, tol=1e-10)
         assert ig == ie and np.array_equal(xg, xe)
-        se.solve(net30.b * (rep + 2.0), tol=1e-6)  # eager work between replays
+        monkeypatch.setenv("BN_NO_GRAPH", "1")
+        s.solve(net30.b * (rep + 2.0), tol=1e-6)  # eager work between replays
+        monkeypatch.delenv("BN_NO_GRAPH")
