@@ -86,3 +86,28 @@ def test_setup_bit_exact_lattice60(oracle):
     for c in ("pmis", "hmis"):
         s = AMGSolver(net.ia, net.ja, net.val, coarsening=c)
         compare(s, oracle.build(net.ia, net.ja, net.val, coarsening=c))
+
+
+def test_hmis_on_hub_level_fixture(oracle):
+    """A coarse level whose strength matrix has hub columns (tree nodes):
+    the reference's HMIS bucket array (rs_classical.c:410-418) is too small
+    for the lambdas reached here; the oracle and the library agree on the
+    split the algorithm defines (fixture written by the library)."""
+    import ctypes as C
+    import os
+    from blood_network_amd._lib import CSR, lib as blib, ip, dp
+    d = np.load(os.path.join(os.path.dirname(__file__), "golden", "hmis_hub_level.npz"))
+    ia, ja, val = d["s_ia"], d["s_ja"], d["s_val"]
+    n = len(ia) - 1
+    lam = np.bincount(ja, minlength=n)
+    assert lam.max() > np.diff(ia).max() * 10  # hub columns
+    S = CSR(n, n, len(ja), ip(ia), ip(ja), dp(val))
+    out = {}
+    for name, lib in (("product", blib), ("oracle", oracle.lib)):
+        cf = np.zeros(n, np.int32)
+        nc = C.c_int()
+        lib.hmis_coarsening(C.byref(S), cf.ctypes.data_as(C.POINTER(C.c_int)), C.byref(nc))
+        out[name] = (cf == 1).astype(np.int32)
+        assert nc.value == int(d["mis"].sum())
+    assert np.array_equal(out["product"], out["oracle"])
+    assert np.array_equal(out["product"], d["mis"])
