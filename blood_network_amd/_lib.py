@@ -32,7 +32,8 @@ class Options(C.Structure):
                 ("fine_ic", C.c_int), ("ic_sweeps", C.c_int),
                 ("max_colours", C.c_int), ("flexible", C.c_int),
                 ("stop", C.c_int), ("symmetrize", C.c_int), ("device", C.c_int),
-                ("verbose", C.c_int), ("graph", C.c_int)]
+                ("verbose", C.c_int), ("graph", C.c_int),
+                ("block_smoother", C.c_int), ("chunk_rows", C.c_int)]
 
 
 class SolveInfo(C.Structure):
@@ -128,6 +129,14 @@ def _load():
         "bn_csr_apply": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
         "bn_csr_apply_binned": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, C.c_int,
                                         C.c_int, VP, VP]),
+        "bn_gs_color_map": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
+        "bn_gs_color_zero_map": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP]),
+        "bn_block_gs": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
+                                C.c_int, C.c_int, VP, VP, VP]),
+        "bn_chunk_order": (C.c_int, [C.POINTER(CSR), IP, C.c_int, C.c_int, IP, C.POINTER(IP), C.POINTER(IP),
+                                     C.POINTER(IP)]),
+        "bn_csr_apply_blocks": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP]),
+        "bn_row_blocks": (C.c_int, [IP, C.c_int, C.c_int, C.c_int, IP]),
         "bn_colour_order": (C.c_int, [C.POINTER(CSR), IP, C.c_int, IP, C.POINTER(IP)]),
         "bn_level_sets": (C.c_int, [IP, IP, C.c_int, C.c_int, IP, C.POINTER(IP)]),
         "bn_trsv_level": (None, [C.c_char, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),

@@ -120,6 +120,8 @@ void bn_options_default(bn_options *o) {
   o->coarse_solver = BN_COARSE_AUTO;
   o->fine_ic = BN_IC_EXACT;
   o->ic_sweeps = 3;
+  o->block_smoother = 0;
+  o->chunk_rows = BN_CHUNK_MAX_ROWS;
   o->max_colours = 32;
   o->flexible = 0;
   o->stop = BN_STOP_BNORM;
@@ -159,6 +161,7 @@ void *bn_solver_create(int n, int nnz, const int *ia, const int *ja, const doubl
   if (s->opt.max_levels <= 0 || s->opt.max_levels > AMG_MAX_LEVELS) s->opt.max_levels = AMG_MAX_LEVELS;
   if (s->opt.nu <= 0) s->opt.nu = 1;
   if (s->opt.max_colours <= 0) s->opt.max_colours = 32;
+  if (s->opt.chunk_rows <= 0 || s->opt.chunk_rows > BN_CHUNK_MAX_ROWS) s->opt.chunk_rows = BN_CHUNK_MAX_ROWS;
   if (s->opt.ic_sweeps <= 0) s->opt.ic_sweeps = 3;
   const double t0 = now_ms();
   copy_input(n, nnz, ia, ja, val, s->opt.symmetrize, &s->A_own);

@@ -26,6 +26,10 @@ typedef struct {
   int long_len;    /* rows longer than this go to the block-per-row bin */
   int nlong;       /* number of such rows                              */
   int *long_rows;  /* device list of those rows                        */
+  int nblk;        /* row blocks for the adaptive kernel               */
+  int *blk;        /* device: nblk + 1 row boundaries                  */
+  int adapt;       /* FP64 apply uses the row-block kernel (autotuned) */
+  int adapt_s;     /* FP32 likewise                                    */
 } bn_dcsr;
 
 /* level-scheduled triangular factor L (forward) and L^T (backward) */
@@ -48,11 +52,25 @@ typedef struct {
   double *l1inv_d;      /* 1/sum_j |a_ij| */
   float *l1inv_s;
   int ncol;             /* colours of the multicolour sweep */
-  int *col_ptr;         /* host: colour boundaries (relabelled rows) */
+  int *col_ptr;         /* host: colour boundaries (rows of Ac) */
   int use_gs;           /* colour count allows the multicolour sweep */
+  bn_dcsr Ac;           /* GS levels: A's rows copied in colour order,
+                           columns in the level numbering */
+  int *cmap_d;          /* device: row r of Ac is level row cmap_d[r] */
+  double *dinvc_d;      /* 1/a_ii in Ac row order */
+  float *dinvc_s;
   int *perm_h;          /* host perm[new] = old */
   int *perm_d;
   void *b, *x, *r;      /* level work vectors in V-cycle precision */
+  /* block smoother (chunked layout) */
+  int block;            /* rows chunked; smoothing by bn_block_gs */
+  int nchunk;           /* chunks */
+  int chunk_max;        /* rows of the largest chunk */
+  int *chunk_d;         /* device: nchunk + 1 row starts */
+  int *ccol_d, *cbnd_d; /* device: per-chunk local colour boundaries */
+  double *bdinv_d;      /* 1 / (a_ii + sum_{j outside the chunk} |a_ij|) */
+  float *bdinv_s;
+  void *x2;             /* second buffer for sweeps from a nonzero start */
 } bn_dlevel;
 
 struct bn_solver {

@@ -96,6 +96,19 @@ static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
     CK(hipMemcpy(d->long_rows, lr, (size_t)d->nlong * sizeof(int), hipMemcpyHostToDevice));
     free(lr);
   }
+  /* row blocks of <= BN_ROWBLOCK_NNZ entries for the adaptive kernel */
+  {
+    int *bk = (int *)bn_xmalloc(((size_t)m->row + 1) * sizeof(int));
+    d->nblk = bn_row_blocks(m->ia, m->row, BN_ROWBLOCK_NNZ, BN_ROWBLOCK_ROWS, bk);
+    d->blk = (int *)dmalloc(s, ((size_t)d->nblk + 1) * sizeof(int));
+    if (!d->blk) {
+      free(bk);
+      return BN_ERR_HIP;
+    }
+    CK(hipMemcpy(d->blk, bk, ((size_t)d->nblk + 1) * sizeof(int), hipMemcpyHostToDevice));
+    free(bk);
+    d->adapt = d->adapt_s = 0;
+  }
   float *tmp = (float *)bn_xmalloc((size_t)m->nnz * sizeof(float));
   for (int k = 0; k < m->nnz; k++) tmp[k] = (float)m->val[k];
   CK(hipMemcpy(d->ia, m->ia, ((size_t)m->row + 1) * sizeof(int), hipMemcpyHostToDevice));
@@ -195,10 +208,10 @@ static int upload_tri(struct bn_solver *s, const ichol_factor *F, const int *per
 /* Autotuner: time the operator's SpMV for a few lanes-per-row choices
  * around the heuristic, in both precisions, and keep the fastest.  x and
  * y are scratch device buffers of at least cols / rows doubles. */
-static void autotune_csr(struct bn_solver *s, bn_dcsr *d, void *x, void *y) {
+static void autotune_csr(struct bn_solver *s, bn_dcsr *d, void *x, void *y, int allow_blocks) {
   if (!d->ia || d->rows <= 0 || d->nnz <= 0) return;
   const int base = d->lanes;
-  int cand[6], nc = 0;
+  int cand[7], nc = 0;
   const int opts[6] = {1, base / 4, base / 2, base, base * 2, base * 4};
   for (int k = 0; k < 6; k++) {
     int l = opts[k];
@@ -207,30 +220,61 @@ static void autotune_csr(struct bn_solver *s, bn_dcsr *d, void *x, void *y) {
     for (int q = 0; q < nc; q++) dup |= (cand[q] == l);
     if (!dup) cand[nc++] = l;
   }
+  if (allow_blocks && d->blk) cand[nc++] = 0; /* 0: the row-block kernel */
   for (int pv = 0; pv < 2; pv++) {
     const char vt = pv ? 's' : 'd';
     const void *val = pv ? (const void *)d->vs : (const void *)d->vd;
-    float best = 1e30f;
-    int bl = base;
+    float best = 1e30f, best_lanes_ms = 1e30f;
+    int bl = base, use_blocks = 0;
     for (int k = 0; k < nc; k++) {
       const int reps = 3;
-      bn_csr_apply_binned(vt, BN_OP_SET, d->rows, d->ia, d->ja, val, x, y, NULL, cand[k], d->long_len,
-                          d->nlong, d->long_rows, s->stream);
-      hipEventRecord(s->ev0, s->stream);
-      for (int r = 0; r < reps; r++)
-        bn_csr_apply_binned(vt, BN_OP_SET, d->rows, d->ia, d->ja, val, x, y, NULL, cand[k], d->long_len,
-                            d->nlong, d->long_rows, s->stream);
+      for (int r = -1; r < reps; r++) {
+        if (r == 0) hipEventRecord(s->ev0, s->stream);
+        if (cand[k] > 0)
+          bn_csr_apply_binned(vt, BN_OP_SET, d->rows, d->ia, d->ja, val, x, y, NULL, cand[k], d->long_len,
+                              d->nlong, d->long_rows, s->stream);
+        else
+          bn_csr_apply_blocks(vt, BN_OP_SET, d->nblk, d->blk, d->ia, d->ja, val, x, y, NULL, s->stream);
+      }
       hipEventRecord(s->ev1, s->stream);
       hipEventSynchronize(s->ev1);
       float ms = 0.0f;
       hipEventElapsedTime(&ms, s->ev0, s->ev1);
-      if (ms < best) {
-        best = ms;
+      if (cand[k] > 0 && ms < best_lanes_ms) {
+        best_lanes_ms = ms;
         bl = cand[k];
       }
+      if (ms < best) {
+        best = ms;
+        use_blocks = cand[k] == 0;
+      }
     }
-    if (pv) d->lanes_s = bl;
-    else d->lanes = bl;
+    if (pv) {
+      d->lanes_s = bl;
+      d->adapt_s = use_blocks;
+    } else {
+      d->lanes = bl;
+      d->adapt = use_blocks;
+    }
+  }
+}
+
+/* BN_APPLY in the environment pins the level-operator kernel choice:
+ * "lanes" (lanes-per-row kernel only), "blocks" (row-block kernel for
+ * every level operator), anything else: timed choice. */
+static int apply_mode(void) {
+  const char *e = getenv("BN_APPLY");
+  if (!e) return 0;
+  if (!strcmp(e, "lanes")) return 1;
+  if (!strcmp(e, "blocks")) return 2;
+  return 0;
+}
+
+static void pin_blocks(struct bn_solver *s) {
+  for (int l = 0; l < s->nlev; l++) {
+    bn_dcsr *ops[3] = {&s->lev[l].A, &s->lev[l].P, &s->lev[l].R};
+    for (int k = 0; k < 3; k++)
+      if (ops[k]->blk) ops[k]->adapt = ops[k]->adapt_s = 1;
   }
 }
 
@@ -238,16 +282,19 @@ static void autotune_all(struct bn_solver *s) {
   void *x = s->x, *y = s->q;
   bn_fill('d', s->lev[0].n, x, 1.0, s->stream);
   for (int l = 0; l < s->nlev; l++) {
-    autotune_csr(s, &s->lev[l].A, x, y);
-    autotune_csr(s, &s->lev[l].P, x, y);
-    autotune_csr(s, &s->lev[l].R, x, y);
+    const int allow = apply_mode() == 0;
+    autotune_csr(s, &s->lev[l].A, x, y, allow);
+    autotune_csr(s, &s->lev[l].P, x, y, allow);
+    autotune_csr(s, &s->lev[l].R, x, y, allow);
+    autotune_csr(s, &s->lev[l].Ac, x, y, 0);
   }
-  if (!s->asolve_shared) autotune_csr(s, &s->Asolve, x, y);
+  /* the outer PCG's q = A p fuses p'q into the lanes kernel */
+  if (!s->asolve_shared) autotune_csr(s, &s->Asolve, x, y, 0);
   else s->Asolve = s->lev[0].A;
-  autotune_csr(s, &s->fine_ic.lo, x, y);
-  autotune_csr(s, &s->fine_ic.up, x, y);
-  autotune_csr(s, &s->coarse_ic.lo, x, y);
-  autotune_csr(s, &s->coarse_ic.up, x, y);
+  autotune_csr(s, &s->fine_ic.lo, x, y, 0);
+  autotune_csr(s, &s->fine_ic.up, x, y, 0);
+  autotune_csr(s, &s->coarse_ic.lo, x, y, 0);
+  autotune_csr(s, &s->coarse_ic.up, x, y, 0);
   hipStreamSynchronize(s->stream);
 }
 
@@ -272,6 +319,91 @@ int bn_tri_upload(struct bn_solver *s, const ichol_factor *F, bn_dtri *t) {
 
 int bn_csr_upload(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) { return upload_csr(s, m, d); }
 
+/* Chunks of the block smoother: at most max_rows rows each; at least 256
+ * chunks (one per CU) while chunks keep >= 64 rows; above 256 a multiple
+ * of 256 so every CU gets the same number of chunks. */
+static int chunk_count(int n, int max_rows) {
+  if (n <= 0) return 1;
+  long long nk = ((long long)n + max_rows - 1) / max_rows;
+  if (nk < 256) {
+    long long small = ((long long)n + 63) / 64;
+    nk = small < 256 ? small : 256;
+  } else {
+    nk = (nk + 255) / 256 * 256;
+  }
+  return (int)(nk > 0 ? nk : 1);
+}
+
+/* Upload a level's chunk schedule and its block-smoother diagonal
+ * 1 / (a_ii + sum over columns outside the row's chunk of |a_ij|), the l1
+ * scaling that keeps the hybrid sweep convergent for any SPD operator;
+ * Ap is the level operator in chunked numbering. */
+static int upload_chunks(struct bn_solver *s, bn_dlevel *d, const dcsr_mat *Ap, const int *chunk, const int *ccol,
+                         const int *cbnd) {
+  const int n = Ap->row, nk = d->nchunk;
+  double *bd = (double *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int k = 0; k < nk; k++) {
+    const int lo = chunk[k], hi = chunk[k + 1];
+    for (int i = lo; i < hi; i++) {
+      double dg = 0.0, off = 0.0;
+      int has = 0;
+      for (int q = Ap->ia[i]; q < Ap->ia[i + 1]; q++) {
+        const int j = Ap->ja[q];
+        if (j == i) {
+          dg = Ap->val[q];
+          has = 1;
+        } else if (j < lo || j >= hi) {
+          off += fabs(Ap->val[q]);
+        }
+      }
+      if (!has) dg = 1.0;
+      bd[i] = 1.0 / (dg + off);
+    }
+  }
+  int rc = upload_vec2(s, bd, n, &d->bdinv_d, &d->bdinv_s);
+  free(bd);
+  if (rc) return rc;
+  d->chunk_d = (int *)dmalloc(s, ((size_t)nk + 1) * sizeof(int));
+  d->ccol_d = (int *)dmalloc(s, ((size_t)nk + 1) * sizeof(int));
+  d->cbnd_d = (int *)dmalloc(s, ((size_t)ccol[nk] + 1) * sizeof(int));
+  if (!d->chunk_d || !d->ccol_d || !d->cbnd_d) {
+    bn_set_error("device out of memory (chunk schedule)");
+    return BN_ERR_HIP;
+  }
+  CK(hipMemcpy(d->chunk_d, chunk, ((size_t)nk + 1) * sizeof(int), hipMemcpyHostToDevice));
+  CK(hipMemcpy(d->ccol_d, ccol, ((size_t)nk + 1) * sizeof(int), hipMemcpyHostToDevice));
+  CK(hipMemcpy(d->cbnd_d, cbnd, (size_t)ccol[nk] * sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
+/* Colour-ordered row copy of a GS level: row r of Ac is row cperm[r] of
+ * Ap with its columns unchanged, plus the row map and 1/a_ii in that
+ * order. */
+static int upload_colour_rows(struct bn_solver *s, bn_dlevel *d, const dcsr_mat *Ap, const int *cperm) {
+  const int n = Ap->row;
+  dcsr_mat Ac;
+  bn_csr_permute(Ap, cperm, NULL, &Ac);
+  int rc = upload_csr(s, &Ac, &d->Ac);
+  double *dc = (double *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(double));
+  /* row r of Ac holds the diagonal of level row cperm[r]; 1.0 when
+   * absent, as diag_inverses */
+  for (int r = 0; r < n; r++) {
+    double dg = 1.0;
+    for (int q = Ac.ia[r]; q < Ac.ia[r + 1]; q++)
+      if (Ac.ja[q] == cperm[r]) dg = Ac.val[q];
+    dc[r] = 1.0 / dg;
+  }
+  if (!rc) rc = upload_vec2(s, dc, n, &d->dinvc_d, &d->dinvc_s);
+  free(dc);
+  bn_csr_free(&Ac);
+  if (rc) return rc;
+  d->cmap_d = (int *)dmalloc(s, (size_t)(n > 0 ? n : 1) * sizeof(int));
+  if (!d->cmap_d) return BN_ERR_HIP;
+  CK(hipMemcpy(d->cmap_d, cperm, (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+  return 0;
+}
+
 int bn_device_upload(struct bn_solver *s) {
   const double t0 = now_ms();
   if (!bn_device_ok()) {
@@ -291,6 +423,7 @@ int bn_device_upload(struct bn_solver *s) {
   s->nlev = nlev;
   const int split = 1; /* C-then-F colours serve both GS smoothers */
   int *inv[AMG_MAX_LEVELS];
+  int *chunk_h[AMG_MAX_LEVELS], *ccol_h[AMG_MAX_LEVELS], *cbnd_h[AMG_MAX_LEVELS], *cperm_h[AMG_MAX_LEVELS];
   int rc = 0;
 
   /* colour orders and inverse permutations */
@@ -301,14 +434,26 @@ int bn_device_upload(struct bn_solver *s) {
     d->n = n;
     d->nc = (l < nlev - 1) ? L->n_coarse : 0;
     d->perm_h = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
-    if (l < nlev - 1) {
-      d->ncol = bn_colour_order(&L->A, L->mis, split, d->perm_h, &d->col_ptr);
+    chunk_h[l] = ccol_h[l] = cbnd_h[l] = cperm_h[l] = NULL;
+    if (l < nlev - 1 && s->opt.block_smoother) {
+      /* chunked layout: natural order kept at the chunk scale */
+      d->block = 1;
+      d->use_gs = 0;
+      d->ncol = 0;
+      d->col_ptr = NULL;
+      d->nchunk = bn_chunk_order(&L->A, L->mis, split, chunk_count(n, s->opt.chunk_rows), d->perm_h,
+                                 &chunk_h[l], &ccol_h[l], &cbnd_h[l]);
+      d->chunk_max = 0;
+      for (int k = 0; k < d->nchunk; k++)
+        if (chunk_h[l][k + 1] - chunk_h[l][k] > d->chunk_max) d->chunk_max = chunk_h[l][k + 1] - chunk_h[l][k];
+    } else if (l < nlev - 1) {
+      /* the level keeps its own (natural) numbering, whose neighbouring
+       * rows share x cache lines; a GS level also gets its rows copied
+       * in colour order for the sweep kernels */
+      cperm_h[l] = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+      d->ncol = bn_colour_order(&L->A, L->mis, split, cperm_h[l], &d->col_ptr);
       d->use_gs = d->ncol <= s->opt.max_colours;
-      if (!d->use_gs) {
-        /* l1-Jacobi level: keep the natural order, whose neighbouring
-         * rows share x cache lines (colour order would scatter them) */
-        for (int i = 0; i < n; i++) d->perm_h[i] = i;
-      }
+      for (int i = 0; i < n; i++) d->perm_h[i] = i;
     } else {
       for (int i = 0; i < n; i++) d->perm_h[i] = i;
       d->ncol = 0;
@@ -331,6 +476,8 @@ int bn_device_upload(struct bn_solver *s) {
     diag_inverses(&Ap, dinv, l1inv);
     if (!rc) rc = upload_vec2(s, dinv, n, &d->dinv_d, &d->dinv_s);
     if (!rc) rc = upload_vec2(s, l1inv, n, &d->l1inv_d, &d->l1inv_s);
+    if (!rc && d->block) rc = upload_chunks(s, d, &Ap, chunk_h[l], ccol_h[l], cbnd_h[l]);
+    if (!rc && d->use_gs) rc = upload_colour_rows(s, d, &Ap, cperm_h[l]);
     free(dinv);
     free(l1inv);
     bn_csr_free(&Ap);
@@ -351,6 +498,13 @@ int bn_device_upload(struct bn_solver *s) {
     if (!rc) rc = alloc_vec(s, &d->b, n);
     if (!rc) rc = alloc_vec(s, &d->x, n);
     if (!rc) rc = alloc_vec(s, &d->r, n);
+    if (!rc && d->block) rc = alloc_vec(s, &d->x2, n);
+  }
+  for (int l = 0; l < nlev; l++) {
+    free(chunk_h[l]);
+    free(ccol_h[l]);
+    free(cbnd_h[l]);
+    free(cperm_h[l]);
   }
 
   /* solve matrix: shares level 0 unless set_matrix replaced it */
@@ -414,6 +568,7 @@ int bn_device_upload(struct bn_solver *s) {
   if (!rc) {
     const char *env = getenv("BN_NO_AUTOTUNE");
     if (!(env && env[0] && env[0] != '0')) autotune_all(s);
+    if (apply_mode() == 2) pin_blocks(s);
     CK(hipDeviceSynchronize());
     s->dev_ready = 1;
   }

@@ -110,10 +110,22 @@ int bn_solver_time_kernel(void *h, int which, int reps, double *ms_avg, double *
         break;
       case 1: {
         const void *val = vt == 's' ? (const void *)d->A.vs : (const void *)d->A.vd;
-        const void *dinv = vt == 's' ? (const void *)d->dinv_s : (const void *)d->dinv_d;
-        for (int c = 0; c < d->ncol; c++)
-          bn_gs_color(vt, d->col_ptr[c], d->col_ptr[c + 1], d->A.ia, d->A.ja, val, dinv, s->r1, s->u1,
+        if (!d->block && !d->use_gs) {
+          /* l1-Jacobi level: residual + update */
+          bn_csr_apply_binned(vt, BN_OP_RESID, n, d->A.ia, d->A.ja, val, s->u1, d->r, s->r1, bn_lanes(&d->A, vt),
+                              d->A.long_len, d->A.nlong, d->A.long_rows, st);
+          bn_jacobi_update(vt, n, d->r, vt == 's' ? (const void *)d->l1inv_s : (const void *)d->l1inv_d, 1.0,
+                           s->u1, st);
+        } else if (d->block)
+          bn_block_gs(vt, d->nchunk, d->chunk_max, d->chunk_d, d->ccol_d, d->cbnd_d, d->A.ia, d->A.ja, val,
+                      vt == 's' ? (const void *)d->bdinv_s : (const void *)d->bdinv_d, s->r1, s->u1, d->x2, 0,
                       bn_lanes(&d->A, vt), NULL, NULL, st);
+        else
+          for (int c = 0; c < d->ncol; c++)
+            bn_gs_color_map(vt, d->col_ptr[c], d->col_ptr[c + 1], d->cmap_d, d->Ac.ia, d->Ac.ja,
+                            vt == 's' ? (const void *)d->Ac.vs : (const void *)d->Ac.vd,
+                            vt == 's' ? (const void *)d->dinvc_s : (const void *)d->dinvc_d, s->r1, s->u1,
+                            bn_lanes(&d->Ac, vt), NULL, NULL, st);
         b = (double)d->A.nnz * (4.0 + v) + (n + 1.0) * 4.0 + 3.0 * n * v + n * v; /* + dinv */
         break;
       }
@@ -131,6 +143,10 @@ int bn_solver_time_kernel(void *h, int which, int reps, double *ms_avg, double *
       case 4:
         bn_vc_ichol_precond(s, s->r, s->z);
         b = 0.0;
+        break;
+      case 5: /* plain FP64 copy of n values: the byte-counter calibration */
+        bn_copy('d', n, s->p, s->q, st);
+        b = 16.0 * n;
         break;
       default:
         return BN_ERR_ARG;

@@ -75,6 +75,82 @@ int bn_colour_order(const dcsr_mat *A, const int *mis, int split, int *perm, int
   return ncol;
 }
 
+int bn_chunk_order(const dcsr_mat *A, const int *mis, int split, int nchunks, int *perm, int **chunk_ptr,
+                   int **ccol_ptr, int **cbnd) {
+  const int n = A->row;
+  if (nchunks < 1) nchunks = 1;
+  const long long S = n > 0 ? ((long long)n + nchunks - 1) / nchunks : 1;
+  const int nk = n > 0 ? (int)(((long long)n + S - 1) / S) : 0;
+  int *cp = (int *)bn_xmalloc(((size_t)nk + 1) * sizeof(int));
+  for (int k = 0; k <= nk; k++) cp[k] = (int)((long long)k * S < n ? (long long)k * S : n);
+  int *col = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  int *nck = (int *)bn_xmalloc((size_t)(nk > 0 ? nk : 1) * sizeof(int));
+  const int use_split = split && mis;
+  for (int i = 0; i < n; i++) col[i] = -1;
+#pragma omp parallel
+  {
+    int cap = 64;
+    int *stamp = (int *)bn_xmalloc((size_t)cap * sizeof(int));
+    for (int c = 0; c < cap; c++) stamp[c] = -1;
+#pragma omp for schedule(dynamic, 16)
+    for (int k = 0; k < nk; k++) {
+      const int lo = cp[k], hi = cp[k + 1];
+      int nclass[2] = {0, 0};
+      for (int cl = 0; cl < (use_split ? 2 : 1); cl++) {
+        int ncol = 0;
+        for (int q = lo; q < hi; q++) {
+          if (use_split && (mis[q] ? 0 : 1) != cl) continue;
+          for (int e = A->ia[q]; e < A->ia[q + 1]; e++) {
+            const int j = A->ja[e];
+            if (j < lo || j >= hi || j == q || col[j] < 0) continue;
+            if (use_split && (mis[j] ? 0 : 1) != cl) continue;
+            stamp[col[j]] = q;
+          }
+          int c = 0;
+          while (c < cap && stamp[c] == q) c++;
+          if (c == cap) {
+            const int nc2 = cap * 2;
+            stamp = (int *)realloc(stamp, (size_t)nc2 * sizeof(int));
+            for (int t = cap; t < nc2; t++) stamp[t] = -1;
+            cap = nc2;
+          }
+          col[q] = c;
+          if (c + 1 > ncol) ncol = c + 1;
+        }
+        nclass[cl] = ncol;
+      }
+      if (use_split)
+        for (int q = lo; q < hi; q++)
+          if (!mis[q]) col[q] += nclass[0];
+      nck[k] = nclass[0] + nclass[1];
+    }
+    free(stamp);
+  }
+  int *cc = (int *)bn_xmalloc(((size_t)nk + 1) * sizeof(int));
+  cc[0] = 0;
+  for (int k = 0; k < nk; k++) cc[k + 1] = cc[k] + nck[k] + 1;
+  int *bnd = (int *)bn_xmalloc((size_t)(cc[nk] > 0 ? cc[nk] : 1) * sizeof(int));
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int k = 0; k < nk; k++) {
+    const int lo = cp[k], hi = cp[k + 1], nc = nck[k];
+    int *b = bnd + cc[k];
+    for (int c = 0; c <= nc; c++) b[c] = 0;
+    for (int q = lo; q < hi; q++) b[col[q] + 1]++;
+    b[0] = lo;
+    for (int c = 0; c < nc; c++) b[c + 1] += b[c];
+    /* b[c] is now the first slot of colour c; fill, then restore */
+    for (int q = lo; q < hi; q++) perm[b[col[q]]++] = q;
+    for (int c = nc; c > 0; c--) b[c] = b[c - 1];
+    b[0] = lo;
+  }
+  free(col);
+  free(nck);
+  *chunk_ptr = cp;
+  *ccol_ptr = cc;
+  *cbnd = bnd;
+  return nk;
+}
+
 int bn_level_sets(const int *ia, const int *ja, int n, int lower, int *rows, int **lev_ptr) {
   int *depth = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
   int nlev = 0;
@@ -130,4 +206,23 @@ void bn_csr_permute(const dcsr_mat *A, const int *perm_r, const int *inv_c, dcsr
       B->val[p] = A->val[q];
     }
   }
+}
+
+int bn_row_blocks(const int *ia, int n, int max_nnz, int max_rows, int *blocks) {
+  int nb = 0, r = 0;
+  blocks[0] = 0;
+  while (r < n) {
+    if (ia[r + 1] - ia[r] > max_nnz) {
+      blocks[++nb] = ++r;
+      continue;
+    }
+    const int r0 = r;
+    long long acc = 0;
+    while (r < n && r - r0 < max_rows && acc + (ia[r + 1] - ia[r]) <= max_nnz) {
+      acc += ia[r + 1] - ia[r];
+      r++;
+    }
+    blocks[++nb] = r;
+  }
+  return nb;
 }

@@ -21,6 +21,26 @@ int bn_colour_order(const dcsr_mat *A, const int *mis, int split, int *perm, int
  * *lev_ptr (malloc'ed) delimits levels.  Returns the level count. */
 int bn_level_sets(const int *ia, const int *ja, int n, int lower, int *rows, int **lev_ptr);
 
+/* Chunked colour order for the block Gauss-Seidel smoother.  The rows are
+ * cut into nchunks contiguous ranges of the current numbering (chunk k =
+ * rows [k*S, min(n, (k+1)*S)), S = ceil(n / nchunks)); inside each chunk
+ * the rows are coloured greedily over the chunk's own couplings (C points
+ * first, then F points, when split != 0) and listed colour by colour,
+ * ascending within a colour.  perm[new] = old never leaves a chunk.
+ * Outputs (malloc'ed): chunk_ptr (nchunks+1 row starts), ccol_ptr
+ * (nchunks+1 offsets into cbnd), cbnd (chunk k's colour boundaries are
+ * cbnd[ccol_ptr[k] .. ccol_ptr[k+1]-1], first = chunk start, last = chunk
+ * end).  Returns the number of chunks actually used (<= nchunks). */
+int bn_chunk_order(const dcsr_mat *A, const int *mis, int split, int nchunks, int *perm, int **chunk_ptr,
+                   int **ccol_ptr, int **cbnd);
+
+/* Row blocks for the adaptive CSR kernel: blocks[0] = 0 < blocks[1] < ...
+ * < blocks[nblk] = n.  Consecutive rows are packed while the block holds
+ * at most max_nnz entries and max_rows rows; a row with more than max_nnz
+ * entries forms a block alone.  blocks must hold n + 1 ints.  Returns
+ * nblk. */
+int bn_row_blocks(const int *ia, int n, int max_nnz, int max_rows, int *blocks);
+
 /* B = A with rows taken in perm_r order and columns renamed by inv_c
  * (either may be NULL for identity); entry order within a row is kept. */
 void bn_csr_permute(const dcsr_mat *A, const int *perm_r, const int *inv_c, dcsr_mat *B);

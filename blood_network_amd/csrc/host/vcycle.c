@@ -31,8 +31,34 @@ static inline const void *vecv(const double *vd, const float *vs, char vt) {
 
 /* y = op(M, x [, b]) in precision vt */
 static void apply(struct bn_solver *s, const bn_dcsr *M, int op, const void *x, void *y, const void *b) {
+  if (s->vt == 's' ? M->adapt_s : M->adapt) {
+    bn_csr_apply_blocks(s->vt, op, M->nblk, M->blk, M->ia, M->ja, opv(M, s->vt), x, y, b, s->stream);
+    return;
+  }
   bn_csr_apply_binned(s->vt, op, M->rows, M->ia, M->ja, opv(M, s->vt), x, y, b, bn_lanes(M, s->vt), M->long_len,
                       M->nlong, M->long_rows, s->stream);
+}
+
+/* Block smoother: nu sweeps that start from xin (NULL: zero) and end in
+ * x.  Sweep k writes x when nu-1-k is even, else the level's x2, so xin
+ * must be block_input(): x2 for odd nu, x for even nu (never the buffer
+ * the first sweep writes). */
+static void *block_input(const bn_dlevel *d, int nu, void *x) { return (nu & 1) ? d->x2 : x; }
+
+static void smooth_block(struct bn_solver *s, int lev, const void *b, const void *xin, void *x, int backward,
+                         const void *zbase, void *zsum) {
+  bn_dlevel *d = &s->lev[lev];
+  const char vt = s->vt;
+  const int nu = s->opt.nu > 0 ? s->opt.nu : 1;
+  const void *src = xin;
+  for (int k = 0; k < nu; k++) {
+    void *dst = ((nu - 1 - k) % 2 == 0) ? x : d->x2;
+    const int last = (k == nu - 1);
+    bn_block_gs(vt, d->nchunk, d->chunk_max, d->chunk_d, d->ccol_d, d->cbnd_d, d->A.ia, d->A.ja, opv(&d->A, vt),
+                vecv(d->bdinv_d, d->bdinv_s, vt), b, src, dst, backward, bn_lanes(&d->A, vt),
+                last ? zbase : NULL, last ? zsum : NULL, s->stream);
+    src = dst;
+  }
 }
 
 /* nu smoothing sweeps on level `lev`; backward runs colours in reverse.
@@ -44,6 +70,16 @@ static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int bac
   const char vt = s->vt;
   const int nu = s->opt.nu > 0 ? s->opt.nu : 1;
   const int n = d->n;
+  if (d->block) {
+    if (x_zero) {
+      smooth_block(s, lev, b, NULL, x, backward, zbase, zsum);
+    } else {
+      void *in = block_input(d, nu, x);
+      if (in != x) bn_copy(vt, n, x, in, s->stream);
+      smooth_block(s, lev, b, in, x, backward, zbase, zsum);
+    }
+    return;
+  }
   if (x_zero) bn_fill(vt, n, x, 0.0, s->stream);
   const int gs = d->use_gs && (s->opt.smoother == SMOOTHER_CF_GS || s->opt.smoother == SMOOTHER_GS);
   if (gs) {
@@ -52,12 +88,13 @@ static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int bac
       for (int t = 0; t < d->ncol; t++) {
         const int c = backward ? d->ncol - 1 - t : t;
         const int lo = d->col_ptr[c], hi = d->col_ptr[c + 1];
+        const void *dinvc = vecv(d->dinvc_d, d->dinvc_s, vt);
         if (sw == 0 && t == 0 && x_zero && !(last && zsum)) {
-          bn_gs_color_zero(vt, lo, hi, vecv(d->dinv_d, d->dinv_s, vt), b, x, s->stream);
+          bn_gs_color_zero_map(vt, lo, hi, d->cmap_d, dinvc, b, x, s->stream);
           continue;
         }
-        bn_gs_color(vt, lo, hi, d->A.ia, d->A.ja, opv(&d->A, vt), vecv(d->dinv_d, d->dinv_s, vt), b, x,
-                    bn_lanes(&d->A, vt), last ? zbase : NULL, last ? zsum : NULL, s->stream);
+        bn_gs_color_map(vt, lo, hi, d->cmap_d, d->Ac.ia, d->Ac.ja, opv(&d->Ac, vt), dinvc, b, x,
+                        bn_lanes(&d->Ac, vt), last ? zbase : NULL, last ? zsum : NULL, s->stream);
       }
     }
   } else {
@@ -116,6 +153,13 @@ static void fwdslash_rec(struct bn_solver *s, int lev, const void *b, void *x, i
     apply(s, &d->R, BN_OP_SET, d->r, c->b, NULL);
   }
   fwdslash_rec(s, lev + 1, c->b, c->x, 1, NULL, NULL);
+  if (d->block && x_zero) {
+    /* interpolate straight into the buffer the first sweep reads */
+    void *in = block_input(d, s->opt.nu > 0 ? s->opt.nu : 1, x);
+    apply(s, &d->P, BN_OP_SET, c->x, in, NULL);
+    smooth_block(s, lev, b, in, x, 1, zbase, zsum);
+    return;
+  }
   apply(s, &d->P, x_zero ? BN_OP_SET : BN_OP_ADD, c->x, x, NULL);
   smooth(s, lev, b, x, 1, 0, zbase, zsum);
 }

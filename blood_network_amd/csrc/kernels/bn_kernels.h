@@ -55,6 +55,22 @@ void bn_csr_apply_binned(char vt, int op, int n, const int *ia, const int *ja,
                          int lanes, int long_len, int nlong, const int *long_rows,
                          hipStream_t stream);
 
+/* Row-block split for bn_csr_apply_blocks: consecutive rows grouped so a
+ * block holds at most BN_ROWBLOCK_NNZ entries and BN_ROWBLOCK_ROWS rows; a
+ * row longer than BN_ROWBLOCK_NNZ is a block of its own. */
+#define BN_ROWBLOCK_NNZ 2048
+#define BN_ROWBLOCK_ROWS 256
+
+/* Same operator, one 256-thread block per row block (blocks[0..nblk],
+ * row indices): the block's entries are streamed with coalesced loads,
+ * their products staged in LDS and summed per row by a power-of-two
+ * group of threads; a single over-long row is summed by the whole block.
+ * Load balance is independent of the row-length distribution. */
+void bn_csr_apply_blocks(char vt, int op, int nblk, const int *blocks,
+                         const int *ia, const int *ja, const void *val,
+                         const void *x, void *y, const void *b,
+                         hipStream_t stream);
+
 /* One multicolour Gauss-Seidel colour: rows lo..hi-1 of a level whose
  * rows are stored in colour order:
  *   x_i <- (b_i - sum_{j != i} a_ij x_j) * dinv_i.
@@ -63,6 +79,29 @@ void bn_gs_color(char vt, int lo, int hi, const int *ia, const int *ja,
                  const void *val, const void *dinv, const void *b, void *x,
                  int lanes, const void *zbase, void *zsum,
                  hipStream_t stream);
+/* One block Gauss-Seidel sweep (l1 hybrid) over a chunked level: chunk
+ * k = rows [chunk[k], chunk[k+1]), its local colours delimited by
+ * cbnd[ccol[k] .. ccol[k+1]-1]; xold = NULL means a zero start.  xnew
+ * must not alias xold.  max_rows (largest chunk) sizes the LDS copy of
+ * the chunk's iterate and must be <= 8192.  zsum: as bn_gs_color. */
+void bn_block_gs(char vt, int nchunk, int max_rows, const int *chunk, const int *ccol,
+                 const int *cbnd, const int *ia, const int *ja, const void *val,
+                 const void *dinv, const void *b, const void *xold, void *xnew,
+                 int backward, int lanes, const void *zbase, void *zsum,
+                 hipStream_t stream);
+#define BN_CHUNK_MAX_ROWS 8192
+
+/* The same colour with the level's rows copied in colour order (row r of
+ * ia/ja/val, columns in natural numbering) updating natural entry
+ * i = map[r]: x_i <- (b_i - sum_{j != i} a_rj x_j) * dinv_r, dinv in
+ * colour order.  Vectors stay in natural order. */
+void bn_gs_color_map(char vt, int lo, int hi, const int *map, const int *ia, const int *ja,
+                     const void *val, const void *dinv, const void *b, void *x,
+                     int lanes, const void *zbase, void *zsum, hipStream_t stream);
+/* first colour from x = 0 in that layout: x_map[r] = b_map[r] * dinv_r */
+void bn_gs_color_zero_map(char vt, int lo, int hi, const int *map, const void *dinv,
+                          const void *b, void *x, hipStream_t stream);
+
 /* first colour of a sweep that starts from x = 0: x_i = b_i * dinv_i */
 void bn_gs_color_zero(char vt, int lo, int hi, const void *dinv,
                       const void *b, void *x, hipStream_t stream);

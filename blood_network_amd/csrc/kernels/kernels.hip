@@ -218,10 +218,96 @@ void launch_long(int op, int nlong, const int *rows, const int *ia, const int *j
 }
 
 // ---------------------------------------------------------------------
-// Gauss-Seidel colour / triangular level: rows either a contiguous range
-// [lo, hi) or listed in rows[].  y_i = (b_i - sum_{j != i} a_ij y_j) dinv_i
+// Row-block CSR apply (adaptive): each block owns a run of consecutive
+// rows holding <= BN_ROWBLOCK_NNZ entries (stream mode) or one longer row
+// (vector mode).  Stream mode reads the run's ja/val as one contiguous,
+// fully coalesced segment, stages val * x[ja] in LDS, then sums each row
+// with tpr = pow2 <= 256 / rows threads.  Fixed summation order per row.
 // ---------------------------------------------------------------------
-template <typename T, int LANES, bool LIST>
+template <typename T, int OP>
+__global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__restrict__ blocks,
+                                                      const int *__restrict__ ia, const int *__restrict__ ja,
+                                                      const T *__restrict__ val, const T *__restrict__ x,
+                                                      T *__restrict__ y, const T *__restrict__ b) {
+  constexpr int PER = BN_ROWBLOCK_NNZ / BLOCK;
+  __shared__ T prod[BN_ROWBLOCK_NNZ];
+  const int bid = logical_block();
+  if (bid >= nblk) return;
+  const int r0 = blocks[bid], r1 = blocks[bid + 1];
+  const int q0 = ia[r0], q1 = ia[r1];
+  const int cnt = q1 - q0;
+  if (cnt > BN_ROWBLOCK_NNZ) {
+    // vector mode: one long row (r1 == r0 + 1)
+    const T s = row_dot<T, BLOCK, false>(q0 + threadIdx.x, q1, ja, val, x, r0);
+    const double t = block_sum((double)s);
+    if (threadIdx.x == 0) {
+      if (OP == BN_OP_SET) y[r0] = (T)t;
+      else if (OP == BN_OP_RESID) y[r0] = b[r0] - (T)t;
+      else y[r0] = y[r0] + (T)t;
+    }
+    return;
+  }
+  int jj[PER];
+  T aa[PER];
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int k = threadIdx.x + u * BLOCK;
+    jj[u] = k < cnt ? ja[q0 + k] : 0;
+    aa[u] = k < cnt ? val[q0 + k] : T(0);
+  }
+#pragma unroll
+  for (int u = 0; u < PER; u++) {
+    const int k = threadIdx.x + u * BLOCK;
+    if (k < cnt) prod[k] = aa[u] * x[jj[u]];
+  }
+  __syncthreads();
+  // tpr = largest power of two <= 256 / rows, at most one wave
+  const int nr = r1 - r0;
+  int lg = nr >= BLOCK ? 0 : 31 - __clz(BLOCK / nr);
+  if (lg > 6) lg = 6;
+  const int tpr = 1 << lg;
+  const int sub = threadIdx.x & (tpr - 1);
+  for (int rb = r0; rb < r1; rb += BLOCK >> lg) {
+    const int r = rb + (threadIdx.x >> lg);
+    T s = 0;
+    if (r < r1) {
+      const int e = ia[r + 1] - q0;
+      for (int k = ia[r] - q0 + sub; k < e; k += tpr) s += prod[k];
+    }
+    for (int o = tpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (sub == 0 && r < r1) {
+      if (OP == BN_OP_SET) y[r] = s;
+      else if (OP == BN_OP_RESID) y[r] = b[r] - s;
+      else y[r] = y[r] + s;
+    }
+  }
+}
+
+template <typename T>
+void launch_blocks(int op, int nblk, const int *blocks, const int *ia, const int *ja, const T *val, const T *x,
+                   T *y, const T *b, hipStream_t st) {
+  if (nblk <= 0) return;
+  const int g = grid_rows(nblk);
+  if (op == BN_OP_SET)
+    hipLaunchKernelGGL((k_csr_blocks<T, BN_OP_SET>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, ia, ja, val, x, y, b);
+  else if (op == BN_OP_RESID)
+    hipLaunchKernelGGL((k_csr_blocks<T, BN_OP_RESID>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, ia, ja, val, x, y, b);
+  else
+    hipLaunchKernelGGL((k_csr_blocks<T, BN_OP_ADD>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, ia, ja, val, x, y, b);
+}
+
+// ---------------------------------------------------------------------
+// Gauss-Seidel colour / triangular level.  Row r of the stored matrix
+// updates vector entry i:
+//   RS_RANGE  r = i = lo + k            (rows stored in sweep order)
+//   RS_LIST   r = i = rows[k]           (level-scheduled triangle)
+//   RS_MAP    r = lo + k, i = rows[r]   (rows copied in colour order,
+//                                        columns and vectors natural)
+// y_i = (b_i - sum_{j != i} a_rj y_j) dinv_r   (dinv_r indexed like r)
+// ---------------------------------------------------------------------
+enum { RS_RANGE = 0, RS_LIST = 1, RS_MAP = 2 };
+
+template <typename T, int LANES, int MODE>
 __global__ __launch_bounds__(BLOCK) void k_rowsolve(int lo, int cnt, const int *__restrict__ rows,
                                                     const int *__restrict__ ia, const int *__restrict__ ja,
                                                     const T *__restrict__ val, const T *__restrict__ dinv,
@@ -231,22 +317,23 @@ __global__ __launch_bounds__(BLOCK) void k_rowsolve(int lo, int cnt, const int *
   const int k = logical_block() * RPB + threadIdx.x / LANES;
   const int sub = threadIdx.x % LANES;
   if (k >= cnt) return;
-  const int i = LIST ? rows[k] : lo + k;
-  T s = row_dot<T, LANES, true>(ia[i] + sub, ia[i + 1], ja, val, y, i);
+  const int r = MODE == RS_LIST ? rows[k] : lo + k;
+  const int i = MODE == RS_MAP ? rows[r] : r;
+  T s = row_dot<T, LANES, true>(ia[r] + sub, ia[r + 1], ja, val, y, i);
   if (LANES > 1) s = group_sum<LANES>(s);
   if (sub == 0) {
-    const T v = (b[i] - s) * dinv[i];
+    const T v = (b[i] - s) * dinv[r];
     y[i] = v;
     if (zsum) zsum[i] = zbase[i] + v;
   }
 }
 
-template <typename T, bool LIST>
+template <typename T, int MODE>
 void dispatch_rowsolve(int lo, int cnt, const int *rows, const int *ia, const int *ja, const T *val,
                        const T *dinv, const T *b, T *y, int lanes, const T *zb, T *zs, hipStream_t st) {
   if (cnt <= 0) return;
 #define BN_RS(L)                                                                                   \
-  hipLaunchKernelGGL((k_rowsolve<T, L, LIST>), dim3(grid_rows((cnt + BLOCK / L - 1) / (BLOCK / L))), dim3(BLOCK), \
+  hipLaunchKernelGGL((k_rowsolve<T, L, MODE>), dim3(grid_rows((cnt + BLOCK / L - 1) / (BLOCK / L))), dim3(BLOCK), \
                      0, st, lo, cnt, rows, ia, ja, val, dinv, b, y, zb, zs)
   switch (lanes) {
     case 1: BN_RS(1); break;
@@ -258,6 +345,114 @@ void dispatch_rowsolve(int lo, int cnt, const int *rows, const int *ia, const in
     default: BN_RS(64); break;
   }
 #undef BN_RS
+}
+
+// ---------------------------------------------------------------------
+// Block Gauss-Seidel (l1 hybrid): one workgroup per chunk of consecutive
+// rows.  The chunk's iterate lives in LDS for the whole sweep; the chunk's
+// local colours are swept in order with workgroup barriers between them,
+// so couplings inside the chunk see Gauss-Seidel values and couplings to
+// other chunks read the previous iterate xold (never written here), which
+// makes the sweep one launch and independent of workgroup timing.  Row
+// update in residual form, x_i += (b_i - sum_j a_ij x~_j) * dinv_i with
+// dinv_i = 1 / (a_ii + sum of |a_ij| over columns outside the chunk), so a
+// sweep is x + M^{-1} (b - A x) with M = D1 + L_in (forward) or D1 + U_in
+// (backward) = M_forward^T: the V-cycle stays symmetric.
+// ---------------------------------------------------------------------
+#define GS_BLOCK 512
+
+template <typename T, bool ZERO>
+__device__ __forceinline__ T chunk_x(int j, const T *xs, const T *__restrict__ xold, int lo, unsigned S) {
+  const unsigned jl = (unsigned)(j - lo);
+  if (jl < S) return xs[jl];
+  return ZERO ? T(0) : xold[j];
+}
+
+template <typename T, int LANES, bool ZERO>
+__device__ __forceinline__ T chunk_row_dot(int q, const int e, const int *__restrict__ ja,
+                                           const T *__restrict__ val, const T *xs, const T *__restrict__ xold,
+                                           int lo, unsigned S) {
+  T s0 = 0, s1 = 0;
+  for (; q + 3 * LANES < e; q += 4 * LANES) {
+    const int j0 = ja[q], j1 = ja[q + LANES], j2 = ja[q + 2 * LANES], j3 = ja[q + 3 * LANES];
+    const T a0 = val[q], a1 = val[q + LANES], a2 = val[q + 2 * LANES], a3 = val[q + 3 * LANES];
+    const T x0 = chunk_x<T, ZERO>(j0, xs, xold, lo, S);
+    const T x1 = chunk_x<T, ZERO>(j1, xs, xold, lo, S);
+    const T x2 = chunk_x<T, ZERO>(j2, xs, xold, lo, S);
+    const T x3 = chunk_x<T, ZERO>(j3, xs, xold, lo, S);
+    s0 += a0 * x0 + a1 * x1;
+    s1 += a2 * x2 + a3 * x3;
+  }
+  for (; q < e; q += LANES) s0 += val[q] * chunk_x<T, ZERO>(ja[q], xs, xold, lo, S);
+  return s0 + s1;
+}
+
+template <typename T, int LANES, bool BACKWARD, bool ZERO>
+__global__ __launch_bounds__(GS_BLOCK) void k_block_gs(const int *__restrict__ chunk, const int *__restrict__ ccol,
+                                                       const int *__restrict__ cbnd, const int *__restrict__ ia,
+                                                       const int *__restrict__ ja, const T *__restrict__ val,
+                                                       const T *__restrict__ dinv, const T *__restrict__ b,
+                                                       const T *__restrict__ xold, T *__restrict__ xnew,
+                                                       const T *__restrict__ zbase, T *__restrict__ zsum) {
+  extern __shared__ double bn_smem[]; /* 8-byte aligned for either T */
+  T *xs = reinterpret_cast<T *>(bn_smem);
+  const int k = logical_block();
+  const int lo = chunk[k], hi = chunk[k + 1];
+  const unsigned S = (unsigned)(hi - lo);
+  for (int i = threadIdx.x; i < (int)S; i += GS_BLOCK) xs[i] = ZERO ? T(0) : xold[lo + i];
+  __syncthreads();
+  const int c0 = ccol[k], nc = ccol[k + 1] - c0 - 1;
+  const int sub = threadIdx.x % LANES;
+  for (int t = 0; t < nc; t++) {
+    const int c = BACKWARD ? nc - 1 - t : t;
+    const int r1 = cbnd[c0 + c + 1];
+    for (int r = cbnd[c0 + c] + (int)threadIdx.x / LANES; r < r1; r += GS_BLOCK / LANES) {
+      // row sum over the whole row, the diagonal on the row's old value
+      T s = chunk_row_dot<T, LANES, ZERO>(ia[r] + sub, ia[r + 1], ja, val, xs, xold, lo, S);
+      if (LANES > 1) s = group_sum<LANES>(s);
+      if (sub == 0) xs[r - lo] += (b[r] - s) * dinv[r];
+    }
+    __syncthreads();
+  }
+  for (int i = threadIdx.x; i < (int)S; i += GS_BLOCK) {
+    const T v = xs[i];
+    xnew[lo + i] = v;
+    if (zsum) zsum[lo + i] = zbase[lo + i] + v;
+  }
+}
+
+template <typename T, int LANES>
+void launch_block_gs(int nchunk, int max_rows, const int *chunk, const int *ccol, const int *cbnd, const int *ia,
+                     const int *ja, const T *val, const T *dinv, const T *b, const T *xold, T *xnew, int backward,
+                     const T *zb, T *zs, hipStream_t st) {
+  const size_t sh = (size_t)max_rows * sizeof(T);
+#define BN_BGS(BW, Z)                                                                                        \
+  hipLaunchKernelGGL((k_block_gs<T, LANES, BW, Z>), dim3(nchunk), dim3(GS_BLOCK), sh, st, chunk, ccol, cbnd, ia, \
+                     ja, val, dinv, b, xold, xnew, zb, zs)
+  if (backward) {
+    if (xold) BN_BGS(true, false);
+    else BN_BGS(true, true);
+  } else {
+    if (xold) BN_BGS(false, false);
+    else BN_BGS(false, true);
+  }
+#undef BN_BGS
+}
+
+template <typename T>
+void dispatch_block_gs(int nchunk, int max_rows, const int *chunk, const int *ccol, const int *cbnd, const int *ia,
+                       const int *ja, const T *val, const T *dinv, const T *b, const T *xold, T *xnew,
+                       int backward, int lanes, const T *zb, T *zs, hipStream_t st) {
+  if (nchunk <= 0) return;
+  switch (lanes) {
+    case 1: launch_block_gs<T, 1>(nchunk, max_rows, chunk, ccol, cbnd, ia, ja, val, dinv, b, xold, xnew, backward, zb, zs, st); break;
+    case 2: launch_block_gs<T, 2>(nchunk, max_rows, chunk, ccol, cbnd, ia, ja, val, dinv, b, xold, xnew, backward, zb, zs, st); break;
+    case 4: launch_block_gs<T, 4>(nchunk, max_rows, chunk, ccol, cbnd, ia, ja, val, dinv, b, xold, xnew, backward, zb, zs, st); break;
+    case 8: launch_block_gs<T, 8>(nchunk, max_rows, chunk, ccol, cbnd, ia, ja, val, dinv, b, xold, xnew, backward, zb, zs, st); break;
+    case 16: launch_block_gs<T, 16>(nchunk, max_rows, chunk, ccol, cbnd, ia, ja, val, dinv, b, xold, xnew, backward, zb, zs, st); break;
+    case 32: launch_block_gs<T, 32>(nchunk, max_rows, chunk, ccol, cbnd, ia, ja, val, dinv, b, xold, xnew, backward, zb, zs, st); break;
+    default: launch_block_gs<T, 64>(nchunk, max_rows, chunk, ccol, cbnd, ia, ja, val, dinv, b, xold, xnew, backward, zb, zs, st); break;
+  }
 }
 
 template <typename T, int LANES>
@@ -290,6 +485,16 @@ void dispatch_jacobi_rows(int n, const int *ia, const int *ja, const T *val, con
     default: BN_JR(64); break;
   }
 #undef BN_JR
+}
+
+// first colour from zero in RS_MAP layout: x_i = b_i dinv_r, i = map[r]
+template <typename T>
+__global__ void k_diag_scale_map(int lo, int cnt, const int *__restrict__ map, const T *__restrict__ dinv,
+                                 const T *__restrict__ b, T *__restrict__ x) {
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= cnt) return;
+  const int r = lo + k, i = map[r];
+  x[i] = b[i] * dinv[r];
 }
 
 template <typename T>
@@ -507,17 +712,67 @@ void bn_csr_apply_binned(char vt, int op, int n, const int *ia, const int *ja, c
   }
 }
 
+void bn_csr_apply_blocks(char vt, int op, int nblk, const int *blocks, const int *ia, const int *ja,
+                         const void *val, const void *x, void *y, const void *b, hipStream_t stream) {
+  if (vt == 's')
+    launch_blocks<float>(op, nblk, blocks, ia, ja, (const float *)val, (const float *)x, (float *)y,
+                         (const float *)b, stream);
+  else
+    launch_blocks<double>(op, nblk, blocks, ia, ja, (const double *)val, (const double *)x, (double *)y,
+                          (const double *)b, stream);
+}
+
+void bn_block_gs(char vt, int nchunk, int max_rows, const int *chunk, const int *ccol, const int *cbnd,
+                 const int *ia, const int *ja, const void *val, const void *dinv, const void *b,
+                 const void *xold, void *xnew, int backward, int lanes, const void *zbase, void *zsum,
+                 hipStream_t stream) {
+  if (max_rows > BN_CHUNK_MAX_ROWS) return; /* the solver never builds such chunks */
+  if (vt == 's')
+    dispatch_block_gs<float>(nchunk, max_rows, chunk, ccol, cbnd, ia, ja, (const float *)val, (const float *)dinv,
+                             (const float *)b, (const float *)xold, (float *)xnew, backward, lanes,
+                             (const float *)zbase, (float *)zsum, stream);
+  else
+    dispatch_block_gs<double>(nchunk, max_rows, chunk, ccol, cbnd, ia, ja, (const double *)val,
+                              (const double *)dinv, (const double *)b, (const double *)xold, (double *)xnew,
+                              backward, lanes, (const double *)zbase, (double *)zsum, stream);
+}
+
 void bn_gs_color(char vt, int lo, int hi, const int *ia, const int *ja, const void *val,
                  const void *dinv, const void *b, void *x, int lanes, const void *zbase, void *zsum,
                  hipStream_t stream) {
   if (vt == 's')
-    dispatch_rowsolve<float, false>(lo, hi - lo, nullptr, ia, ja, (const float *)val, (const float *)dinv,
+    dispatch_rowsolve<float, RS_RANGE>(lo, hi - lo, nullptr, ia, ja, (const float *)val, (const float *)dinv,
                                     (const float *)b, (float *)x, lanes, (const float *)zbase,
                                     (float *)zsum, stream);
   else
-    dispatch_rowsolve<double, false>(lo, hi - lo, nullptr, ia, ja, (const double *)val,
+    dispatch_rowsolve<double, RS_RANGE>(lo, hi - lo, nullptr, ia, ja, (const double *)val,
                                      (const double *)dinv, (const double *)b, (double *)x, lanes,
                                      (const double *)zbase, (double *)zsum, stream);
+}
+
+void bn_gs_color_map(char vt, int lo, int hi, const int *map, const int *ia, const int *ja, const void *val,
+                     const void *dinv, const void *b, void *x, int lanes, const void *zbase, void *zsum,
+                     hipStream_t stream) {
+  if (vt == 's')
+    dispatch_rowsolve<float, RS_MAP>(lo, hi - lo, map, ia, ja, (const float *)val, (const float *)dinv,
+                                     (const float *)b, (float *)x, lanes, (const float *)zbase, (float *)zsum,
+                                     stream);
+  else
+    dispatch_rowsolve<double, RS_MAP>(lo, hi - lo, map, ia, ja, (const double *)val, (const double *)dinv,
+                                      (const double *)b, (double *)x, lanes, (const double *)zbase,
+                                      (double *)zsum, stream);
+}
+
+void bn_gs_color_zero_map(char vt, int lo, int hi, const int *map, const void *dinv, const void *b, void *x,
+                          hipStream_t stream) {
+  const int cnt = hi - lo;
+  if (cnt <= 0) return;
+  if (vt == 's')
+    hipLaunchKernelGGL(k_diag_scale_map<float>, g1(cnt), dim3(BLOCK), 0, stream, lo, cnt, map,
+                       (const float *)dinv, (const float *)b, (float *)x);
+  else
+    hipLaunchKernelGGL(k_diag_scale_map<double>, g1(cnt), dim3(BLOCK), 0, stream, lo, cnt, map,
+                       (const double *)dinv, (const double *)b, (double *)x);
 }
 
 void bn_gs_color_zero(char vt, int lo, int hi, const void *dinv, const void *b, void *x,
@@ -562,11 +817,11 @@ void bn_trsv_level(char vt, int cnt, const int *rows, const int *ia, const int *
                    const void *dinv, const void *b, void *y, int lanes, const void *zbase, void *zsum,
                    hipStream_t stream) {
   if (vt == 's')
-    dispatch_rowsolve<float, true>(0, cnt, rows, ia, ja, (const float *)val, (const float *)dinv,
+    dispatch_rowsolve<float, RS_LIST>(0, cnt, rows, ia, ja, (const float *)val, (const float *)dinv,
                                    (const float *)b, (float *)y, lanes, (const float *)zbase,
                                    (float *)zsum, stream);
   else
-    dispatch_rowsolve<double, true>(0, cnt, rows, ia, ja, (const double *)val, (const double *)dinv,
+    dispatch_rowsolve<double, RS_LIST>(0, cnt, rows, ia, ja, (const double *)val, (const double *)dinv,
                                     (const double *)b, (double *)y, lanes, (const double *)zbase,
                                     (double *)zsum, stream);
 }

@@ -60,6 +60,11 @@ typedef struct {
   int device;         /* HIP device ordinal, -1 = current                 */
   int verbose;        /* 1: hierarchy report on stderr                    */
   int graph;          /* 1: replay each PCG step as one captured hipGraph */
+  int block_smoother; /* 1: every level smooths with chunked l1 block     */
+                      /*    Gauss-Seidel (one launch per sweep) instead   */
+                      /*    of multicolour GS / l1-Jacobi                 */
+  int chunk_rows;     /* largest chunk of the block smoother (<= 8192;    */
+                      /*    0: 8192)                                      */
 } bn_options;
 
 typedef struct {
@@ -113,6 +118,7 @@ int bn_solver_ic_levels(void *h, int *fwd, int *bwd);
  *   2  fine-level residual r = b - A x, V-cycle precision
  *   3  fine-level IC(0) forward + backward application (current IC mode)
  *   4  one preconditioner application (AMG + IC)
+ *   5  plain FP64 copy of n values (16 n bytes): calibrates byte counters
  * Returns 0 or a BN_ERR_* code. */
 int bn_solver_time_kernel(void *h, int which, int reps, double *ms_avg, double *bytes);
 

@@ -71,6 +71,52 @@ def test_csr_ops_against_oracle(oracle, vt):
                 assert relerr(got, ref) < tol, (lev, lanes, op)
 
 
+def hub_matrix(seed=0):
+    """Square CSR with empty rows, short rows and hub rows longer than one
+    row block (vector mode), columns sorted and unique per row."""
+    rng = np.random.default_rng(seed)
+    n = 6000
+    lens = rng.integers(1, 14, n)
+    lens[rng.choice(n, 300, replace=False)] = 0
+    lens[rng.choice(n, 12, replace=False)] = rng.integers(2049, 5900, 12)
+    lens[rng.choice(n, 40, replace=False)] = rng.integers(200, 2048, 40)
+    ia = np.zeros(n + 1, np.int32)
+    ia[1:] = np.cumsum(lens)
+    ja = np.concatenate([np.sort(rng.choice(n, k, replace=False)) for k in lens]).astype(np.int32)
+    val = rng.random(len(ja)) + 0.5
+    return ia, ja, val
+
+
+@pytest.mark.parametrize("vt", ["d", "s"])
+def test_row_block_kernel_against_oracle(oracle, vt):
+    s, net = coarse_level_matrix()
+    mats = [("A%d" % lev, s.level(lev)["A"]) for lev in range(s.num_levels())]
+    mats.append(("hubs", hub_matrix()))
+    rng = np.random.default_rng(8)
+    dt = np.float64 if vt == "d" else np.float32
+    tol = 1e-12 if vt == "d" else 1e-5
+    for name, (ia, ja, val) in mats:
+        ia = np.ascontiguousarray(ia, np.int32)
+        ja = np.ascontiguousarray(ja, np.int32)
+        n = len(ia) - 1
+        blk = np.zeros(n + 1, np.int32)
+        nb = lib.bn_row_blocks(ip(ia), n, 2048, 256, ip(blk))
+        case(f"row-block csr {name} n={n} nnz={len(ja)} blocks={nb} vt={vt}")
+        x, b, y0 = rng.random(n), rng.random(n), rng.random(n)
+        ref_set, ref_res, ref_add = np.zeros(n), np.zeros(n), y0.copy()
+        oracle.lib.abyvg(ip(ia), ip(ja), dp(val), dp(x), n, dp(ref_set))
+        oracle.lib.abyvam(dp(b), ip(ia), ip(ja), dp(val), dp(x), n, dp(ref_res))
+        oracle.lib.abyvcs(ip(ia), ip(ja), dp(val), dp(x), n, dp(ref_add))
+        dia, dja, dv, dblk = dev(ia), dev(ja), dev(val.astype(dt)), dev(blk[: nb + 1])
+        dx, db = dev(x.astype(dt)), dev(b.astype(dt))
+        for op, ref, init in ((0, ref_set, None), (1, ref_res, None), (2, ref_add, y0)):
+            dy = dev((init if init is not None else np.zeros(n)).astype(dt))
+            lib.bn_csr_apply_blocks(vt.encode(), op, nb, dblk.ptr, dia.ptr, dja.ptr, dv.ptr, dx.ptr, dy.ptr,
+                                    db.ptr, None)
+            got = dy.numpy().astype(np.float64)
+            assert relerr(got, ref) < tol, (name, op)
+
+
 def test_restriction_gather_equals_scatter(oracle):
     s, net = coarse_level_matrix()
     rng = np.random.default_rng(1)
@@ -127,6 +173,109 @@ def test_multicolour_sweep_is_exact_gauss_seidel(vt):
     assert relerr(lhs, rhs) < (1e-12 if vt == "d" else 2e-5)
 
 
+@pytest.mark.parametrize("vt", ["d", "s"])
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_mapped_multicolour_sweep_is_exact_gauss_seidel(vt, lanes):
+    # The solver's layout: vectors in natural order, the level's rows
+    # copied in colour order (columns natural), row r updating entry
+    # map[r].  A forward sweep in colour order pi solves
+    # (D + L_pi) x_new = b - U_pi x_old, L_pi/U_pi = couplings to rows
+    # earlier/later in pi.
+    net = make_network(14, 8, seed=3)
+    A = sp.csr_matrix((net.val, net.ja, net.ia), shape=(net.n, net.n))
+    n = net.n
+    m = L.CSR(n, n, net.nnz, ip(net.ia), ip(net.ja), dp(net.val))
+    perm = np.zeros(n, np.int32)
+    cptr = C.POINTER(C.c_int)()
+    mis = (np.arange(n) % 3 == 0).astype(np.int32)
+    ncol = lib.bn_colour_order(C.byref(m), ip(mis), 1, ip(perm), C.byref(cptr))
+    col_ptr = np.ctypeslib.as_array(cptr, (ncol + 1,)).copy()
+    Ac = A[perm].tocsr()                      # rows in colour order, columns natural
+    Ac.sort_indices()
+    pos = np.empty(n, np.int64)
+    pos[perm] = np.arange(n)
+    case(f"mapped multicolour GS n={n} colours={ncol} vt={vt} lanes={lanes}")
+    rng = np.random.default_rng(4)
+    b = rng.random(n) * 1e-9
+    x_old = rng.random(n) * 1e3
+    zbase = rng.random(n)
+    D = A.diagonal()
+    dt = np.float64 if vt == "d" else np.float32
+    dia, dja, dv = dev(Ac.indptr.astype(np.int32)), dev(Ac.indices.astype(np.int32)), dev(Ac.data.astype(dt))
+    dmap, ddinv = dev(perm), dev((1.0 / D[perm]).astype(dt))
+    db, dx = dev(b.astype(dt)), dev(x_old.astype(dt))
+    dzb, dzs = dev(zbase.astype(dt)), dev(np.zeros(n, dt))
+    for c in range(ncol):
+        lib.bn_gs_color_map(vt.encode(), int(col_ptr[c]), int(col_ptr[c + 1]), dmap.ptr, dia.ptr, dja.ptr, dv.ptr,
+                            ddinv.ptr, db.ptr, dx.ptr, lanes, dzb.ptr, dzs.ptr, None)
+    x_new = dx.numpy().astype(np.float64)
+    coo = A.tocoo()
+    lower = pos[coo.col] < pos[coo.row]
+    upper = pos[coo.col] > pos[coo.row]
+    Lp = sp.csr_matrix((coo.data[lower], (coo.row[lower], coo.col[lower])), shape=(n, n))
+    Up = sp.csr_matrix((coo.data[upper], (coo.row[upper], coo.col[upper])), shape=(n, n))
+    lhs = D * x_new + Lp @ x_new
+    rhs = b - Up @ x_old
+    assert relerr(lhs, rhs) < (1e-12 if vt == "d" else 2e-5)
+    # every colour wrote zsum = zbase + x for its rows
+    assert relerr(dzs.numpy().astype(np.float64), zbase + x_new) < (1e-15 if vt == "d" else 1e-6)
+    # zero start: the first colour is a pure diagonal scaling
+    dx0 = dev(np.full(n, np.nan, dt))
+    lo, hi = int(col_ptr[0]), int(col_ptr[1])
+    lib.bn_gs_color_zero_map(vt.encode(), lo, hi, dmap.ptr, ddinv.ptr, db.ptr, dx0.ptr, None)
+    got = dx0.numpy().astype(np.float64)[perm[lo:hi]]
+    assert relerr(got, (b / D)[perm[lo:hi]]) < (1e-15 if vt == "d" else 1e-6)
+
+
+@pytest.mark.parametrize("vt", ["d", "s"])
+@pytest.mark.parametrize("backward", [0, 1])
+@pytest.mark.parametrize("zero", [False, True])
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_block_gs_sweep_solves_its_splitting(vt, backward, zero, lanes):
+    # One block sweep over chunked rows is a residual correction whose
+    # splitting matrix is the chunk-local lower (upper when backward)
+    # triangle plus the diagonal with the l1 sum of the off-chunk couplings:
+    #   (D1 + L_in) (x_new - x_old) = b - A x_old
+    from test_schedule import chunk_order
+    net = make_network(14, 8, seed=3)
+    n = net.n
+    A = sp.csr_matrix((net.val, net.ja, net.ia), shape=(n, n))
+    mis = (np.arange(n) % 3 == 0).astype(np.int32)
+    nk, perm, chunk, ccol, cbnd = chunk_order(net.ia, net.ja, net.val, mis, 40)
+    Ap = A[perm][:, perm].tocsr()
+    Ap.sort_indices()
+    chunk_of = np.repeat(np.arange(nk), np.diff(chunk))
+    coo = Ap.tocoo()
+    inside = chunk_of[coo.row] == chunk_of[coo.col]
+    D = Ap.diagonal()
+    offsum = np.zeros(n)
+    np.add.at(offsum, coo.row[~inside], np.abs(coo.data[~inside]))
+    D1 = D + offsum
+    case(f"block GS n={n} chunks={nk} vt={vt} backward={backward} zero={zero} lanes={lanes}")
+    rng = np.random.default_rng(4)
+    b = rng.random(n) * 1e-9
+    x_old = np.zeros(n) if zero else rng.random(n) * 1e3
+    zbase = rng.random(n) * 1e3
+    dt = np.float64 if vt == "d" else np.float32
+    dia, dja, dv = dev(Ap.indptr.astype(np.int32)), dev(Ap.indices.astype(np.int32)), dev(Ap.data.astype(dt))
+    dbd, db = dev((1.0 / D1).astype(dt)), dev(b.astype(dt))
+    dxo, dxn = dev(x_old.astype(dt)), dev(np.zeros(n, dt))
+    dzb, dzs = dev(zbase.astype(dt)), dev(np.zeros(n, dt))
+    dch, dcc, dcb = dev(chunk), dev(ccol), dev(cbnd)
+    lib.bn_block_gs(vt.encode(), nk, int(np.diff(chunk).max()), dch.ptr, dcc.ptr, dcb.ptr, dia.ptr, dja.ptr, dv.ptr,
+                    dbd.ptr, db.ptr, None if zero else dxo.ptr, dxn.ptr, backward, lanes, dzb.ptr, dzs.ptr, None)
+    x_new = dxn.numpy().astype(np.float64)
+    tri = sp.triu(Ap, k=1) if backward else sp.tril(Ap, k=-1)
+    tri = tri.tocoo()
+    keep = chunk_of[tri.row] == chunk_of[tri.col]
+    T_in = sp.csr_matrix((tri.data[keep], (tri.row[keep], tri.col[keep])), shape=(n, n))
+    e = x_new - x_old
+    lhs = D1 * e + T_in @ e
+    rhs = b - Ap @ x_old
+    assert relerr(lhs, rhs) < (1e-12 if vt == "d" else 2e-5)
+    assert relerr(dzs.numpy().astype(np.float64), zbase + x_new) < (1e-15 if vt == "d" else 1e-6)
+
+
 def test_level_scheduled_ic_against_oracle(oracle):
     net = make_network(16, 9)
     h = oracle.build(net.ia, net.ja, net.val, coarsening="rs")
@@ -173,9 +322,10 @@ def test_dense_gemv():
         assert relerr(dy.numpy(), M @ x) < 1e-13
 
 
-def test_preconditioner_symmetric_and_matches_library_entry(net30):
-    case("preconditioner symmetry")
-    s = AMGSolver(net30.ia, net30.ja, net30.val, min_size=500)
+@pytest.mark.parametrize("block", [False, True])
+def test_preconditioner_symmetric_and_matches_library_entry(net30, block):
+    case(f"preconditioner symmetry block={block}")
+    s = AMGSolver(net30.ia, net30.ja, net30.val, min_size=500, block_smoother=block, chunk_rows=512)
     rng = np.random.default_rng(7)
     u, v = rng.random(net30.n), rng.random(net30.n)
     Bu, Bv = s.precondition(u), s.precondition(v)

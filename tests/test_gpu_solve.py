@@ -108,6 +108,40 @@ def test_maxiter_failure_returns_negative(net16):
     assert it == -1
 
 
+@pytest.mark.parametrize("apply", ["lanes", "blocks"])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_operator_kernel_choice_matches_reference(net30, oracle, apply, precision, monkeypatch):
+    # level operators pinned to the lanes-per-row or the row-block kernel
+    # (normally chosen by timing at upload): both give the reference solve
+    _case(f"lattice30 hmis apply={apply} {precision}")
+    monkeypatch.setenv("BN_APPLY", apply)
+    xo, it_o, _ = oracle.build(net30.ia, net30.ja, net30.val, coarsening="hmis", min_size=300).solve(
+        net30.b, 1e-10, 1000)
+    s = AMGSolver(net30.ia, net30.ja, net30.val, coarsening="hmis", min_size=300, precision=precision,
+                  fine_ic="jacobi" if precision == "fp32" else "exact")
+    x, it = s.solve(net30.b, tol=1e-10, maxiter=1000)
+    monkeypatch.delenv("BN_APPLY")
+    print(f"iterations gpu={it} reference={it_o} rel(x, ref)={rel(x, xo):.2e}")
+    assert it > 0
+    assert rel(x, xo) < 1e-8
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+@pytest.mark.parametrize("coarsening", ["pmis", "hmis"])
+def test_block_smoother_matches_reference(net30, oracle, coarsening, precision):
+    # chunked l1 block Gauss-Seidel on every level instead of the
+    # multicolour sweep: a different smoother, the same converged solve
+    _case(f"lattice30 {coarsening} block smoother {precision}")
+    xo, it_o, _ = oracle.build(net30.ia, net30.ja, net30.val, coarsening=coarsening, min_size=300).solve(
+        net30.b, 1e-10, 1000)
+    s = AMGSolver(net30.ia, net30.ja, net30.val, coarsening=coarsening, min_size=300, precision=precision,
+                  fine_ic="jacobi" if precision == "fp32" else "exact", block_smoother=True, chunk_rows=512)
+    x, it = s.solve(net30.b, tol=1e-10, maxiter=1000)
+    print(f"iterations gpu={it} reference={it_o} rel(x, ref)={rel(x, xo):.2e}")
+    assert 0 < it <= 2 * it_o + 5
+    assert rel(x, xo) < 1e-8
+
+
 def test_solve_is_deterministic(net30):
     _case("lattice30 bitwise reproducibility")
     s = AMGSolver(net30.ia, net30.ja, net30.val, min_size=500)
@@ -116,15 +150,16 @@ def test_solve_is_deterministic(net30):
     assert i1 == i2 and np.array_equal(x1, x2)
 
 
+@pytest.mark.parametrize("block", [False, True])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_graph_replay_equals_eager(net30, precision, monkeypatch):
+def test_graph_replay_equals_eager(net30, precision, block, monkeypatch):
     # every PCG step replays one captured hipGraph; eager launches of the
     # same kernels (BN_NO_GRAPH=1, same solver, so the same autotuned
     # launch shapes) must give bit-identical iterates, across several
     # solves with other work in between
-    _case(f"lattice30 graph vs eager {precision}")
+    _case(f"lattice30 graph vs eager {precision} block={block}")
     kw = dict(coarsening="pmis", min_size=500, precision=precision,
-              fine_ic="jacobi" if precision == "fp32" else "exact")
+              fine_ic="jacobi" if precision == "fp32" else "exact", block_smoother=block, chunk_rows=512)
     s = AMGSolver(net30.ia, net30.ja, net30.val, **kw)
     monkeypatch.setenv("BN_NO_GRAPH", "1")
     xe, ie = s.solve(net30.b, tol=1e-10)
