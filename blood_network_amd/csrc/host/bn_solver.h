@@ -23,9 +23,10 @@ typedef struct {
   int *ia, *ja;
   double *vd;
   float *vs;
-  int long_len;    /* rows longer than this go to the block-per-row bin */
-  int nlong;       /* number of such rows                              */
-  int *long_rows;  /* device list of those rows                        */
+  int nlong_for[7];/* rows longer than 16 * 2^k entries (lanes 2^k go */
+                   /* to the block-per-row bin with that many rows)     */
+  int *long_rows;  /* device: rows longer than 16 entries, longest     */
+                   /* first, so every bin is a prefix                  */
   int nblk;        /* row blocks for the adaptive kernel               */
   int *blk;        /* device: nblk + 1 row boundaries                  */
   int adapt;       /* FP64 apply uses the row-block kernel (autotuned) */
@@ -121,6 +122,14 @@ struct bn_solver {
 
 /* lanes per row for precision vt */
 static inline int bn_lanes(const bn_dcsr *m, char vt) { return vt == 's' ? m->lanes_s : m->lanes; }
+
+/* long-row bin for a lanes-per-row kernel: rows over 16 strides */
+static inline int bn_long_len(int lanes) { return 16 * lanes; }
+static inline int bn_nlong(const bn_dcsr *m, int lanes) {
+  int k = 0;
+  while ((1 << k) < lanes && k < 6) k++;
+  return m->nlong_for[k];
+}
 
 /* device.c */
 int bn_device_upload(struct bn_solver *s);

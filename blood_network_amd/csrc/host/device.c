@@ -64,6 +64,11 @@ static int lanes_for(long long nnz, long long rows) {
   return l;
 }
 
+static int cmp_desc_ll(const void *a, const void *b) {
+  const long long x = *(const long long *)a, y = *(const long long *)b;
+  return (x < y) - (x > y);
+}
+
 static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
   d->rows = m->row;
   d->cols = m->col;
@@ -78,22 +83,34 @@ static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
     return BN_ERR_HIP;
   }
   /* row-length bins: rows over 16 strides of the lane group go to the
-   * block-per-row kernel (coarse Galerkin rows are heavy-tailed) */
-  d->long_len = 16 * d->lanes;
-  d->nlong = 0;
+   * block-per-row kernel (coarse Galerkin rows are heavy-tailed).  One
+   * list, longest rows first, serves every lanes choice as a prefix. */
   d->long_rows = NULL;
-  for (int i = 0; i < m->row; i++) d->nlong += (m->ia[i + 1] - m->ia[i] > d->long_len);
-  if (d->nlong > 0) {
-    int *lr = (int *)bn_xmalloc((size_t)d->nlong * sizeof(int));
+  int nl = 0;
+  for (int i = 0; i < m->row; i++) nl += (m->ia[i + 1] - m->ia[i] > bn_long_len(1));
+  for (int k = 0; k < 7; k++) d->nlong_for[k] = 0;
+  if (nl > 0) {
+    long long *key = (long long *)bn_xmalloc((size_t)nl * sizeof(long long));
     int q = 0;
-    for (int i = 0; i < m->row; i++)
-      if (m->ia[i + 1] - m->ia[i] > d->long_len) lr[q++] = i;
-    d->long_rows = (int *)dmalloc(s, (size_t)d->nlong * sizeof(int));
+    for (int i = 0; i < m->row; i++) {
+      const int len = m->ia[i + 1] - m->ia[i];
+      if (len > bn_long_len(1)) key[q++] = ((long long)len << 32) | (long long)(0x7fffffff - i);
+    }
+    qsort(key, (size_t)nl, sizeof(long long), cmp_desc_ll);
+    int *lr = (int *)bn_xmalloc((size_t)nl * sizeof(int));
+    for (int t = 0; t < nl; t++) {
+      lr[t] = 0x7fffffff - (int)(key[t] & 0xffffffffLL);
+      const int len = (int)(key[t] >> 32);
+      for (int k = 0; k < 7; k++)
+        if (len > bn_long_len(1 << k)) d->nlong_for[k] = t + 1;
+    }
+    free(key);
+    d->long_rows = (int *)dmalloc(s, (size_t)nl * sizeof(int));
     if (!d->long_rows) {
       free(lr);
       return BN_ERR_HIP;
     }
-    CK(hipMemcpy(d->long_rows, lr, (size_t)d->nlong * sizeof(int), hipMemcpyHostToDevice));
+    CK(hipMemcpy(d->long_rows, lr, (size_t)nl * sizeof(int), hipMemcpyHostToDevice));
     free(lr);
   }
   /* row blocks of <= BN_ROWBLOCK_NNZ entries for the adaptive kernel */
@@ -231,8 +248,8 @@ static void autotune_csr(struct bn_solver *s, bn_dcsr *d, void *x, void *y, int 
       for (int r = -1; r < reps; r++) {
         if (r == 0) hipEventRecord(s->ev0, s->stream);
         if (cand[k] > 0)
-          bn_csr_apply_binned(vt, BN_OP_SET, d->rows, d->ia, d->ja, val, x, y, NULL, cand[k], d->long_len,
-                              d->nlong, d->long_rows, s->stream);
+          bn_csr_apply_binned(vt, BN_OP_SET, d->rows, d->ia, d->ja, val, x, y, NULL, cand[k],
+                              bn_long_len(cand[k]), bn_nlong(d, cand[k]), d->long_rows, s->stream);
         else
           bn_csr_apply_blocks(vt, BN_OP_SET, d->nblk, d->blk, d->ia, d->ja, val, x, y, NULL, s->stream);
       }

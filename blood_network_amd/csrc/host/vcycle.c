@@ -35,8 +35,9 @@ static void apply(struct bn_solver *s, const bn_dcsr *M, int op, const void *x, 
     bn_csr_apply_blocks(s->vt, op, M->nblk, M->blk, M->ia, M->ja, opv(M, s->vt), x, y, b, s->stream);
     return;
   }
-  bn_csr_apply_binned(s->vt, op, M->rows, M->ia, M->ja, opv(M, s->vt), x, y, b, bn_lanes(M, s->vt), M->long_len,
-                      M->nlong, M->long_rows, s->stream);
+  const int l = bn_lanes(M, s->vt);
+  bn_csr_apply_binned(s->vt, op, M->rows, M->ia, M->ja, opv(M, s->vt), x, y, b, l, bn_long_len(l), bn_nlong(M, l),
+                      M->long_rows, s->stream);
 }
 
 /* Block smoother: nu sweeps that start from xin (NULL: zero) and end in
@@ -80,9 +81,9 @@ static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int bac
     }
     return;
   }
-  if (x_zero) bn_fill(vt, n, x, 0.0, s->stream);
   const int gs = d->use_gs && (s->opt.smoother == SMOOTHER_CF_GS || s->opt.smoother == SMOOTHER_GS);
   if (gs) {
+    if (x_zero) bn_fill(vt, n, x, 0.0, s->stream);
     for (int sw = 0; sw < nu; sw++) {
       const int last = (sw == nu - 1);
       for (int t = 0; t < d->ncol; t++) {
@@ -103,11 +104,10 @@ static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int bac
     const double om = jac ? s->opt.omega : 1.0;
     for (int sw = 0; sw < nu; sw++) {
       if (sw == 0 && x_zero) {
-        /* x = om * w * b */
-        bn_copy(vt, n, b, d->r, s->stream);
-      } else {
-        apply(s, &d->A, BN_OP_RESID, x, d->r, b);
+        bn_jacobi_zero(vt, n, b, w, om, x, s->stream); /* x = om * w * b */
+        continue;
       }
+      apply(s, &d->A, BN_OP_RESID, x, d->r, b);
       bn_jacobi_update(vt, n, d->r, w, om, x, s->stream);
     }
     if (zsum) {

@@ -110,6 +110,9 @@ void bn_gs_color_zero(char vt, int lo, int hi, const void *dinv,
  * the residual must come from a separate bn_csr_apply into r. */
 void bn_jacobi_update(char vt, int n, const void *r, const void *dscale,
                       double omega, void *x, hipStream_t stream);
+/* first sweep from x = 0: x_i = omega * dscale_i * b_i */
+void bn_jacobi_zero(char vt, int n, const void *b, const void *dscale,
+                    double omega, void *x, hipStream_t stream);
 
 /* One level of a level-scheduled triangular solve: for the `cnt` rows in
  * rows[], y_i = (b_i - sum_{j != i} T_ij y_j) * dinv_i.  If zsum != NULL,

@@ -505,6 +505,13 @@ __global__ void k_diag_scale(int lo, int cnt, const T *__restrict__ dinv, const 
 }
 
 template <typename T>
+__global__ void k_jacobi_zero(int n, const T *__restrict__ b, const T *__restrict__ d, T omega,
+                              T *__restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = omega * b[i] * d[i];
+}
+
+template <typename T>
 __global__ void k_jacobi_update(int n, const T *__restrict__ r, const T *__restrict__ d, T omega,
                                 T *__restrict__ x) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -810,6 +817,17 @@ void bn_jacobi_update(char vt, int n, const void *r, const void *dscale, double 
                        (const float *)dscale, (float)omega, (float *)x);
   else
     hipLaunchKernelGGL(k_jacobi_update<double>, g1(n), dim3(BLOCK), 0, stream, n, (const double *)r,
+                       (const double *)dscale, omega, (double *)x);
+}
+
+void bn_jacobi_zero(char vt, int n, const void *b, const void *dscale, double omega, void *x,
+                    hipStream_t stream) {
+  if (n <= 0) return;
+  if (vt == 's')
+    hipLaunchKernelGGL(k_jacobi_zero<float>, g1(n), dim3(BLOCK), 0, stream, n, (const float *)b,
+                       (const float *)dscale, (float)omega, (float *)x);
+  else
+    hipLaunchKernelGGL(k_jacobi_zero<double>, g1(n), dim3(BLOCK), 0, stream, n, (const double *)b,
                        (const double *)dscale, omega, (double *)x);
 }
 
