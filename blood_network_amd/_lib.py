@@ -135,13 +135,15 @@ def _load():
                                 C.c_int, C.c_int, VP, VP, VP]),
         "bn_chunk_order": (C.c_int, [C.POINTER(CSR), IP, C.c_int, C.c_int, IP, C.POINTER(IP), C.POINTER(IP),
                                      C.POINTER(IP)]),
+        "bn_csr_apply_seg": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, C.c_int, C.c_int,
+                                     VP, VP, VP, C.c_int, C.c_int, VP, VP]),
         "bn_csr_apply_blocks": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP]),
         "bn_row_blocks": (C.c_int, [IP, C.c_int, C.c_int, C.c_int, IP]),
         "bn_colour_order": (C.c_int, [C.POINTER(CSR), IP, C.c_int, IP, C.POINTER(IP)]),
         "bn_level_sets": (C.c_int, [IP, IP, C.c_int, C.c_int, IP, C.POINTER(IP)]),
         "bn_trsv_level": (None, [C.c_char, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
         "bn_gs_color": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
-        "bn_jacobi_rows": (None, [C.c_char, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP]),
+        "bn_jacobi_rows": (None, [C.c_char, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
         "bn_dense_gemv": (None, [C.c_char, C.c_int, VP, VP, VP, VP]),
         "bn_dot_partial": (None, [C.c_int, VP, VP, VP, VP, VP, VP, VP]),
         "bn_scalar_finish": (None, [C.c_int, VP, VP, VP, VP, C.c_int, VP]),

@@ -47,6 +47,14 @@ void bn_track_device(struct bn_solver *s, void *p) {
   s->dptr[s->ndptr++] = p;
 }
 
+void bn_track_host(struct bn_solver *s, void *p) {
+  if (s->nhptr == s->caphptr) {
+    s->caphptr = s->caphptr ? 2 * s->caphptr : 64;
+    s->hptr = (void **)realloc(s->hptr, (size_t)s->caphptr * sizeof(void *));
+  }
+  s->hptr[s->nhptr++] = p;
+}
+
 static void free_tri_host(bn_dtri *t) {
   free(t->ptr_f);
   free(t->ptr_b);
@@ -66,6 +74,10 @@ void bn_device_release(struct bn_solver *s) {
   free(s->dptr);
   s->dptr = NULL;
   s->ndptr = s->capdptr = 0;
+  for (int k = 0; k < s->nhptr; k++) free(s->hptr[k]);
+  free(s->hptr);
+  s->hptr = NULL;
+  s->nhptr = s->caphptr = 0;
   for (int l = 0; l < AMG_MAX_LEVELS; l++) {
     free(s->lev[l].col_ptr);
     free(s->lev[l].perm_h);

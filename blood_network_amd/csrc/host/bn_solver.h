@@ -27,6 +27,11 @@ typedef struct {
                    /* to the block-per-row bin with that many rows)     */
   int *long_rows;  /* device: rows longer than 16 entries, longest     */
                    /* first, so every bin is a prefix                  */
+  int *seg_ptr;    /* device: long row t owns segments seg_ptr[t]..    */
+  int *seg_row;    /* device: segment -> long row index                */
+  int *seg_ptr_h;  /* host copy of seg_ptr                             */
+  int nmulti_all;  /* long rows with more than one segment (a prefix)  */
+  double *seg_part;/* device: partial sums of multi-segment rows       */
   int nblk;        /* row blocks for the adaptive kernel               */
   int *blk;        /* device: nblk + 1 row boundaries                  */
   int adapt;       /* FP64 apply uses the row-block kernel (autotuned) */
@@ -118,6 +123,8 @@ struct bn_solver {
   /* device allocations owned by this solver */
   void **dptr;
   int ndptr, capdptr;
+  void **hptr;          /* host tables that live as long as the mirror */
+  int nhptr, caphptr;
 };
 
 /* lanes per row for precision vt */
@@ -129,6 +136,16 @@ static inline int bn_nlong(const bn_dcsr *m, int lanes) {
   int k = 0;
   while ((1 << k) < lanes && k < 6) k++;
   return m->nlong_for[k];
+}
+/* y = op(M, x [, b]) with a lanes-per-row kernel and the segmented bin */
+static inline void bn_dcsr_apply(const bn_dcsr *m, char vt, int op, int lanes, const void *x, void *y,
+                                 const void *b, hipStream_t st) {
+  const int nl = bn_nlong(m, lanes);
+  const int nseg = nl > 0 ? m->seg_ptr_h[nl] : 0;
+  const int nmulti = nl < m->nmulti_all ? nl : m->nmulti_all;
+  bn_csr_apply_seg(vt, op, m->rows, m->ia, m->ja, vt == 's' ? (const void *)m->vs : (const void *)m->vd, x, y, b,
+                   lanes, bn_long_len(lanes), nl, m->long_rows, m->seg_ptr, m->seg_row, nseg, nmulti, m->seg_part,
+                   st);
 }
 
 /* device.c */

@@ -22,6 +22,7 @@
 int bn_dense_inverse(const dcsr_mat *A, double *inv);
 void bn_set_error(const char *fmt, ...);
 void bn_track_device(struct bn_solver *s, void *p);
+void bn_track_host(struct bn_solver *s, void *p);
 
 static int g_device_state = -1; /* -1 unknown, 0 none, 1 usable */
 
@@ -86,6 +87,9 @@ static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
    * block-per-row kernel (coarse Galerkin rows are heavy-tailed).  One
    * list, longest rows first, serves every lanes choice as a prefix. */
   d->long_rows = NULL;
+  d->seg_ptr = d->seg_row = d->seg_ptr_h = NULL;
+  d->seg_part = NULL;
+  d->nmulti_all = 0;
   int nl = 0;
   for (int i = 0; i < m->row; i++) nl += (m->ia[i + 1] - m->ia[i] > bn_long_len(1));
   for (int k = 0; k < 7; k++) d->nlong_for[k] = 0;
@@ -98,20 +102,38 @@ static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
     }
     qsort(key, (size_t)nl, sizeof(long long), cmp_desc_ll);
     int *lr = (int *)bn_xmalloc((size_t)nl * sizeof(int));
+    int *sp = (int *)bn_xmalloc(((size_t)nl + 1) * sizeof(int));
+    sp[0] = 0;
+    d->nmulti_all = 0;
     for (int t = 0; t < nl; t++) {
       lr[t] = 0x7fffffff - (int)(key[t] & 0xffffffffLL);
       const int len = (int)(key[t] >> 32);
       for (int k = 0; k < 7; k++)
         if (len > bn_long_len(1 << k)) d->nlong_for[k] = t + 1;
+      sp[t + 1] = sp[t] + (len + BN_SEG_NNZ - 1) / BN_SEG_NNZ;
+      if (len > BN_SEG_NNZ) d->nmulti_all = t + 1;
     }
     free(key);
+    int *sr = (int *)bn_xmalloc((size_t)sp[nl] * sizeof(int));
+    for (int t = 0; t < nl; t++)
+      for (int q = sp[t]; q < sp[t + 1]; q++) sr[q] = t;
     d->long_rows = (int *)dmalloc(s, (size_t)nl * sizeof(int));
-    if (!d->long_rows) {
+    d->seg_ptr = (int *)dmalloc(s, ((size_t)nl + 1) * sizeof(int));
+    d->seg_row = (int *)dmalloc(s, (size_t)sp[nl] * sizeof(int));
+    d->seg_part = (double *)dmalloc(s, (size_t)(sp[d->nmulti_all] > 0 ? sp[d->nmulti_all] : 1) * sizeof(double));
+    if (!d->long_rows || !d->seg_ptr || !d->seg_row || !d->seg_part) {
       free(lr);
+      free(sp);
+      free(sr);
       return BN_ERR_HIP;
     }
     CK(hipMemcpy(d->long_rows, lr, (size_t)nl * sizeof(int), hipMemcpyHostToDevice));
+    CK(hipMemcpy(d->seg_ptr, sp, ((size_t)nl + 1) * sizeof(int), hipMemcpyHostToDevice));
+    CK(hipMemcpy(d->seg_row, sr, (size_t)sp[nl] * sizeof(int), hipMemcpyHostToDevice));
     free(lr);
+    free(sr);
+    d->seg_ptr_h = sp; /* kept for the per-lanes segment counts */
+    bn_track_host(s, sp);
   }
   /* row blocks of <= BN_ROWBLOCK_NNZ entries for the adaptive kernel */
   {
@@ -248,8 +270,7 @@ static void autotune_csr(struct bn_solver *s, bn_dcsr *d, void *x, void *y, int 
       for (int r = -1; r < reps; r++) {
         if (r == 0) hipEventRecord(s->ev0, s->stream);
         if (cand[k] > 0)
-          bn_csr_apply_binned(vt, BN_OP_SET, d->rows, d->ia, d->ja, val, x, y, NULL, cand[k],
-                              bn_long_len(cand[k]), bn_nlong(d, cand[k]), d->long_rows, s->stream);
+          bn_dcsr_apply(d, vt, BN_OP_SET, cand[k], x, y, NULL, s->stream);
         else
           bn_csr_apply_blocks(vt, BN_OP_SET, d->nblk, d->blk, d->ia, d->ja, val, x, y, NULL, s->stream);
       }

@@ -112,9 +112,7 @@ int bn_solver_time_kernel(void *h, int which, int reps, double *ms_avg, double *
         const void *val = vt == 's' ? (const void *)d->A.vs : (const void *)d->A.vd;
         if (!d->block && !d->use_gs) {
           /* l1-Jacobi level: residual + update */
-          const int l = bn_lanes(&d->A, vt);
-          bn_csr_apply_binned(vt, BN_OP_RESID, n, d->A.ia, d->A.ja, val, s->u1, d->r, s->r1, l, bn_long_len(l),
-                              bn_nlong(&d->A, l), d->A.long_rows, st);
+          bn_dcsr_apply(&d->A, vt, BN_OP_RESID, bn_lanes(&d->A, vt), s->u1, d->r, s->r1, st);
           bn_jacobi_update(vt, n, d->r, vt == 's' ? (const void *)d->l1inv_s : (const void *)d->l1inv_d, 1.0,
                            s->u1, st);
         } else if (d->block)

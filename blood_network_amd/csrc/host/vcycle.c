@@ -35,9 +35,7 @@ static void apply(struct bn_solver *s, const bn_dcsr *M, int op, const void *x, 
     bn_csr_apply_blocks(s->vt, op, M->nblk, M->blk, M->ia, M->ja, opv(M, s->vt), x, y, b, s->stream);
     return;
   }
-  const int l = bn_lanes(M, s->vt);
-  bn_csr_apply_binned(s->vt, op, M->rows, M->ia, M->ja, opv(M, s->vt), x, y, b, l, bn_long_len(l), bn_nlong(M, l),
-                      M->long_rows, s->stream);
+  bn_dcsr_apply(M, s->vt, op, bn_lanes(M, s->vt), x, y, b, s->stream);
 }
 
 /* Block smoother: nu sweeps that start from xin (NULL: zero) and end in
@@ -214,30 +212,35 @@ static void tri_solve(struct bn_solver *s, bn_dtri *t, char vt, const void *b, v
 }
 
 /* k Jacobi sweeps per triangle (approximate IC application; only used in
- * the flexible outer iteration) */
+ * the flexible outer iteration).  The result goes to x, or, when zsum is
+ * given, only zsum = zbase + result is written (fused into the last
+ * sweep). */
 static void tri_jacobi(struct bn_solver *s, bn_dtri *t, char vt, const void *b, void *y, void *x,
                        void *tmp, int sweeps, const void *zbase, void *zsum) {
   const void *dinv = vt == 's' ? (const void *)t->dinv_s : (const void *)t->dinv_d;
   const int n = t->n;
   void *bufs[2] = {y, tmp};
   int cur = 0;
-  bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, NULL, bufs[cur], bn_lanes(&t->lo, vt),
-                 s->stream);
+  bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, NULL, bufs[cur], bn_lanes(&t->lo, vt), NULL,
+                 NULL, s->stream);
   for (int k = 1; k < sweeps; k++) {
     bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, bufs[cur], bufs[1 - cur],
-                   bn_lanes(&t->lo, vt), s->stream);
+                   bn_lanes(&t->lo, vt), NULL, NULL, s->stream);
     cur = 1 - cur;
   }
   const void *yy = bufs[cur];
   void *xb[2] = {x, bufs[1 - cur]};
   int cx = 0;
-  bn_jacobi_rows(vt, n, t->up.ia, t->up.ja, opv(&t->up, vt), dinv, yy, NULL, xb[cx], bn_lanes(&t->up, vt),
-                 s->stream);
+  const int fuse = zsum && sweeps > 1;
+  bn_jacobi_rows(vt, n, t->up.ia, t->up.ja, opv(&t->up, vt), dinv, yy, NULL, xb[cx], bn_lanes(&t->up, vt), NULL,
+                 NULL, s->stream);
   for (int k = 1; k < sweeps; k++) {
+    const int last = (k == sweeps - 1);
     bn_jacobi_rows(vt, n, t->up.ia, t->up.ja, opv(&t->up, vt), dinv, yy, xb[cx], xb[1 - cx],
-                   bn_lanes(&t->up, vt), s->stream);
+                   bn_lanes(&t->up, vt), fuse && last ? zbase : NULL, fuse && last ? zsum : NULL, s->stream);
     cx = 1 - cx;
   }
+  if (fuse) return;
   if (xb[cx] != x) bn_copy(vt, n, xb[cx], x, s->stream);
   if (zsum) {
     bn_copy(vt, n, zbase, zsum, s->stream);

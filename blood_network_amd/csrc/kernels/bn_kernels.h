@@ -55,6 +55,19 @@ void bn_csr_apply_binned(char vt, int op, int n, const int *ia, const int *ja,
                          int lanes, int long_len, int nlong, const int *long_rows,
                          hipStream_t stream);
 
+/* Same split, long rows cut into segments of BN_SEG_NNZ entries, one
+ * block per segment: long row t = long_rows[t] owns segments
+ * seg_ptr[t] .. seg_ptr[t+1]-1 (seg_row[s] = t); the first nmulti long
+ * rows have more than one segment and their partial sums go through
+ * part[0 .. seg_ptr[nmulti]) before a fixed-order finish.  nseg =
+ * seg_ptr[nlong]. */
+#define BN_SEG_NNZ 1024
+void bn_csr_apply_seg(char vt, int op, int n, const int *ia, const int *ja,
+                      const void *val, const void *x, void *y, const void *b,
+                      int lanes, int long_len, int nlong, const int *long_rows,
+                      const int *seg_ptr, const int *seg_row, int nseg, int nmulti,
+                      double *part, hipStream_t stream);
+
 /* Row-block split for bn_csr_apply_blocks: consecutive rows grouped so a
  * block holds at most BN_ROWBLOCK_NNZ entries and BN_ROWBLOCK_ROWS rows; a
  * row longer than BN_ROWBLOCK_NNZ is a block of its own. */
@@ -124,10 +137,11 @@ void bn_trsv_level(char vt, int cnt, const int *rows, const int *ia,
 
 /* One Jacobi sweep of a triangular (or any) system over all n rows:
  * xout_i = (b_i - sum_{j != i} T_ij xin_j) * dinv_i.  xin == NULL means
- * xin = 0 (xout = b * dinv). */
+ * xin = 0 (xout = b * dinv).  With zsum != NULL (and xin != NULL) the
+ * sweep writes zsum_i = zbase_i + xout_i instead of xout. */
 void bn_jacobi_rows(char vt, int n, const int *ia, const int *ja, const void *val,
                     const void *dinv, const void *b, const void *xin, void *xout,
-                    int lanes, hipStream_t stream);
+                    int lanes, const void *zbase, void *zsum, hipStream_t stream);
 
 /* dense y = M x (row-major n x n), M in vt, x,y in vt */
 void bn_dense_gemv(char vt, int n, const void *M, const void *x, void *y,

@@ -205,6 +205,69 @@ __global__ __launch_bounds__(BLOCK) void k_csr_long(const int *__restrict__ rows
   }
 }
 
+// Segmented long-row bin: long row t (rows[t], longest first) is cut
+// into segments of BN_SEG_NNZ entries, segments seg_ptr[t] ..
+// seg_ptr[t+1]-1, one block each.  A one-segment row is finished by its
+// block; a longer row's block writes its partial sum and k_seg_finish adds
+// the partials in segment order (deterministic, no atomics).
+template <typename T, int OP>
+__global__ __launch_bounds__(BLOCK) void k_csr_seg(const int *__restrict__ seg_row, const int *__restrict__ seg_ptr,
+                                                   const int *__restrict__ rows, const int *__restrict__ ia,
+                                                   const int *__restrict__ ja, const T *__restrict__ val,
+                                                   const T *__restrict__ x, T *__restrict__ y,
+                                                   const T *__restrict__ b, double *__restrict__ part) {
+  const int sg = blockIdx.x;
+  const int t = seg_row[sg];
+  const int row = rows[t];
+  const int s0 = seg_ptr[t], s1 = seg_ptr[t + 1];
+  const int beg = ia[row] + (sg - s0) * BN_SEG_NNZ;
+  const int end = min(beg + BN_SEG_NNZ, ia[row + 1]);
+  const T v = row_dot<T, BLOCK, false>(beg + threadIdx.x, end, ja, val, x, row);
+  const double tsum = block_sum((double)v);
+  if (threadIdx.x != 0) return;
+  if (s1 - s0 > 1) {
+    part[sg] = tsum;
+    return;
+  }
+  if (OP == BN_OP_SET) y[row] = (T)tsum;
+  else if (OP == BN_OP_RESID) y[row] = b[row] - (T)tsum;
+  else y[row] = y[row] + (T)tsum;
+}
+
+template <typename T, int OP>
+__global__ void k_seg_finish(int nmulti, const int *__restrict__ seg_ptr, const int *__restrict__ rows,
+                             const double *__restrict__ part, T *__restrict__ y, const T *__restrict__ b) {
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nmulti) return;
+  double acc = 0.0;
+  for (int k = seg_ptr[t]; k < seg_ptr[t + 1]; k++) acc += part[k];
+  const int row = rows[t];
+  if (OP == BN_OP_SET) y[row] = (T)acc;
+  else if (OP == BN_OP_RESID) y[row] = b[row] - (T)acc;
+  else y[row] = y[row] + (T)acc;
+}
+
+template <typename T>
+void launch_seg(int op, int nlong, int nmulti, const int *rows, const int *seg_ptr, const int *seg_row, int nseg,
+                double *part, const int *ia, const int *ja, const T *val, const T *x, T *y, const T *b,
+                hipStream_t st) {
+  if (nlong <= 0 || nseg <= 0) return;
+  const dim3 gf((unsigned)((nmulti + BLOCK - 1) / BLOCK));
+#define BN_SEG(OPV)                                                                                          \
+  hipLaunchKernelGGL((k_csr_seg<T, OPV>), dim3(nseg), dim3(BLOCK), 0, st, seg_row, seg_ptr, rows, ia, ja, val, x, \
+                     y, b, part);                                                                             \
+  if (nmulti > 0)                                                                                            \
+    hipLaunchKernelGGL((k_seg_finish<T, OPV>), gf, dim3(BLOCK), 0, st, nmulti, seg_ptr, rows, part, y, b)
+  if (op == BN_OP_SET) {
+    BN_SEG(BN_OP_SET);
+  } else if (op == BN_OP_RESID) {
+    BN_SEG(BN_OP_RESID);
+  } else {
+    BN_SEG(BN_OP_ADD);
+  }
+#undef BN_SEG
+}
+
 template <typename T>
 void launch_long(int op, int nlong, const int *rows, const int *ia, const int *ja, const T *val, const T *x,
                  T *y, const T *b, hipStream_t st) {
@@ -459,22 +522,27 @@ template <typename T, int LANES>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_rows(int n, const int *__restrict__ ia,
                                                        const int *__restrict__ ja, const T *__restrict__ val,
                                                        const T *__restrict__ dinv, const T *__restrict__ b,
-                                                       const T *__restrict__ xin, T *__restrict__ xout) {
+                                                       const T *__restrict__ xin, T *__restrict__ xout,
+                                                       const T *__restrict__ zbase, T *__restrict__ zsum) {
   constexpr int RPB = BLOCK / LANES;
   const int i = logical_block() * RPB + threadIdx.x / LANES;
   const int sub = threadIdx.x % LANES;
   if (i >= n) return;
   T s = row_dot<T, LANES, true>(ia[i] + sub, ia[i + 1], ja, val, xin, i);
   if (LANES > 1) s = group_sum<LANES>(s);
-  if (sub == 0) xout[i] = (b[i] - s) * dinv[i];
+  if (sub == 0) {
+    const T v = (b[i] - s) * dinv[i];
+    if (zsum) zsum[i] = zbase[i] + v;
+    else xout[i] = v;
+  }
 }
 
 template <typename T>
 void dispatch_jacobi_rows(int n, const int *ia, const int *ja, const T *val, const T *dinv, const T *b,
-                          const T *xin, T *xout, int lanes, hipStream_t st) {
+                          const T *xin, T *xout, int lanes, const T *zb, T *zs, hipStream_t st) {
 #define BN_JR(L)                                                                                       \
   hipLaunchKernelGGL((k_jacobi_rows<T, L>), dim3(grid_rows((n + BLOCK / L - 1) / (BLOCK / L))), dim3(BLOCK), 0, st, \
-                     n, ia, ja, val, dinv, b, xin, xout)
+                     n, ia, ja, val, dinv, b, xin, xout, zb, zs)
   switch (lanes) {
     case 1: BN_JR(1); break;
     case 2: BN_JR(2); break;
@@ -744,6 +812,25 @@ void bn_block_gs(char vt, int nchunk, int max_rows, const int *chunk, const int 
                               backward, lanes, (const double *)zbase, (double *)zsum, stream);
 }
 
+void bn_csr_apply_seg(char vt, int op, int n, const int *ia, const int *ja, const void *val, const void *x,
+                      void *y, const void *b, int lanes, int long_len, int nlong, const int *long_rows,
+                      const int *seg_ptr, const int *seg_row, int nseg, int nmulti, double *part,
+                      hipStream_t stream) {
+  if (n <= 0) return;
+  if (nlong <= 0) long_len = 0x7fffffff;
+  if (vt == 's') {
+    dispatch_csr<float>(op, n, ia, ja, (const float *)val, (const float *)x, (float *)y, (const float *)b,
+                        lanes, nullptr, nullptr, long_len, stream);
+    launch_seg<float>(op, nlong, nmulti, long_rows, seg_ptr, seg_row, nseg, part, ia, ja, (const float *)val,
+                      (const float *)x, (float *)y, (const float *)b, stream);
+  } else {
+    dispatch_csr<double>(op, n, ia, ja, (const double *)val, (const double *)x, (double *)y,
+                         (const double *)b, lanes, nullptr, nullptr, long_len, stream);
+    launch_seg<double>(op, nlong, nmulti, long_rows, seg_ptr, seg_row, nseg, part, ia, ja, (const double *)val,
+                       (const double *)x, (double *)y, (const double *)b, stream);
+  }
+}
+
 void bn_gs_color(char vt, int lo, int hi, const int *ia, const int *ja, const void *val,
                  const void *dinv, const void *b, void *x, int lanes, const void *zbase, void *zsum,
                  hipStream_t stream) {
@@ -795,7 +882,8 @@ void bn_gs_color_zero(char vt, int lo, int hi, const void *dinv, const void *b, 
 }
 
 void bn_jacobi_rows(char vt, int n, const int *ia, const int *ja, const void *val, const void *dinv,
-                    const void *b, const void *xin, void *xout, int lanes, hipStream_t stream) {
+                    const void *b, const void *xin, void *xout, int lanes, const void *zbase, void *zsum,
+                    hipStream_t stream) {
   if (n <= 0) return;
   if (!xin) {
     bn_gs_color_zero(vt, 0, n, dinv, b, xout, stream);
@@ -803,10 +891,12 @@ void bn_jacobi_rows(char vt, int n, const int *ia, const int *ja, const void *va
   }
   if (vt == 's')
     dispatch_jacobi_rows<float>(n, ia, ja, (const float *)val, (const float *)dinv, (const float *)b,
-                                (const float *)xin, (float *)xout, lanes, stream);
+                                (const float *)xin, (float *)xout, lanes, (const float *)zbase, (float *)zsum,
+                                stream);
   else
     dispatch_jacobi_rows<double>(n, ia, ja, (const double *)val, (const double *)dinv, (const double *)b,
-                                 (const double *)xin, (double *)xout, lanes, stream);
+                                 (const double *)xin, (double *)xout, lanes, (const double *)zbase,
+                                 (double *)zsum, stream);
 }
 
 void bn_jacobi_update(char vt, int n, const void *r, const void *dscale, double omega, void *x,

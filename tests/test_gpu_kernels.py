@@ -117,6 +117,55 @@ def test_row_block_kernel_against_oracle(oracle, vt):
             assert relerr(got, ref) < tol, (name, op)
 
 
+def seg_bin(ia, long_len, seg=1024):
+    """Long-row bin with segments: rows longer than long_len, longest first
+    (ties by index), cut into `seg`-entry segments."""
+    lens = np.diff(ia)
+    rows = np.where(lens > long_len)[0]
+    rows = rows[np.lexsort((rows, -lens[rows]))].astype(np.int32)
+    nseg_row = -(-lens[rows] // seg)
+    seg_ptr = np.zeros(len(rows) + 1, np.int32)
+    seg_ptr[1:] = np.cumsum(nseg_row)
+    seg_row = np.repeat(np.arange(len(rows), dtype=np.int32), nseg_row)
+    nmulti = int(np.sum(lens[rows] > seg))
+    return rows, seg_ptr, seg_row, nmulti
+
+
+@pytest.mark.parametrize("vt", ["d", "s"])
+def test_segmented_long_rows_against_oracle(oracle, vt):
+    s, net = coarse_level_matrix()
+    mats = [("A%d" % lev, s.level(lev)["A"]) for lev in range(s.num_levels())]
+    mats.append(("hubs", hub_matrix(1)))
+    rng = np.random.default_rng(9)
+    dt = np.float64 if vt == "d" else np.float32
+    tol = 1e-12 if vt == "d" else 1e-5
+    for name, (ia, ja, val) in mats:
+        ia = np.ascontiguousarray(ia, np.int32)
+        ja = np.ascontiguousarray(ja, np.int32)
+        n = len(ia) - 1
+        for lanes in (1, 4, 16):
+            rows, seg_ptr, seg_row, nmulti = seg_bin(ia, 16 * lanes)
+            case(f"segmented csr {name} n={n} lanes={lanes} long={len(rows)} segments={seg_ptr[-1]} multi={nmulti}")
+            x, b, y0 = rng.random(n), rng.random(n), rng.random(n)
+            ref_set, ref_res, ref_add = np.zeros(n), np.zeros(n), y0.copy()
+            oracle.lib.abyvg(ip(ia), ip(ja), dp(val), dp(x), n, dp(ref_set))
+            oracle.lib.abyvam(dp(b), ip(ia), ip(ja), dp(val), dp(x), n, dp(ref_res))
+            oracle.lib.abyvcs(ip(ia), ip(ja), dp(val), dp(x), n, dp(ref_add))
+            dia, dja, dv = dev(ia), dev(ja), dev(val.astype(dt))
+            dx, db = dev(x.astype(dt)), dev(b.astype(dt))
+            one = np.zeros(1, np.int32)
+            drows = dev(rows if len(rows) else one)
+            dsp, dsr = dev(seg_ptr), dev(seg_row if len(seg_row) else one)
+            dpart = dev(np.zeros(max(1, int(seg_ptr[nmulti]))))
+            for op, ref, init in ((0, ref_set, None), (1, ref_res, None), (2, ref_add, y0)):
+                dy = dev((init if init is not None else np.zeros(n)).astype(dt))
+                lib.bn_csr_apply_seg(vt.encode(), op, n, dia.ptr, dja.ptr, dv.ptr, dx.ptr, dy.ptr, db.ptr, lanes,
+                                     16 * lanes, len(rows), drows.ptr, dsp.ptr, dsr.ptr, int(seg_ptr[-1]), nmulti,
+                                     dpart.ptr, None)
+                got = dy.numpy().astype(np.float64)
+                assert relerr(got, ref) < tol, (name, lanes, op)
+
+
 def test_restriction_gather_equals_scatter(oracle):
     s, net = coarse_level_matrix()
     rng = np.random.default_rng(1)
