@@ -260,13 +260,26 @@ static void autotune_csr(struct bn_solver *s, bn_dcsr *d, void *x, void *y, int 
     if (!dup) cand[nc++] = l;
   }
   if (allow_blocks && d->blk) cand[nc++] = 0; /* 0: the row-block kernel */
+  /* a lanes choice that sends more than 1/64 of the rows to the long
+   * bin is never competitive: drop it (keeps at least one candidate) */
+  {
+    int keep = 0;
+    for (int k = 0; k < nc; k++)
+      if (cand[k] == 0 || (long long)bn_nlong(d, cand[k]) * 64 <= d->rows || cand[k] == 64) cand[keep++] = cand[k];
+    if (keep > 0) nc = keep;
+  }
   for (int pv = 0; pv < 2; pv++) {
     const char vt = pv ? 's' : 'd';
     const void *val = pv ? (const void *)d->vs : (const void *)d->vd;
     float best = 1e30f, best_lanes_ms = 1e30f;
     int bl = base, use_blocks = 0;
+    /* enough repetitions that launch overhead does not decide the
+     * choice for small operators (about 200 MB of traffic per candidate) */
+    const double opb = (double)d->nnz * (4.0 + (pv ? 4.0 : 8.0)) + (double)d->rows * 12.0;
+    int reps = (int)(2e8 / (opb > 1.0 ? opb : 1.0));
+    if (reps < 3) reps = 3;
+    if (reps > 40) reps = 40;
     for (int k = 0; k < nc; k++) {
-      const int reps = 3;
       for (int r = -1; r < reps; r++) {
         if (r == 0) hipEventRecord(s->ev0, s->stream);
         if (cand[k] > 0)

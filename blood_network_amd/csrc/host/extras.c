@@ -147,6 +147,16 @@ int bn_solver_time_kernel(void *h, int which, int reps, double *ms_avg, double *
         bn_copy('d', n, s->p, s->q, st);
         b = 16.0 * n;
         break;
+      case 6: { /* one Jacobi sweep over the fine IC(0) lower triangle */
+        const bn_dtri *t = &s->fine_ic;
+        if (!t->lo.ia) return BN_ERR_ARG;
+        bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, vt == 's' ? (const void *)t->lo.vs : (const void *)t->lo.vd,
+                       vt == 's' ? (const void *)t->dinv_s : (const void *)t->dinv_d, s->r1, s->icy, s->icx,
+                       bn_lanes(&t->lo, vt), NULL, NULL, st);
+        /* L entries (index + value), row pointers, b, dinv, x gathered once, x written */
+        b = (double)t->lo.nnz * (4.0 + v) + (n + 1.0) * 4.0 + 4.0 * n * v;
+        break;
+      }
       default:
         return BN_ERR_ARG;
     }

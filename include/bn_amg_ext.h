@@ -119,6 +119,8 @@ int bn_solver_ic_levels(void *h, int *fwd, int *bwd);
  *   3  fine-level IC(0) forward + backward application (current IC mode)
  *   4  one preconditioner application (AMG + IC)
  *   5  plain FP64 copy of n values (16 n bytes): calibrates byte counters
+ *   6  one Jacobi sweep over the fine IC(0) lower triangle, V-cycle
+ *      precision (the kernel the Jacobi-swept IC repeats)
  * Returns 0 or a BN_ERR_* code. */
 int bn_solver_time_kernel(void *h, int which, int reps, double *ms_avg, double *bytes);
 
