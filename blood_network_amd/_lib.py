@@ -23,6 +23,30 @@ class CSR(C.Structure):
                 ("ia", IP), ("ja", IP), ("val", DP)]
 
 
+class IcholFactor(C.Structure):
+    """ichol_factor (C_amg1/include/amg.h:29-35)."""
+    _fields_ = [("n", C.c_int), ("ia", IP), ("ja", IP), ("val", DP), ("nnz", C.c_int)]
+
+
+class AMGLevel(C.Structure):
+    """amg_level (C_amg1/include/amg.h:42-53)."""
+    _fields_ = [("A", CSR), ("P", CSR), ("A_filtered", CSR), ("mis", IP), ("isolated", IP),
+                ("cf_order", IP), ("n", C.c_int), ("n_coarse", C.c_int), ("L_ichol", IcholFactor),
+                ("l1_diag", DP)]
+
+
+AMG_MAX_LEVELS = 20
+
+
+class AMGData(C.Structure):
+    """amg_data (C_amg1/include/amg.h:56-69): hierarchy plus options, the
+    argument of the library-level entry points."""
+    _fields_ = [("levels", AMGLevel * AMG_MAX_LEVELS), ("num_levels", C.c_int),
+                ("threshold", C.c_double), ("min_size", C.c_int), ("max_levels", C.c_int),
+                ("maxit_prol", C.c_int), ("lump", C.c_int), ("nu", C.c_int), ("smoother", C.c_int),
+                ("omega_jacobi", C.c_double), ("coarsening", C.c_int)]
+
+
 class Options(C.Structure):
     """bn_options (include/bn_amg_ext.h)."""
     _fields_ = [("coarsening", C.c_int), ("threshold", C.c_double),

@@ -111,3 +111,29 @@ def test_solve_without_device_fails_loudly(net16):
     s = AMGSolver(net16.ia, net16.ja, net16.val)
     with pytest.raises(RuntimeError, match="no HIP device"):
         s.solve(net16.b)
+
+
+def test_ctypes_mirrors_match_the_c_headers(tmp_path):
+    """The Python structs that mirror include/*.h (used for the library-level
+    entry points and the options/info blocks) have the C layouts."""
+    import shutil
+    import subprocess
+    if not shutil.which("gcc"):
+        pytest.skip("gcc not available")
+    from blood_network_amd import _lib as L
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    src = tmp_path / "sz.c"
+    src.write_text(
+        '#include <stdio.h>\n#include <stddef.h>\n#include "amg.h"\n#include "bn_amg_ext.h"\n'
+        'int main(void){printf("%zu %zu %zu %zu %zu %zu %zu %zu %zu\\n", sizeof(dcsr_mat), sizeof(ichol_factor),'
+        ' sizeof(amg_level), sizeof(amg_data), offsetof(amg_data, num_levels), offsetof(amg_data, coarsening),'
+        ' sizeof(bn_options), offsetof(bn_options, chunk_rows), sizeof(bn_solve_info)); return 0;}\n')
+    exe = tmp_path / "sz"
+    inc = os.path.join(root, "include")
+    subprocess.run(["gcc", "-I", inc, "-D__HIP_PLATFORM_AMD__", "-I/opt/rocm/include", str(src), "-o", str(exe)],
+                   check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    want = [C.sizeof(L.CSR), C.sizeof(L.IcholFactor), C.sizeof(L.AMGLevel), C.sizeof(L.AMGData),
+            L.AMGData.num_levels.offset, L.AMGData.coarsening.offset, C.sizeof(L.Options),
+            L.Options.chunk_rows.offset, C.sizeof(L.SolveInfo)]
+    assert got == want
