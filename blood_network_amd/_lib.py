@@ -94,7 +94,7 @@ SMOOTHER_L1, SMOOTHER_GS, SMOOTHER_CF_GS, SMOOTHER_JACOBI = 0, 1, 2, 3
 COARSEN_RS, COARSEN_PMIS, COARSEN_HMIS = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1
 COARSE_AUTO, COARSE_DENSE, COARSE_ICPCG = 0, 1, 2
-IC_EXACT, IC_JACOBI = 0, 1
+IC_EXACT, IC_JACOBI, IC_NONE = 0, 1, 2
 STOP_BNORM, STOP_RELRES = 0, 1
 
 
@@ -154,7 +154,7 @@ def _load():
         "bn_csr_apply_binned": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, C.c_int,
                                         C.c_int, VP, VP]),
         "bn_gs_color_map": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
-        "bn_gs_color_zero_map": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP]),
+        "bn_gs_color_zero_map": (None, [C.c_char, C.c_int, C.c_int, C.c_int, VP, VP, VP, VP, VP]),
         "bn_block_gs": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
                                 C.c_int, C.c_int, VP, VP, VP]),
         "bn_chunk_order": (C.c_int, [C.POINTER(CSR), IP, C.c_int, C.c_int, IP, C.POINTER(IP), C.POINTER(IP),

@@ -55,6 +55,7 @@ class AMGSolver:
 
     _COARSEN = {"rs": L.COARSEN_RS, "pmis": L.COARSEN_PMIS, "hmis": L.COARSEN_HMIS}
     _COARSE = {"auto": L.COARSE_AUTO, "dense": L.COARSE_DENSE, "icpcg": L.COARSE_ICPCG}
+    _IC = {"exact": L.IC_EXACT, "jacobi": L.IC_JACOBI, "none": L.IC_NONE}
     _SMOOTH = {"l1": L.SMOOTHER_L1, "gs": L.SMOOTHER_GS, "cf_gs": L.SMOOTHER_CF_GS,
                "jacobi": L.SMOOTHER_JACOBI}
 
@@ -82,7 +83,7 @@ class AMGSolver:
             o.smoother = self._SMOOTH[smoother]
             o.precision = L.PREC_FP32 if precision == "fp32" else L.PREC_FP64
             o.coarse_solver = self._COARSE[coarse]
-            o.fine_ic = L.IC_JACOBI if fine_ic == "jacobi" else L.IC_EXACT
+            o.fine_ic = self._IC[fine_ic]
             o.ic_sweeps = ic_sweeps
             o.max_colours = max_colours
             if flexible is None:
@@ -140,7 +141,7 @@ class AMGSolver:
         if flexible is None:
             flexible = precision == "fp32" or fine_ic == "jacobi"
         L.lib.bn_solver_set_mode(self.h, L.PREC_FP32 if precision == "fp32" else L.PREC_FP64,
-                                 L.IC_JACOBI if fine_ic == "jacobi" else L.IC_EXACT, ic_sweeps,
+                                 self._IC[fine_ic], ic_sweeps,
                                  int(bool(flexible)), L.STOP_RELRES if stop == "relres" else L.STOP_BNORM)
 
     def size(self):

@@ -157,6 +157,9 @@ int bn_device_ok(void);
 void bn_vc_backslash(struct bn_solver *s, int lev, const void *b, void *x, int x_zero);
 void bn_vc_fwdslash(struct bn_solver *s, int lev, const void *b, void *x, int x_zero);
 void bn_vc_ichol_precond(struct bn_solver *s, const double *g, double *z);
+/* the outer PCG's preconditioner: AMG + fine IC, or the plain V-cycle when
+ * fine_ic == BN_IC_NONE */
+void bn_vc_precond(struct bn_solver *s, const double *g, double *z);
 void bn_vc_vcycle_precond(struct bn_solver *s, const double *g, double *z);
 void bn_vc_fine_ic(struct bn_solver *s, const void *b, void *x);
 

@@ -42,7 +42,7 @@ static void enqueue_step(struct bn_solver *s) {
   bn_csr_apply('d', BN_OP_SET, n, A->ia, A->ja, A->vd, s->p, s->q, NULL, A->lanes, s->p, s->part1, st);
   bn_scalar_finish(1, s->part1, NULL, NULL, sc, 0, st);
   bn_pcg_update_xr(n, s->x, s->p, s->r, s->q, sc, s->part2, st);
-  bn_vc_ichol_precond(s, s->r, s->z);
+  bn_vc_precond(s, s->r, s->z);
   bn_dot_partial(n, s->r, s->z, flex ? s->z : NULL, flex ? s->q : NULL, s->part1, s->part3, st);
   bn_scalar_finish(flex ? 4 : 2, s->part1, s->part2, flex ? s->part3 : NULL, sc, 0, st);
 }
@@ -99,7 +99,7 @@ int bn_pcg_run(struct bn_solver *s, const double *d_b, double *d_x, double tol, 
   bn_csr_apply('d', BN_OP_RESID, n, A->ia, A->ja, A->vd, s->x, s->r, s->bb, A->lanes, NULL, NULL, st);
   bn_dot_partial(n, s->bb, s->bb, NULL, NULL, s->part1, NULL, st);
   bn_scalar_finish(3, s->part1, NULL, NULL, sc, 0, st);
-  bn_vc_ichol_precond(s, s->r, s->z);
+  bn_vc_precond(s, s->r, s->z);
   bn_fill('d', n, s->p, 0.0, st);
   bn_dot_partial(n, s->r, s->z, s->r, s->r, s->part1, s->part2, st);
   bn_scalar_finish(0, s->part1, s->part2, NULL, sc, 0, st);

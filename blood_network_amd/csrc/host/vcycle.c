@@ -81,15 +81,18 @@ static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int bac
   }
   const int gs = d->use_gs && (s->opt.smoother == SMOOTHER_CF_GS || s->opt.smoother == SMOOTHER_GS);
   if (gs) {
-    if (x_zero) bn_fill(vt, n, x, 0.0, s->stream);
+    /* from zero, the first colour's kernel also zeroes the other rows
+     * (unless the first sweep is also the last and must write zsum) */
+    const int fused_zero = x_zero && !(nu == 1 && zsum) && d->ncol > 0;
+    if (x_zero && !fused_zero) bn_fill(vt, n, x, 0.0, s->stream);
     for (int sw = 0; sw < nu; sw++) {
       const int last = (sw == nu - 1);
       for (int t = 0; t < d->ncol; t++) {
         const int c = backward ? d->ncol - 1 - t : t;
         const int lo = d->col_ptr[c], hi = d->col_ptr[c + 1];
         const void *dinvc = vecv(d->dinvc_d, d->dinvc_s, vt);
-        if (sw == 0 && t == 0 && x_zero && !(last && zsum)) {
-          bn_gs_color_zero_map(vt, lo, hi, d->cmap_d, dinvc, b, x, s->stream);
+        if (sw == 0 && t == 0 && fused_zero) {
+          bn_gs_color_zero_map(vt, lo, hi, n, d->cmap_d, dinvc, b, x, s->stream);
           continue;
         }
         bn_gs_color_map(vt, lo, hi, d->cmap_d, d->Ac.ia, d->Ac.ja, opv(&d->Ac, vt), dinvc, b, x,
@@ -336,4 +339,9 @@ void bn_vc_vcycle_precond(struct bn_solver *s, const double *g, double *z) {
   void *zv = (vt == 's') ? s->zout : (void *)z;
   fwdslash_rec(s, 0, s->r1, s->corr, 1, s->u1, zv);
   if (vt == 's') bn_convert('s', 'd', n, s->zout, z, s->stream);
+}
+
+void bn_vc_precond(struct bn_solver *s, const double *g, double *z) {
+  if (s->opt.fine_ic == BN_IC_NONE) bn_vc_vcycle_precond(s, g, z);
+  else bn_vc_ichol_precond(s, g, z);
 }

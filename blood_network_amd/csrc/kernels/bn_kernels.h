@@ -111,8 +111,10 @@ void bn_block_gs(char vt, int nchunk, int max_rows, const int *chunk, const int 
 void bn_gs_color_map(char vt, int lo, int hi, const int *map, const int *ia, const int *ja,
                      const void *val, const void *dinv, const void *b, void *x,
                      int lanes, const void *zbase, void *zsum, hipStream_t stream);
-/* first colour from x = 0 in that layout: x_map[r] = b_map[r] * dinv_r */
-void bn_gs_color_zero_map(char vt, int lo, int hi, const int *map, const void *dinv,
+/* first colour from x = 0 in that layout: x_map[r] = b_map[r] * dinv_r
+ * for lo <= r < hi, and x_map[r] = 0 for hi <= r < all_n (the rows of
+ * the later colours), in one pass */
+void bn_gs_color_zero_map(char vt, int lo, int hi, int all_n, const int *map, const void *dinv,
                           const void *b, void *x, hipStream_t stream);
 
 /* first colour of a sweep that starts from x = 0: x_i = b_i * dinv_i */

@@ -58,40 +58,72 @@ __device__ __forceinline__ void clear_tail(double *partial) {
 }
 
 // Sum of val[q] * x[col[q]] over one CSR row segment, `LANES` lanes
-// striding through it.  Every round issues four strides of loads before
+// striding through it.  Every round issues R strides of loads before
 // using any (the last, partial round with predicated loads), so a lane
 // keeps its gathers in flight together instead of forming a chain of
-// dependent load rounds.  SKIP_DIAG drops the entry whose column equals
-// `self`.
+// dependent load rounds.  R = 8 for two lanes per row (a 7-entry fine
+// row then finishes in one round), 4 otherwise (one lane: the extra
+// registers and masked loads of R = 8 measured slower).  SKIP_DIAG drops
+// the entry whose column equals `self`.
 template <typename T, int LANES, bool SKIP_DIAG>
 __device__ __forceinline__ T row_dot(int q, const int e, const int *__restrict__ ja,
                                      const T *__restrict__ val, const T *__restrict__ x, int self) {
-  T s0 = 0, s1 = 0;
-  for (; q + 3 * LANES < e; q += 4 * LANES) {
-    const int j0 = ja[q], j1 = ja[q + LANES], j2 = ja[q + 2 * LANES], j3 = ja[q + 3 * LANES];
-    const T a0 = val[q], a1 = val[q + LANES], a2 = val[q + 2 * LANES], a3 = val[q + 3 * LANES];
-    const T x0 = (SKIP_DIAG && j0 == self) ? T(0) : x[j0];
-    const T x1 = (SKIP_DIAG && j1 == self) ? T(0) : x[j1];
-    const T x2 = (SKIP_DIAG && j2 == self) ? T(0) : x[j2];
-    const T x3 = (SKIP_DIAG && j3 == self) ? T(0) : x[j3];
-    s0 += a0 * x0 + a1 * x1;
-    s1 += a2 * x2 + a3 * x3;
+  constexpr int R = LANES == 2 ? 8 : 4;
+  T acc[4] = {0, 0, 0, 0};
+  for (; q + (R - 1) * LANES < e; q += R * LANES) {
+    int j[R];
+    T a[R];
+#pragma unroll
+    for (int u = 0; u < R; u++) {
+      j[u] = ja[q + u * LANES];
+      a[u] = val[q + u * LANES];
+    }
+#pragma unroll
+    for (int u = 0; u < R; u++) {
+      const T xv = (SKIP_DIAG && j[u] == self) ? T(0) : x[j[u]];
+      acc[u & 3] += a[u] * xv;
+    }
   }
   if (q < e) {
-    const bool h1 = q + LANES < e, h2 = q + 2 * LANES < e;
-    const int j0 = ja[q];
-    const int j1 = h1 ? ja[q + LANES] : self;
-    const int j2 = h2 ? ja[q + 2 * LANES] : self;
-    const T a0 = val[q];
-    const T a1 = h1 ? val[q + LANES] : T(0);
-    const T a2 = h2 ? val[q + 2 * LANES] : T(0);
-    const T x0 = (SKIP_DIAG && j0 == self) ? T(0) : x[j0];
-    const T x1 = (!h1 || (SKIP_DIAG && j1 == self)) ? T(0) : x[j1];
-    const T x2 = (!h2 || (SKIP_DIAG && j2 == self)) ? T(0) : x[j2];
-    s0 += a0 * x0 + a1 * x1;
-    s1 += a2 * x2;
+    int j[R - 1];
+    T a[R - 1];
+#pragma unroll
+    for (int u = 0; u < R - 1; u++) {
+      const bool h = q + u * LANES < e;
+      j[u] = h ? ja[q + u * LANES] : self;
+      a[u] = h ? val[q + u * LANES] : T(0);
+    }
+#pragma unroll
+    for (int u = 0; u < R - 1; u++) {
+      const bool h = q + u * LANES < e;
+      const T xv = (!h || (SKIP_DIAG && j[u] == self)) ? T(0) : x[j[u]];
+      acc[u & 3] += a[u] * xv;
+    }
   }
-  return s0 + s1;
+  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
+// Block-wide sum over one long row: each thread keeps 16 strides of
+// loads in flight per round, so a row of tens of thousands of entries
+// takes a few memory round trips instead of one per 4 strides.
+template <typename T>
+__device__ __forceinline__ T row_dot_deep(int q, const int e, const int *__restrict__ ja,
+                                          const T *__restrict__ val, const T *__restrict__ x) {
+  constexpr int U = 16;
+  T acc[4] = {0, 0, 0, 0};
+  for (; q + (U - 1) * BLOCK < e; q += U * BLOCK) {
+    int j[U];
+    T a[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      j[u] = ja[q + u * BLOCK];
+      a[u] = val[q + u * BLOCK];
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) acc[u & 3] += a[u] * x[j[u]];
+  }
+  for (; q < e; q += BLOCK) acc[0] += val[q] * x[ja[q]];
+  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
 
 // Grid for one-row-block-per-block kernels: rounded up to a multiple of
@@ -196,7 +228,7 @@ __global__ __launch_bounds__(BLOCK) void k_csr_long(const int *__restrict__ rows
                                                     const T *__restrict__ x, T *__restrict__ y,
                                                     const T *__restrict__ b) {
   const int row = rows[blockIdx.x];
-  const T s = row_dot<T, BLOCK, false>(ia[row] + threadIdx.x, ia[row + 1], ja, val, x, row);
+  const T s = row_dot_deep<T>(ia[row] + threadIdx.x, ia[row + 1], ja, val, x);
   const double t = block_sum((double)s);
   if (threadIdx.x == 0) {
     if (OP == BN_OP_SET) y[row] = (T)t;
@@ -301,7 +333,7 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__res
   const int cnt = q1 - q0;
   if (cnt > BN_ROWBLOCK_NNZ) {
     // vector mode: one long row (r1 == r0 + 1)
-    const T s = row_dot<T, BLOCK, false>(q0 + threadIdx.x, q1, ja, val, x, r0);
+    const T s = row_dot_deep<T>(q0 + threadIdx.x, q1, ja, val, x);
     const double t = block_sum((double)s);
     if (threadIdx.x == 0) {
       if (OP == BN_OP_SET) y[r0] = (T)t;
@@ -555,14 +587,16 @@ void dispatch_jacobi_rows(int n, const int *ia, const int *ja, const T *val, con
 #undef BN_JR
 }
 
-// first colour from zero in RS_MAP layout: x_i = b_i dinv_r, i = map[r]
+// first colour from zero in RS_MAP layout: x_i = b_i dinv_r, i = map[r],
+// for r < hi; with all_n > hi the remaining rows (later colours) are
+// zeroed in the same pass, which replaces a separate fill of x.
 template <typename T>
-__global__ void k_diag_scale_map(int lo, int cnt, const int *__restrict__ map, const T *__restrict__ dinv,
-                                 const T *__restrict__ b, T *__restrict__ x) {
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= cnt) return;
-  const int r = lo + k, i = map[r];
-  x[i] = b[i] * dinv[r];
+__global__ void k_diag_scale_map(int lo, int hi, int all_n, const int *__restrict__ map,
+                                 const T *__restrict__ dinv, const T *__restrict__ b, T *__restrict__ x) {
+  const int r = lo + blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= all_n) return;
+  const int i = map[r];
+  x[i] = r < hi ? b[i] * dinv[r] : T(0);
 }
 
 template <typename T>
@@ -857,15 +891,16 @@ void bn_gs_color_map(char vt, int lo, int hi, const int *map, const int *ia, con
                                       (double *)zsum, stream);
 }
 
-void bn_gs_color_zero_map(char vt, int lo, int hi, const int *map, const void *dinv, const void *b, void *x,
-                          hipStream_t stream) {
-  const int cnt = hi - lo;
+void bn_gs_color_zero_map(char vt, int lo, int hi, int all_n, const int *map, const void *dinv, const void *b,
+                          void *x, hipStream_t stream) {
+  if (all_n < hi) all_n = hi;
+  const int cnt = all_n - lo;
   if (cnt <= 0) return;
   if (vt == 's')
-    hipLaunchKernelGGL(k_diag_scale_map<float>, g1(cnt), dim3(BLOCK), 0, stream, lo, cnt, map,
+    hipLaunchKernelGGL(k_diag_scale_map<float>, g1(cnt), dim3(BLOCK), 0, stream, lo, hi, all_n, map,
                        (const float *)dinv, (const float *)b, (float *)x);
   else
-    hipLaunchKernelGGL(k_diag_scale_map<double>, g1(cnt), dim3(BLOCK), 0, stream, lo, cnt, map,
+    hipLaunchKernelGGL(k_diag_scale_map<double>, g1(cnt), dim3(BLOCK), 0, stream, lo, hi, all_n, map,
                        (const double *)dinv, (const double *)b, (double *)x);
 }
 

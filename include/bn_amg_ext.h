@@ -36,6 +36,8 @@ extern "C" {
 /* fine-level IC(0) correction inside the preconditioner */
 #define BN_IC_EXACT 0   /* level-scheduled forward/backward solves          */
 #define BN_IC_JACOBI 1  /* ic_sweeps Jacobi sweeps per triangle (flexible)   */
+#define BN_IC_NONE 2    /* no fine IC(0) step: the PCG preconditioner is     */
+                        /* the plain V-cycle (amg_vcycle_precond)           */
 
 /* outer stopping test */
 #define BN_STOP_BNORM 0  /* sqrt(r'z / r0'z0) < tol (reference, amg_pcg.c:75) */

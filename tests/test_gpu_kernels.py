@@ -268,12 +268,14 @@ def test_mapped_multicolour_sweep_is_exact_gauss_seidel(vt, lanes):
     assert relerr(lhs, rhs) < (1e-12 if vt == "d" else 2e-5)
     # every colour wrote zsum = zbase + x for its rows
     assert relerr(dzs.numpy().astype(np.float64), zbase + x_new) < (1e-15 if vt == "d" else 1e-6)
-    # zero start: the first colour is a pure diagonal scaling
+    # zero start: the first colour is a pure diagonal scaling, and the
+    # same pass zeroes every later colour's rows
     dx0 = dev(np.full(n, np.nan, dt))
     lo, hi = int(col_ptr[0]), int(col_ptr[1])
-    lib.bn_gs_color_zero_map(vt.encode(), lo, hi, dmap.ptr, ddinv.ptr, db.ptr, dx0.ptr, None)
-    got = dx0.numpy().astype(np.float64)[perm[lo:hi]]
-    assert relerr(got, (b / D)[perm[lo:hi]]) < (1e-15 if vt == "d" else 1e-6)
+    lib.bn_gs_color_zero_map(vt.encode(), lo, hi, n, dmap.ptr, ddinv.ptr, db.ptr, dx0.ptr, None)
+    got = dx0.numpy().astype(np.float64)
+    assert relerr(got[perm[lo:hi]], (b / D)[perm[lo:hi]]) < (1e-15 if vt == "d" else 1e-6)
+    assert np.all(got[perm[hi:]] == 0.0)
 
 
 @pytest.mark.parametrize("vt", ["d", "s"])
