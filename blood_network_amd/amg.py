@@ -198,6 +198,30 @@ class AMGSolver:
         self.report = SolveReport(info)
         return X, its
 
+    def solve_block(self, B, X0=None, tol=1e-8, maxiter=1000):
+        """Block PCG over the k <= 32 right-hand sides B (k, n) in one
+        solve (bn_solver_solve_block); returns X (k, n) and per-column
+        iteration counts."""
+        B = np.ascontiguousarray(B, dtype=np.float64)
+        k = B.shape[0]
+        X = np.zeros_like(B) if X0 is None else np.array(X0, dtype=np.float64, order="C")
+        its = np.zeros(k, np.int32)
+        info = L.SolveInfo()
+        it = L.lib.bn_solver_solve_block(self.h, k, L.dp(B), L.dp(X), tol, maxiter, L.ip(its), C.byref(info))
+        self._check(it)
+        self.report = SolveReport(info)
+        return X, its
+
+    def solve_block_device(self, k, b_ptr, x_ptr, tol=1e-8, maxiter=1000):
+        """Device-pointer block solve: b_ptr, x_ptr hold k x n doubles."""
+        its = np.zeros(k, np.int32)
+        info = L.SolveInfo()
+        it = L.lib.bn_solver_solve_block_device(self.h, k, C.c_void_p(b_ptr), C.c_void_p(x_ptr), tol, maxiter,
+                                                L.ip(its), C.byref(info))
+        self._check(it)
+        self.report = SolveReport(info)
+        return its
+
     def level_dims(self, lev):
         d = np.zeros(8, np.int32)
         self._check(L.lib.bn_solver_level_dims(self.h, lev, L.ip(d)))

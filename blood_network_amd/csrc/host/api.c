@@ -78,6 +78,12 @@ void bn_device_release(struct bn_solver *s) {
   free(s->hptr);
   s->hptr = NULL;
   s->nhptr = s->caphptr = 0;
+  for (int k = 0; k < s->blk.ngraph; k++) {
+    if (s->blk.gexec[k]) hipGraphExecDestroy(s->blk.gexec[k]);
+    if (s->blk.graph[k]) hipGraphDestroy(s->blk.graph[k]);
+  }
+  if (s->blk.host_scal) hipHostFree(s->blk.host_scal);
+  memset(&s->blk, 0, sizeof(s->blk));
   for (int l = 0; l < AMG_MAX_LEVELS; l++) {
     free(s->lev[l].col_ptr);
     free(s->lev[l].perm_h);

@@ -79,6 +79,27 @@ typedef struct {
   void *x2;             /* second buffer for sweeps from a nonzero start */
 } bn_dlevel;
 
+/* device work space of the block (multi-right-hand-side) solve for K
+ * interleaved columns; allocated on the first block solve with that K */
+typedef struct {
+  int Kcap;                        /* columns the buffers hold (0: none) */
+  int K;                           /* columns of the solve in progress */
+  void *b[AMG_MAX_LEVELS];         /* per level n_l x K, V-cycle precision */
+  void *x[AMG_MAX_LEVELS];
+  void *r[AMG_MAX_LEVELS];
+  double *x0, *r0, *z0, *p0, *q0, *bb0; /* outer PCG, FP64 n x K */
+  void *g_in, *u1, *r1, *icy, *icx, *r2, *corr, *zout;
+  double *cin, *cout;              /* coarsest FP64 n_c x K */
+  double *scal;                    /* BN_S_COUNT per column */
+  double *part1, *part2, *part3;   /* BN_BLOCK_RED x K */
+  double *segpart;                 /* long-row segment partials x K */
+  double *host_scal;               /* pinned mirror of scal */
+  hipGraph_t graph[4];
+  hipGraphExec_t gexec[4];
+  int graph_key[4];
+  int ngraph;
+} bn_block;
+
 struct bn_solver {
   /* host side */
   dcsr_mat A_own;       /* symmetrised solve matrix (handle API)     */
@@ -125,6 +146,7 @@ struct bn_solver {
   int ndptr, capdptr;
   void **hptr;          /* host tables that live as long as the mirror */
   int nhptr, caphptr;
+  bn_block blk;         /* block solve work space */
 };
 
 /* lanes per row for precision vt */

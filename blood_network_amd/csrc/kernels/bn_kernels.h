@@ -26,6 +26,11 @@ enum {
   BN_S_BB = 8,     /* b^T b                                     */
   BN_S_RZOLD = 9,  /* previous r^T z (flexible beta)            */
   BN_S_ZRPREV = 10,/* z^T r_prev (flexible beta)                */
+  BN_S_DONE = 11,  /* block solve: column converged (frozen)    */
+  BN_S_ITS = 12,   /* block solve: iteration it converged at    */
+  BN_S_ITER = 13,  /* block solve: iterations done              */
+  BN_S_TOL = 14,   /* block solve: stop tolerance               */
+  BN_S_STOPREL = 15,/* block solve: 1 = ||r||/||b|| stop test   */
   BN_S_COUNT = 16
 };
 
@@ -195,6 +200,55 @@ void bn_permute(int n, const int *perm, const double *x, double *y,
 void bn_edge_grad(int m, const int *eu, const int *ev, const double *g,
                   const double *p, const double *lam, const double *pdir,
                   double *dg, double *grad_logr, hipStream_t stream);
+
+/* ---------------------------------------------------------------------
+ * Block (multi-right-hand-side) kernels, batch.hip.  A block vector holds
+ * K columns interleaved by row (element (i, c) at i*K + c); K is a power
+ * of two <= 32 and every call of one solve uses the same K.  Reductions
+ * use bn_block_nb(n) blocks; partial buffers hold BN_BLOCK_RED * K
+ * doubles; scalar blocks hold BN_S_COUNT doubles per column.
+ * ------------------------------------------------------------------- */
+#define BN_BLOCK_RED 1024
+#define BN_BLOCK_KMAX 32
+
+/* Y = op(A, X [, B]); rows longer than long_len go through the segmented
+ * long-row bin (same arrays as bn_csr_apply_seg; part holds
+ * seg_ptr[nmulti] * K doubles). */
+void bn_spmm(char vt, int K, int op, int n, const int *ia, const int *ja, const void *val,
+             const void *X, void *Y, const void *B, int long_len, int nlong, const int *rows,
+             const int *seg_ptr, const int *seg_row, int nseg, int nmulti, double *part,
+             hipStream_t stream);
+/* bn_gs_color_map / bn_gs_color_zero_map on block vectors */
+void bn_gs_color_map_k(char vt, int K, int lo, int hi, const int *map, const int *ia, const int *ja,
+                       const void *val, const void *dinv, const void *B, void *X, const void *zb,
+                       void *zs, hipStream_t stream);
+void bn_gs_zero_map_k(char vt, int K, int lo, int hi, int all_n, const int *map, const void *dinv,
+                      const void *B, void *X, hipStream_t stream);
+/* bn_jacobi_rows on block vectors (Xin must not be NULL) */
+void bn_jacobi_rows_k(char vt, int K, int n, const int *ia, const int *ja, const void *val,
+                      const void *dinv, const void *B, const void *Xin, void *Xout, const void *zb,
+                      void *zs, hipStream_t stream);
+/* X = om * d_i * B (add = 0) or X += om * d_i * B (add = 1), d per row */
+void bn_row_scale_k(char vt, int K, int n, const void *d, double om, const void *B, void *X, int add,
+                    hipStream_t stream);
+/* dense Y = M X, M n x n row-major FP64 */
+void bn_gemm_k(int K, int n, const double *M, const double *X, double *Y, hipStream_t stream);
+int bn_block_nb(int n);
+/* per-column partials of x.y (and u.v) */
+void bn_dot_k(int K, int n, const double *x, const double *y, const double *u, const double *v,
+              double *p1, double *p2, hipStream_t stream);
+/* x += alpha_c p, r -= alpha_c q, partials of r.r */
+void bn_update_xr_k(int K, int n, double *x, const double *p, double *r, const double *q,
+                    const double *s, double *part, hipStream_t stream);
+/* p = z + beta_c p */
+void bn_update_p_k(int K, int n, double *p, const double *z, const double *s, hipStream_t stream);
+/* per-column scalar logic (modes as bn_scalar_finish) with the stop test
+ * and freezing of converged columns in modes 2 and 4 */
+void bn_finish_k(int K, int which, int n, const double *p1, const double *p2, const double *p3,
+                 double *s, hipStream_t stream);
+/* k right-hand sides stored one after another (k x n) <-> interleaved
+ * n x K (columns k..K-1 zero); back = 1 copies the first k columns out */
+void bn_interleave(int n, int k, int K, const double *src, double *dst, int back, hipStream_t stream);
 
 #ifdef __cplusplus
 }

@@ -152,6 +152,22 @@ int bn_solver_adjoint_grad(void *h, const double *p, const double *dLdp, int m, 
 int bn_solver_solve_batch(void *h, int k, const double *b, double *x, double tol, int maxiter,
                           int *its, bn_solve_info *info);
 
+/* Block PCG: the k <= 32 right-hand sides (b, k x n, one after another;
+ * x the initial guesses, same layout, overwritten) are solved together.
+ * Every sparse operation streams its matrix once for all columns (CSR
+ * SpMM over columns interleaved by row); each column keeps its own PCG
+ * scalars and stops on its own test (flexible beta, the solver's stop
+ * mode and tol).  Needs the dense coarsest solve and a Jacobi-swept or
+ * absent fine IC(0) (BN_IC_JACOBI / BN_IC_NONE); precision follows the
+ * solver's mode.  its[j] = column j's iterations (-maxiter if it did not
+ * converge); returns the largest count, negated if any column failed.
+ * No counterpart in the reference (north-star batching form). */
+int bn_solver_solve_block(void *h, int k, const double *b, double *x, double tol, int maxiter,
+                          int *its, bn_solve_info *info);
+/* Same with device pointers. */
+int bn_solver_solve_block_device(void *h, int k, const double *d_b, double *d_x, double tol,
+                                 int maxiter, int *its, bn_solve_info *info);
+
 #ifdef __cplusplus
 }
 #endif
