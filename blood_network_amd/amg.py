@@ -198,6 +198,32 @@ class AMGSolver:
         self.report = SolveReport(info)
         return X, its
 
+    def solve_segments(self, b, seg_ptr, x0=None, tol=1e-8, maxiter=1000):
+        """Independent systems stacked block-diagonally (rows of system g:
+        seg_ptr[g] .. seg_ptr[g+1]-1) solved in one PCG with per-system
+        scalars (bn_solver_solve_segments); returns x and per-system
+        iteration counts."""
+        b = np.ascontiguousarray(b, dtype=np.float64)
+        seg = np.ascontiguousarray(seg_ptr, dtype=np.int32)
+        x = np.zeros_like(b) if x0 is None else np.array(x0, dtype=np.float64)
+        its = np.zeros(len(seg) - 1, np.int32)
+        info = L.SolveInfo()
+        it = L.lib.bn_solver_solve_segments(self.h, len(seg) - 1, L.ip(seg), L.dp(b), L.dp(x), tol, maxiter,
+                                            L.ip(its), C.byref(info))
+        self._check(it)
+        self.report = SolveReport(info)
+        return x, its
+
+    def solve_segments_device(self, seg_ptr, b_ptr, x_ptr, tol=1e-8, maxiter=1000):
+        seg = np.ascontiguousarray(seg_ptr, dtype=np.int32)
+        its = np.zeros(len(seg) - 1, np.int32)
+        info = L.SolveInfo()
+        it = L.lib.bn_solver_solve_segments_device(self.h, len(seg) - 1, L.ip(seg), C.c_void_p(b_ptr),
+                                                   C.c_void_p(x_ptr), tol, maxiter, L.ip(its), C.byref(info))
+        self._check(it)
+        self.report = SolveReport(info)
+        return its
+
     def solve_block(self, B, X0=None, tol=1e-8, maxiter=1000):
         """Block PCG over the k <= 32 right-hand sides B (k, n) in one
         solve (bn_solver_solve_block); returns X (k, n) and per-column

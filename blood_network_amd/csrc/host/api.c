@@ -63,6 +63,7 @@ static void free_tri_host(bn_dtri *t) {
 
 void bn_device_release(struct bn_solver *s) {
   if (s->stream) hipStreamSynchronize(s->stream);
+  bn_segments_release(s);
   for (int k = 0; k < s->ngraph; k++) {
     if (s->gexec[k]) hipGraphExecDestroy(s->gexec[k]);
     if (s->graph[k]) hipGraphDestroy(s->graph[k]);

@@ -147,6 +147,7 @@ struct bn_solver {
   void **hptr;          /* host tables that live as long as the mirror */
   int nhptr, caphptr;
   bn_block blk;         /* block solve work space */
+  void *segs;           /* cached segmentations of segmented solves (segments.c) */
 };
 
 /* lanes per row for precision vt */
@@ -169,6 +170,9 @@ static inline void bn_dcsr_apply(const bn_dcsr *m, char vt, int op, int lanes, c
                    lanes, bn_long_len(lanes), nl, m->long_rows, m->seg_ptr, m->seg_row, nseg, nmulti, m->seg_part,
                    st);
 }
+
+/* segments.c */
+void bn_segments_release(struct bn_solver *s);
 
 /* device.c */
 int bn_device_upload(struct bn_solver *s);

@@ -478,6 +478,142 @@ __global__ __launch_bounds__(BLOCK) void k_finish_k(int which, int nb, const dou
   }
 }
 
+// ---------------------------------------------------------------------
+// Segmented BLAS-1 for a block-diagonal union of independent systems:
+// work item w covers rows [item_lo[w], item_lo[w+1]) of system
+// item_seg[w] (items never cross systems), so the system's scalars are
+// uniform over a block.  Partials go to part[w]; k_finish_seg sums system
+// g's items item_ptr[g] .. item_ptr[g+1]-1 in order.
+// ---------------------------------------------------------------------
+__device__ __forceinline__ double block_sum256(double v, double *red) {
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int w = BLOCK / 2; w > 0; w >>= 1) {
+    if ((int)threadIdx.x < w) red[threadIdx.x] += red[threadIdx.x + w];
+    __syncthreads();
+  }
+  const double t = red[0];
+  __syncthreads();
+  return t;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_dot_seg(const int *__restrict__ item_lo, const double *__restrict__ x,
+                                                   const double *__restrict__ y, const double *__restrict__ u,
+                                                   const double *__restrict__ v, double *__restrict__ p1,
+                                                   double *__restrict__ p2) {
+  __shared__ double red[BLOCK];
+  const int w = blockIdx.x;
+  const int i0 = item_lo[w], i1 = item_lo[w + 1];
+  double a = 0.0, c = 0.0;
+  for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+    a += x[i] * y[i];
+    if (u) c += u[i] * v[i];
+  }
+  a = block_sum256(a, red);
+  if (u) c = block_sum256(c, red);
+  if (threadIdx.x == 0) {
+    p1[w] = a;
+    if (u) p2[w] = c;
+  }
+}
+
+#define SS(g, slot) s[(size_t)(g) * BN_S_COUNT + (slot)]
+
+__global__ __launch_bounds__(BLOCK) void k_update_xr_seg(const int *__restrict__ item_lo,
+                                                         const int *__restrict__ item_seg, double *__restrict__ x,
+                                                         const double *__restrict__ p, double *__restrict__ r,
+                                                         const double *__restrict__ q, const double *__restrict__ s,
+                                                         double *__restrict__ part) {
+  __shared__ double red[BLOCK];
+  const int w = blockIdx.x;
+  const int i0 = item_lo[w], i1 = item_lo[w + 1];
+  const double al = SS(item_seg[w], BN_S_ALPHA);
+  double a = 0.0;
+  for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+    x[i] = x[i] + p[i] * al;
+    const double rn = r[i] + q[i] * (-al);
+    r[i] = rn;
+    a += rn * rn;
+  }
+  a = block_sum256(a, red);
+  if (threadIdx.x == 0) part[w] = a;
+}
+
+__global__ __launch_bounds__(BLOCK) void k_update_p_seg(const int *__restrict__ item_lo,
+                                                        const int *__restrict__ item_seg, double *__restrict__ p,
+                                                        const double *__restrict__ z, const double *__restrict__ s) {
+  const int w = blockIdx.x;
+  const int i0 = item_lo[w], i1 = item_lo[w + 1];
+  const double be = SS(item_seg[w], BN_S_BETA);
+  for (int i = i0 + threadIdx.x; i < i1; i += BLOCK) p[i] = be * p[i] + z[i];
+}
+
+// one block per system; the scalar logic of k_finish_k
+__global__ __launch_bounds__(BLOCK) void k_finish_seg(int which, const int *__restrict__ item_ptr,
+                                                      const double *__restrict__ p1, const double *__restrict__ p2,
+                                                      const double *__restrict__ p3, double *__restrict__ s) {
+  __shared__ double red[BLOCK];
+  const int g = blockIdx.x;
+  double a = 0.0, b = 0.0, d = 0.0;
+  for (int k = item_ptr[g] + threadIdx.x; k < item_ptr[g + 1]; k += BLOCK) {
+    if (p1) a += p1[k];
+    if (p2) b += p2[k];
+    if (p3) d += p3[k];
+  }
+  a = block_sum256(a, red);
+  b = block_sum256(b, red);
+  d = block_sum256(d, red);
+  if (threadIdx.x != 0) return;
+  const int done = SS(g, BN_S_DONE) != 0.0;
+  switch (which) {
+    case 0:
+      SS(g, BN_S_RZ) = a;
+      SS(g, BN_S_RZ0) = a;
+      SS(g, BN_S_RR) = b;
+      SS(g, BN_S_REL) = (a == 0.0) ? 0.0 : 1.0;
+      SS(g, BN_S_BETA) = 0.0;
+      SS(g, BN_S_ITER) = 0.0;
+      SS(g, BN_S_ITS) = 0.0;
+      SS(g, BN_S_DONE) = (a == 0.0) ? 1.0 : 0.0;
+      break;
+    case 1:
+      SS(g, BN_S_PAP) = a;
+      SS(g, BN_S_ALPHA) = done ? 0.0 : SS(g, BN_S_RZ) / a;
+      break;
+    case 2:
+    case 4: {
+      if (done) {
+        SS(g, BN_S_BETA) = 0.0;
+        break;
+      }
+      const double it = SS(g, BN_S_ITER) + 1.0;
+      SS(g, BN_S_ITER) = it;
+      SS(g, BN_S_RZNEW) = a;
+      SS(g, BN_S_RR) = b;
+      const double rel = sqrt(a / SS(g, BN_S_RZ0));
+      SS(g, BN_S_REL) = rel;
+      double bn = sqrt(SS(g, BN_S_BB));
+      if (bn == 0.0) bn = 1.0;
+      const double test = SS(g, BN_S_STOPREL) != 0.0 ? sqrt(b) / bn : rel;
+      if (test < SS(g, BN_S_TOL)) {
+        SS(g, BN_S_DONE) = 1.0;
+        SS(g, BN_S_ITS) = it;
+        SS(g, BN_S_BETA) = 0.0;
+      } else {
+        SS(g, BN_S_BETA) = (which == 4) ? -SS(g, BN_S_ALPHA) * d / SS(g, BN_S_RZ) : a / SS(g, BN_S_RZ);
+      }
+      SS(g, BN_S_RZOLD) = SS(g, BN_S_RZ);
+      SS(g, BN_S_RZ) = a;
+      break;
+    }
+    case 3:
+      SS(g, BN_S_BB) = a;
+      break;
+    default:
+      break;
+  }
+}
+
 // column-major k x n (each right-hand side contiguous) <-> interleaved
 __global__ void k_interleave(int n, int k, int K, const double *__restrict__ src, double *__restrict__ dst,
                              int back) {
@@ -636,6 +772,30 @@ void bn_finish_k(int K, int which, int n, const double *p1, const double *p2, co
                  hipStream_t st) {
   const int nb = bn_block_nb(n);
   BN_K_DISPATCH(K, hipLaunchKernelGGL((k_finish_k<KK>), dim3(KK), dim3(BLOCK), 0, st, which, nb, p1, p2, p3, s));
+}
+
+void bn_dot_seg(int nitem, const int *item_lo, const double *x, const double *y, const double *u, const double *v,
+                double *p1, double *p2, hipStream_t st) {
+  if (nitem <= 0) return;
+  hipLaunchKernelGGL(k_dot_seg, dim3(nitem), dim3(BLOCK), 0, st, item_lo, x, y, u, v, p1, p2);
+}
+
+void bn_update_xr_seg(int nitem, const int *item_lo, const int *item_seg, double *x, const double *p, double *r,
+                      const double *q, const double *s, double *part, hipStream_t st) {
+  if (nitem <= 0) return;
+  hipLaunchKernelGGL(k_update_xr_seg, dim3(nitem), dim3(BLOCK), 0, st, item_lo, item_seg, x, p, r, q, s, part);
+}
+
+void bn_update_p_seg(int nitem, const int *item_lo, const int *item_seg, double *p, const double *z, const double *s,
+                     hipStream_t st) {
+  if (nitem <= 0) return;
+  hipLaunchKernelGGL(k_update_p_seg, dim3(nitem), dim3(BLOCK), 0, st, item_lo, item_seg, p, z, s);
+}
+
+void bn_finish_seg(int nsys, int which, const int *item_ptr, const double *p1, const double *p2, const double *p3,
+                   double *s, hipStream_t st) {
+  if (nsys <= 0) return;
+  hipLaunchKernelGGL(k_finish_seg, dim3(nsys), dim3(BLOCK), 0, st, which, item_ptr, p1, p2, p3, s);
 }
 
 void bn_interleave(int n, int k, int K, const double *src, double *dst, int back, hipStream_t st) {

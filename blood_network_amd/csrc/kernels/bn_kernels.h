@@ -246,6 +246,19 @@ void bn_update_p_k(int K, int n, double *p, const double *z, const double *s, hi
  * and freezing of converged columns in modes 2 and 4 */
 void bn_finish_k(int K, int which, int n, const double *p1, const double *p2, const double *p3,
                  double *s, hipStream_t stream);
+/* Segmented BLAS-1 and scalar logic for a block-diagonal union of nsys
+ * independent systems: work item w = rows [item_lo[w], item_lo[w+1]) of
+ * system item_seg[w]; system g owns items item_ptr[g] .. item_ptr[g+1]-1;
+ * scalars BN_S_COUNT per system; partials one per item. */
+void bn_dot_seg(int nitem, const int *item_lo, const double *x, const double *y, const double *u,
+                const double *v, double *p1, double *p2, hipStream_t stream);
+void bn_update_xr_seg(int nitem, const int *item_lo, const int *item_seg, double *x, const double *p,
+                      double *r, const double *q, const double *s, double *part, hipStream_t stream);
+void bn_update_p_seg(int nitem, const int *item_lo, const int *item_seg, double *p, const double *z,
+                     const double *s, hipStream_t stream);
+void bn_finish_seg(int nsys, int which, const int *item_ptr, const double *p1, const double *p2,
+                   const double *p3, double *s, hipStream_t stream);
+
 /* k right-hand sides stored one after another (k x n) <-> interleaved
  * n x K (columns k..K-1 zero); back = 1 copies the first k columns out */
 void bn_interleave(int n, int k, int K, const double *src, double *dst, int back, hipStream_t stream);

@@ -12,7 +12,7 @@ import numpy as np
 
 from . import _lib as L
 
-__all__ = ["VesselNetwork", "make_network", "WORKLOADS"]
+__all__ = ["VesselNetwork", "make_network", "WORKLOADS", "union_networks", "uneven_batch"]
 
 # name -> (lattice edge, tree depth, trees per kind, coarsening)
 WORKLOADS = {
@@ -105,3 +105,47 @@ def make_network(L_edge, depth, n_trees=1, seed=0, **overrides):
         return VesselNetwork(net, p)
     finally:
         L.lib.bn_netgen_free(C.byref(net))
+
+
+def union_networks(nets, rhs=None):
+    """Stack independent networks into one block-diagonal system.
+
+    Returns (ia, ja, val, b, seg_ptr): the union CSR (rows of network g
+    are seg_ptr[g] .. seg_ptr[g+1]-1, columns shifted by the same offset,
+    no coupling between networks) and its right-hand side (each network's
+    own b, or rhs[g] when given)."""
+    sizes = np.array([net.n for net in nets], dtype=np.int64)
+    seg_ptr = np.zeros(len(nets) + 1, dtype=np.int64)
+    seg_ptr[1:] = np.cumsum(sizes)
+    nnz = np.array([net.nnz for net in nets], dtype=np.int64)
+    if seg_ptr[-1] >= 2**31 or nnz.sum() >= 2**31:
+        raise ValueError("union too large for int32 CSR")
+    ia = np.zeros(seg_ptr[-1] + 1, dtype=np.int32)
+    ja = np.empty(nnz.sum(), dtype=np.int32)
+    val = np.empty(nnz.sum())
+    b = np.empty(seg_ptr[-1])
+    q = 0
+    for g, net in enumerate(nets):
+        r0, r1 = seg_ptr[g], seg_ptr[g + 1]
+        ia[r0 + 1:r1 + 1] = net.ia[1:] + q
+        ja[q:q + net.nnz] = net.ja + r0
+        val[q:q + net.nnz] = net.val
+        b[r0:r1] = net.b if rhs is None else rhs[g]
+        q += net.nnz
+    return ia, ja, val, b, seg_ptr.astype(np.int32)
+
+
+def uneven_batch(count=64, seed=0, edge=(12, 40), depth=(6, 10)):
+    """`count` independent networks with lattice edge ~ U{edge} and tree
+    depth ~ U{depth} (one arterial and one venous tree each), network g
+    generated with seed + g."""
+    rng = np.random.default_rng(seed)
+    Ls = rng.integers(edge[0], edge[1] + 1, count)
+    ds = rng.integers(depth[0], depth[1] + 1, count)
+    nets = []
+    for g, (Le, d) in enumerate(zip(Ls, ds)):
+        # the leaves (2 * 2^(d-1)) must fit on distinct lattice nodes
+        while 2 * 2 ** (int(d) - 1) > int(Le) ** 3:
+            d -= 1
+        nets.append(make_network(int(Le), int(d), 1, seed=seed + g))
+    return nets

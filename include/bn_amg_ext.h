@@ -164,6 +164,19 @@ int bn_solver_solve_batch(void *h, int k, const double *b, double *x, double tol
  * No counterpart in the reference (north-star batching form). */
 int bn_solver_solve_block(void *h, int k, const double *b, double *x, double tol, int maxiter,
                           int *its, bn_solve_info *info);
+/* Many independent systems in one solve: the solver was built on their
+ * block-diagonal union, rows of system g are seg_ptr[g] .. seg_ptr[g+1]-1
+ * (seg_ptr: nsys+1 host ints, 0 .. n).  One V-cycle serves all systems;
+ * each keeps its own PCG scalars (flexible beta) and stop test.  b, x:
+ * n-vectors (x the initial guess, overwritten); its[g] = system g's
+ * iterations (-maxiter if it did not converge); returns the largest count,
+ * negated if any system failed.  No counterpart in the reference
+ * (north-star batching of uneven networks). */
+int bn_solver_solve_segments(void *h, int nsys, const int *seg_ptr, const double *b, double *x,
+                             double tol, int maxiter, int *its, bn_solve_info *info);
+int bn_solver_solve_segments_device(void *h, int nsys, const int *seg_ptr, const double *d_b,
+                                    double *d_x, double tol, int maxiter, int *its,
+                                    bn_solve_info *info);
 /* Same with device pointers. */
 int bn_solver_solve_block_device(void *h, int k, const double *d_b, double *d_x, double tol,
                                  int maxiter, int *its, bn_solve_info *info);

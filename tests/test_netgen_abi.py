@@ -137,3 +137,21 @@ def test_ctypes_mirrors_match_the_c_headers(tmp_path):
             L.AMGData.num_levels.offset, L.AMGData.coarsening.offset, C.sizeof(L.Options),
             L.Options.chunk_rows.offset, C.sizeof(L.SolveInfo)]
     assert got == want
+
+
+def test_union_of_networks_is_block_diagonal():
+    from blood_network_amd import union_networks, uneven_batch
+    nets = uneven_batch(5, seed=3, edge=(4, 9), depth=(3, 6))
+    ia, ja, val, b, seg = union_networks(nets)
+    A = sp.csr_matrix((val, ja, ia), shape=(seg[-1], seg[-1]))
+    assert np.array_equal(np.diff(seg), [net.n for net in nets])
+    for g, net in enumerate(nets):
+        blk = A[seg[g]:seg[g + 1]]
+        # every entry of network g's rows lies in its own diagonal block
+        assert blk.indices.min() >= seg[g] and blk.indices.max() < seg[g + 1]
+        Ag = sp.csr_matrix((net.val, net.ja, net.ia), shape=(net.n, net.n))
+        assert (blk[:, seg[g]:seg[g + 1]] != Ag).nnz == 0
+        assert np.array_equal(b[seg[g]:seg[g + 1]], net.b)
+    # sizes follow the requested ranges; every network fits its leaves
+    Ls = [round((net.n_nodes - 2 * (2 ** net.params.depth - 1)) ** (1 / 3)) for net in nets]
+    assert all(4 <= Le <= 9 for Le in Ls)
