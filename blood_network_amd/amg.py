@@ -185,6 +185,23 @@ class AMGSolver:
         self._check(it)
         return lam, dg, dlr, it
 
+    def adjoint_grad_device(self, p_ptr, dLdp_ptr, m, eu_ptr, ev_ptr, g_ptr, pdir_ptr, nd, lam_ptr,
+                            dLdg_ptr=None, dLdlogr_ptr=None, tol=1e-10, maxiter=1000):
+        """The adjoint gradient on device pointers; lam_ptr is the initial
+        guess on entry.  Returns the adjoint iteration count."""
+        it = L.lib.bn_solver_adjoint_grad_device(self.h, p_ptr, dLdp_ptr, m, eu_ptr, ev_ptr, g_ptr, pdir_ptr,
+                                                 nd, tol, maxiter, lam_ptr, dLdg_ptr, dLdlogr_ptr)
+        self._check(it)
+        info = L.SolveInfo()
+        L.lib.bn_solver_last_info(self.h, C.byref(info))
+        self.report = SolveReport(info)
+        return it
+
+    def set_values_device(self, nnz, val_ptr):
+        """New solve-matrix values (device pointer, nnz entries in the
+        creation order); the preconditioner is kept."""
+        self._check(L.lib.bn_solver_set_values_device(self.h, nnz, val_ptr))
+
     def solve_batch(self, B, tol=1e-8, maxiter=1000):
         """B: (k, n) right-hand sides; returns X (k, n) and per-column counts."""
         B = np.ascontiguousarray(B, dtype=np.float64)

@@ -61,7 +61,9 @@ static void free_tri_host(bn_dtri *t) {
   memset(t, 0, sizeof(*t));
 }
 
-void bn_device_release(struct bn_solver *s) {
+/* every captured step graph goes (they hold device pointers as kernel
+ * arguments); the next solve captures again */
+void bn_drop_graphs(struct bn_solver *s) {
   if (s->stream) hipStreamSynchronize(s->stream);
   bn_segments_release(s);
   for (int k = 0; k < s->ngraph; k++) {
@@ -71,6 +73,17 @@ void bn_device_release(struct bn_solver *s) {
     s->graph[k] = NULL;
   }
   s->ngraph = 0;
+  for (int k = 0; k < s->blk.ngraph; k++) {
+    if (s->blk.gexec[k]) hipGraphExecDestroy(s->blk.gexec[k]);
+    if (s->blk.graph[k]) hipGraphDestroy(s->blk.graph[k]);
+    s->blk.gexec[k] = NULL;
+    s->blk.graph[k] = NULL;
+  }
+  s->blk.ngraph = 0;
+}
+
+void bn_device_release(struct bn_solver *s) {
+  bn_drop_graphs(s);
   for (int k = 0; k < s->ndptr; k++) hipFree(s->dptr[k]);
   free(s->dptr);
   s->dptr = NULL;
@@ -79,10 +92,6 @@ void bn_device_release(struct bn_solver *s) {
   free(s->hptr);
   s->hptr = NULL;
   s->nhptr = s->caphptr = 0;
-  for (int k = 0; k < s->blk.ngraph; k++) {
-    if (s->blk.gexec[k]) hipGraphExecDestroy(s->blk.gexec[k]);
-    if (s->blk.graph[k]) hipGraphDestroy(s->blk.graph[k]);
-  }
   if (s->blk.host_scal) hipHostFree(s->blk.host_scal);
   memset(&s->blk, 0, sizeof(s->blk));
   for (int l = 0; l < AMG_MAX_LEVELS; l++) {
@@ -93,6 +102,9 @@ void bn_device_release(struct bn_solver *s) {
   free_tri_host(&s->fine_ic);
   free_tri_host(&s->coarse_ic);
   memset(&s->Asolve, 0, sizeof(s->Asolve));
+  s->asolve_shared = 0;
+  s->vperm_state = 0;
+  s->vperm_d = NULL;
   s->cinv_d = NULL;
   s->cinv_s = NULL;
   s->dev_ready = 0;

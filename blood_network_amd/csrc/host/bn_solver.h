@@ -119,6 +119,9 @@ struct bn_solver {
   bn_dlevel lev[AMG_MAX_LEVELS];
   bn_dcsr Asolve;       /* FP64 solve matrix in level-0 numbering */
   int asolve_shared;    /* Asolve aliases lev[0].A's FP64 arrays */
+  int vperm_state;      /* caller value order -> Asolve slots: 0 not yet
+                           known, 1 the same order, 2 gathered by vperm_d */
+  int *vperm_d;         /* device: Asolve slot k takes caller slot vperm_d[k] */
   bn_dtri fine_ic;      /* IC(0) of the solve matrix, level-0 numbering */
   int coarse_dense;     /* coarsest solve: 1 dense inverse, 0 IC-PCG */
   double *cinv_d;       /* dense inverse of the coarsest operator */
@@ -173,6 +176,7 @@ static inline void bn_dcsr_apply(const bn_dcsr *m, char vt, int op, int lanes, c
 
 /* segments.c */
 void bn_segments_release(struct bn_solver *s);
+void bn_drop_graphs(struct bn_solver *s);
 
 /* device.c */
 int bn_device_upload(struct bn_solver *s);

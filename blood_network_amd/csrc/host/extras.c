@@ -7,6 +7,8 @@
 #include <string.h>
 
 #include "bn_solver.h"
+#include "csr.h"
+#include "schedule.h"
 
 void bn_set_error(const char *fmt, ...);
 void bn_track_device(struct bn_solver *s, void *p);
@@ -56,6 +58,110 @@ int bn_solver_adjoint_grad(void *h, const double *p, const double *dLdp, int m, 
   hipError_t e = hipStreamSynchronize(st);
   for (int k = 0; k < 8; k++) hipFree(buf[k]);
   return e == hipSuccess ? it : BN_ERR_HIP;
+}
+
+/* The same gradient with every array in device memory: no allocation and
+ * no host copies, for descent loops whose pressures already live in HBM.
+ * d_lam holds the initial guess on entry (zeros for a cold start; the
+ * previous step's multipliers warm-start the next).  nd is the number of
+ * Dirichlet pressures in d_pdir (endpoints -1-d with d < nd). */
+int bn_solver_adjoint_grad_device(void *h, const double *d_p, const double *d_dLdp, int m, const int *d_eu,
+                                  const int *d_ev, const double *d_g, const double *d_pdir, int nd, double tol,
+                                  int maxiter, double *d_lam, double *d_dLdg, double *d_dLdlogr) {
+  struct bn_solver *s = (struct bn_solver *)h;
+  if (!s || !d_p || !d_dLdp || !d_lam || m < 0 || nd < 0 || (nd > 0 && !d_pdir) ||
+      (m > 0 && (!d_eu || !d_ev || (d_dLdlogr && !d_g))))
+    return BN_ERR_ARG;
+  int it = bn_solver_solve_device(h, d_dLdp, d_lam, tol, maxiter, 0, NULL);
+  if (it == BN_ERR_HIP || it == BN_ERR_NO_DEVICE || it == BN_ERR_ARG) return it;
+  if (m == 0 || (!d_dLdg && !d_dLdlogr)) return it;
+  bn_edge_grad(m, d_eu, d_ev, d_g, d_p, d_lam, d_pdir, d_dLdg, d_dLdlogr, s->stream);
+  return hipStreamSynchronize(s->stream) == hipSuccess ? it : BN_ERR_HIP;
+}
+
+/* Caller slot of every stored solve-matrix entry: the stored matrix is
+ * the caller's pattern relabelled by the level-0 row order, so relabelling
+ * a matrix whose values are the slot numbers (exact in FP64) gives the
+ * map.  The same order needs no gather. */
+static int value_map(struct bn_solver *s) {
+  const dcsr_mat *A = s->A;
+  const int n = A->row, nnz = A->nnz;
+  int *inv = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  for (int k = 0; k < n; k++) inv[s->lev[0].perm_h[k]] = k;
+  dcsr_mat I = *A, Ip;
+  I.val = (double *)bn_xmalloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(double));
+  for (int k = 0; k < nnz; k++) I.val[k] = (double)k;
+  bn_csr_permute(&I, s->lev[0].perm_h, inv, &Ip);
+  free(inv);
+  free(I.val);
+  int *map = (int *)bn_xmalloc((size_t)(nnz > 0 ? nnz : 1) * sizeof(int));
+  int same = 1;
+  for (int k = 0; k < nnz; k++) {
+    map[k] = (int)Ip.val[k];
+    same &= (map[k] == k);
+  }
+  bn_csr_free(&Ip);
+  int rc = 0;
+  if (same) {
+    s->vperm_state = 1;
+  } else {
+    void *d = NULL;
+    if (hipMalloc(&d, (size_t)nnz * sizeof(int)) != hipSuccess) {
+      bn_set_error("device out of memory for the value map");
+      rc = BN_ERR_HIP;
+    } else {
+      bn_track_device(s, d);
+      if (hipMemcpy(d, map, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) rc = BN_ERR_HIP;
+      s->vperm_d = (int *)d;
+      s->vperm_state = 2;
+    }
+  }
+  free(map);
+  return rc;
+}
+
+/* New FP64 values for the solve matrix, in device memory, on the pattern
+ * and in the entry order of the stored (symmetrised) matrix: the creation
+ * order whenever the input pattern was symmetric without duplicates, as
+ * for every assembled network; nnz must equal the stored count.  Only the matrix
+ * PCG multiplies by changes: the hierarchy and the fine IC(0) stay those
+ * of the matrix they were built from (two-matrix preconditioning, as
+ * C_amg1/src/amg_wrapper.c:168-199 set_matrix, here without the host
+ * factorisation and the upload).  The host copy of the matrix is not
+ * updated; amg_solver_update / amg_solver_set_matrix replace both. */
+int bn_solver_set_values_device(void *h, int nnz_in, const double *d_val) {
+  struct bn_solver *s = (struct bn_solver *)h;
+  if (!s || !d_val || s->borrowed) return BN_ERR_ARG;
+  if (nnz_in != s->A->nnz) {
+    bn_set_error("set_values: %d values given, the stored solve matrix has %d", nnz_in, s->A->nnz);
+    return BN_ERR_ARG;
+  }
+  int rc = bn_ensure_device(s);
+  if (rc) return rc;
+  if (s->Asolve.nnz != s->A->nnz) return BN_ERR_ARG;
+  if (!s->vperm_state && (rc = value_map(s))) return rc;
+  const size_t nnz = (size_t)s->A->nnz;
+  if (s->asolve_shared) {
+    /* the level-0 V-cycle operator keeps its values: the solve matrix
+       gets FP64 values of its own (pattern arrays stay shared) */
+    void *v = NULL;
+    if (hipMalloc(&v, (nnz ? nnz : 1) * sizeof(double)) != hipSuccess) {
+      bn_set_error("device out of memory for the solve-matrix values");
+      return BN_ERR_HIP;
+    }
+    bn_track_device(s, v);
+    bn_drop_graphs(s);
+    s->Asolve.vd = (double *)v;
+    s->asolve_shared = 0;
+  }
+  hipStream_t st = s->stream;
+  if (s->vperm_state == 1) {
+    if (hipMemcpyAsync(s->Asolve.vd, d_val, nnz * sizeof(double), hipMemcpyDeviceToDevice, st) != hipSuccess)
+      return BN_ERR_HIP;
+  } else {
+    bn_permute((int)nnz, s->vperm_d, d_val, s->Asolve.vd, 0, st);
+  }
+  return hipStreamSynchronize(st) == hipSuccess ? 0 : BN_ERR_HIP;
 }
 
 /* k right-hand sides, column-major; every column is solved to tol with

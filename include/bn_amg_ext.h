@@ -147,6 +147,26 @@ int bn_solver_adjoint_grad(void *h, const double *p, const double *dLdp, int m, 
                            const int *ev, const double *g, const double *pdir, double tol,
                            int maxiter, double *lam, double *dLdg, double *dLdlogr);
 
+/* The same with every array in device memory (no allocation, no host
+ * copies).  d_lam is the initial guess on entry (zeros: cold start) and
+ * the multipliers on return; nd = number of Dirichlet pressures in d_pdir.
+ * Extension: C_amg1 has no gradient; it serves descent loops whose
+ * pressures stay in HBM. */
+int bn_solver_adjoint_grad_device(void *h, const double *d_p, const double *d_dLdp, int m,
+                                  const int *d_eu, const int *d_ev, const double *d_g,
+                                  const double *d_pdir, int nd, double tol, int maxiter,
+                                  double *d_lam, double *d_dLdg, double *d_dLdlogr);
+
+/* ---- new solve-matrix values from device memory: nnz values on the
+ * stored pattern, in its entry order (the creation order when the input
+ * pattern was symmetric without duplicates; nnz must match).  The
+ * hierarchy and the fine IC(0) are kept: the
+ * two-matrix preconditioning of amg_solver_set_matrix
+ * (C_amg1/src/amg_wrapper.c:168-199) without the host IC(0) and the
+ * upload.  For finite-difference probes and descent steps between
+ * hierarchy rebuilds; amg_solver_update rebuilds everything. */
+int bn_solver_set_values_device(void *h, int nnz, const double *d_val);
+
 /* ---- batched solves: k right-hand sides stored column-major (b[j*n+i]),
  * each solved to tol; iterations per column into its[k]. */
 int bn_solver_solve_batch(void *h, int k, const double *b, double *x, double tol, int maxiter,
