@@ -77,6 +77,120 @@ def cpu_baseline(net, coarsening, rtol, maxiter):
                        f"relres {relres:.2e}; set-up {t_setup:.1f} s not included")}, x, it
 
 
+def descent_bench(args, n_obs=10_000, steps=20):
+    """Radius gradients on the 1,065,532-unknown network (100^3 lattice,
+    depth-15 trees): true radii = base x lognormal(0, 0.1), pressures
+    observed at 10,000 random nodes, L = 1/2 sum (p_obs - p)^2.  Times the
+    gradient (forward solve + adjoint solve through the same hierarchy +
+    edge scatter, zero initial guesses, radii fixed) in FP64 and with the
+    FP32 V-cycle in flexible FP64 PCG, each against an FP64 gradient solved
+    to 1e-13; then 20 descent steps with the hierarchy rebuilt per radius
+    set and with it kept (device value swap).  The CPU leg: the compiled
+    reference's forward and adjoint PCG solves on one host core."""
+    import numpy as np
+
+    from blood_network_amd import AMGSolver, PressureFit, make_network
+
+    net = make_network(100, 15, 1, seed=0)
+    rng = np.random.default_rng(4)
+    logr0 = np.log(net.radius)
+    logr_true = logr0 + rng.normal(0.0, 0.1, net.n_edges)
+    obs = rng.choice(net.n, size=n_obs, replace=False)
+    vt, bt = net.assemble(net.conductance(np.exp(logr_true)))
+    st = AMGSolver(net.ia, net.ja, vt, coarsening="pmis")
+    p_obs = st.solve(bt, tol=1e-13, maxiter=500)[0][obs]
+    st.free()
+    ref = PressureFit(net, obs, p_obs, tol=1e-13, coarsening="pmis")
+    f_ref, g_ref = ref.value_and_grad(logr0)
+    ref.free()
+    log(f"[descent] n={net.n} segments={net.n_edges}; loss at base radii {f_ref:.6e}")
+    modes = {"fp64": dict(coarsening="pmis"),
+             "fp32_flex": dict(coarsening="pmis", precision="fp32", fine_ic="jacobi", ic_sweeps=FAST_IC_SWEEPS,
+                               max_colours=FAST_COLOURS)}
+    grad = {}
+    for name, kw in modes.items():
+        fit = PressureFit(net, obs, p_obs, tol=args.rtol, **kw)
+        fit.value_and_grad(logr0)  # set-up, upload, graph capture
+        wall = []
+        for rep in range(args.warmup + args.steps):
+            fit.cold_start()
+            t0 = time.perf_counter()
+            f = fit.forward()
+            g = fit.grad()
+            if rep >= args.warmup:
+                wall.append((time.perf_counter() - t0) * 1e3)
+        rec = fit.log[-1]
+        err = float(np.linalg.norm(g - g_ref) / np.linalg.norm(g_ref))
+        grad[name] = {"ms": round(float(np.median(wall)), 3), "forward_its": rec["forward_its"],
+                      "adjoint_its": rec["adjoint_its"], "forward_device_ms": round(rec["forward_ms"], 3),
+                      "adjoint_device_ms": round(rec["adjoint_ms"], 3), "rel_err_vs_fp64_1e-13": err,
+                      "loss_rel_err": abs(f - f_ref) / f_ref}
+        log(f"[descent] gradient {name}: {grad[name]}")
+        fit.free()
+    descents = {}
+    for refresh in ("rebuild", "values"):
+        fit = PressureFit(net, obs, p_obs, tol=args.rtol, refresh=refresh, **modes["fp32_flex"])
+        t0 = time.perf_counter()
+        _, losses = fit.descend(logr0, steps=steps)
+        total = time.perf_counter() - t0
+        evals = fit.log
+        descents[refresh] = {
+            "losses": [float("%.6e" % x) for x in losses], "seconds": round(total, 2),
+            "step_ms_median": round(float(np.median([s["wall_ms"] for s in fit.steps])), 1),
+            "refresh_ms_median": round(float(np.median([e["refresh_ms"] for e in evals[1:]])), 1),
+            "forward_its": [e["forward_its"] for e in evals],
+            "trials": [s["trials"] for s in fit.steps]}
+        log(f"[descent] {steps} steps, refresh={refresh}: loss {losses[0]:.4e} -> {losses[-1]:.4e} "
+            f"in {total:.1f} s; median step {descents[refresh]['step_ms_median']} ms")
+        fit.free()
+    # CPU leg last (it pins the process to one core)
+    cpu = None
+    if not args.no_cpu:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        from oracle_lib import oracle, reference
+        hx, kind = reference(), "reference"
+        if hx is None:
+            hx, kind = oracle(), "port"
+        try:
+            os.sched_setaffinity(0, {sorted(os.sched_getaffinity(0))[0]})
+        except (AttributeError, OSError):
+            pass
+        g0 = net.conductance(np.exp(logr0))
+        v0, b0 = net.assemble(g0)
+        h = hx.build(net.ia, net.ja, v0, coarsening="pmis")
+        t0 = time.perf_counter()
+        p, it1, _ = h.solve(b0, args.rtol, 1000)
+        d = np.zeros(net.n)
+        np.add.at(d, obs, p[obs] - p_obs)
+        lam, it2, _ = h.solve(d, args.rtol, 1000)
+        pd = net.p_dirichlet
+        pu = np.where(net.eu >= 0, p[np.maximum(net.eu, 0)], pd[np.maximum(-1 - net.eu, 0)])
+        pv = np.where(net.ev >= 0, p[np.maximum(net.ev, 0)], pd[np.maximum(-1 - net.ev, 0)])
+        lu = np.where(net.eu >= 0, lam[np.maximum(net.eu, 0)], 0.0)
+        lv = np.where(net.ev >= 0, lam[np.maximum(net.ev, 0)], 0.0)
+        gc = 4.0 * g0 * (-(lu - lv) * (pu - pv))
+        sec = time.perf_counter() - t0
+        cpu = {"value": round(sec * 1e3, 1), "unit": "ms", "cores": 1, "kind": kind,
+               "sample": (f"forward + adjoint PCG solves (AMG+IC, B-norm stop {args.rtol:g}; {it1} + {it2} "
+                          f"iterations) and the numpy edge formula on the same system, 1 core of {cpu_model()}; "
+                          f"gradient rel. error vs the FP64 1e-13 GPU gradient "
+                          f"{float(np.linalg.norm(gc - g_ref) / np.linalg.norm(g_ref)):.2e}")}
+        log(f"[descent] CPU gradient {cpu['value']} ms")
+    value = grad["fp32_flex"]["ms"]
+    out = {
+        "metric": "radius gradient, 1.07M unknowns / 3.07M segments (forward + adjoint + edge scatter)",
+        "value": value, "unit": "ms", "higher_is_better": False, "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "data": "synthetic (deterministic generator, seed 0; observations seed 4)",
+        "dtype": "fp32 V-cycle / fp64 PCG and scatter", "vs_baseline": None,
+        "config": {"workload": "lattice100_trees15 radius fit", "unknowns": net.n, "segments": net.n_edges,
+                   "observations": n_obs, "rtol_bnorm": args.rtol, "gradient": grad, "descent": descents},
+        "cpu_baseline": cpu,
+    }
+    if cpu:
+        out["config"]["speedup_vs_cpu_baseline"] = round(cpu["value"] / value, 1)
+    print(json.dumps(out), flush=True)
+
+
 def batch_bench(args, k=32, workload="lattice100_16x16trees11"):
     """Block PCG over k right-hand sides (one solve, CSR SpMM) against a
     loop of k single-right-hand-side solves of the same systems, FP32
@@ -217,13 +331,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="lattice160_trees17")
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--rtol", type=float, default=1e-8)
+    ap.add_argument("--rtol", type=float, default=None,
+                    help="PCG tolerance (default 1e-8; 1e-10 B-norm for --descent)")
     ap.add_argument("--kernels-only", action="store_true",
                     help="build the hierarchy, time the fine-level kernels and the copy "
                          "calibration kernel, print them as one JSON line (for counter runs)")
     ap.add_argument("--batch", action="store_true",
                     help="measure the block solve instead: k=32 right-hand sides on the 1M-unknown "
                          "network with 16+16 depth-11 trees, block PCG vs a loop of single solves "
+                         "(prints its own JSON line)")
+    ap.add_argument("--descent", action="store_true",
+                    help="radius gradients and 20 descent steps on the 1.07M-unknown network "
                          "(prints its own JSON line)")
     ap.add_argument("--networks", action="store_true",
                     help="measure 64 uneven networks (lattice edge U{12..40}, tree depth U{6..10}) solved "
@@ -232,6 +350,8 @@ def main():
                     help="enqueue each PCG step kernel by kernel instead of replaying the captured "
                          "hipGraph (for kernel-tracing profilers)")
     args = ap.parse_args()
+    if args.rtol is None:
+        args.rtol = 1e-10 if args.descent else 1e-8
     if args.no_graph:
         os.environ["BN_NO_GRAPH"] = "1"
     if args.gpus != 1:
@@ -250,6 +370,8 @@ def main():
         return batch_bench(args)
     if args.networks:
         return networks_bench(args)
+    if args.descent:
+        return descent_bench(args)
 
     Ledge, depth, ntrees, coarsening = WORKLOADS[args.workload]
     t0 = time.perf_counter()

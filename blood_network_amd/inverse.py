@@ -76,7 +76,17 @@ class PressureFit:
     def loss(self, logr):
         """L at log-radii ``logr``; the pressures stay on the device for a
         following ``grad()`` (the previous pressures are the initial guess)."""
-        rec = self._set_radii(np.asarray(logr, dtype=np.float64))
+        return self.forward(self._set_radii(np.asarray(logr, dtype=np.float64)))
+
+    def cold_start(self):
+        """Zero initial guesses for the next forward and adjoint solves."""
+        zeros = np.zeros(self.net.n)
+        self.d_p.upload(zeros)
+        self.d_lam.upload(zeros)
+
+    def forward(self, rec=None):
+        """Pressures and L for the radii the solver holds."""
+        rec = {} if rec is None else rec
         it = self.solver.solve_device(self.d_b.ptr, self.d_p.ptr, tol=self.tol, maxiter=self.maxiter)
         if it < 0:
             raise RuntimeError(f"forward solve did not converge ({-it} iterations)")
