@@ -95,6 +95,8 @@ COARSEN_RS, COARSEN_PMIS, COARSEN_HMIS = 0, 1, 2
 PREC_FP64, PREC_FP32 = 0, 1
 COARSE_AUTO, COARSE_DENSE, COARSE_ICPCG = 0, 1, 2
 IC_EXACT, IC_JACOBI, IC_NONE = 0, 1, 2
+# lanes-argument bits of the SpMV launchers: 8 or 16 load rounds per lane
+LANES_DEEP8, LANES_DEEP16 = 0x100, 0x200
 STOP_BNORM, STOP_RELRES = 0, 1
 
 

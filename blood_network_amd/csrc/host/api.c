@@ -376,6 +376,7 @@ void amg_solver_free(void *h) {
   struct bn_solver *s = (struct bn_solver *)h;
   if (!s) return;
   release_all(s);
+  free(s->tune);
   if (!s->borrowed) {
     ichol_free(&s->L_own);
     amg_free(&s->amg_own);

@@ -36,6 +36,8 @@ typedef struct {
   int *blk;        /* device: nblk + 1 row boundaries                  */
   int adapt;       /* FP64 apply uses the row-block kernel (autotuned) */
   int adapt_s;     /* FP32 likewise                                    */
+  int deep;        /* FP64 SpMV load-round variant (BN_LANES_DEEP* bit) */
+  int deep_s;      /* FP32 likewise                                    */
 } bn_dcsr;
 
 /* level-scheduled triangular factor L (forward) and L^T (backward) */
@@ -119,6 +121,9 @@ struct bn_solver {
   bn_dlevel lev[AMG_MAX_LEVELS];
   bn_dcsr Asolve;       /* FP64 solve matrix in level-0 numbering */
   int asolve_shared;    /* Asolve aliases lev[0].A's FP64 arrays */
+  int *tune;            /* tuned kernel choices of the last upload (6 ints
+                           per operator), reused while the shapes match */
+  int ntune;
   int vperm_state;      /* caller value order -> Asolve slots: 0 not yet
                            known, 1 the same order, 2 gathered by vperm_d */
   int *vperm_d;         /* device: Asolve slot k takes caller slot vperm_d[k] */
@@ -157,13 +162,36 @@ struct bn_solver {
 static inline int bn_lanes(const bn_dcsr *m, char vt) { return vt == 's' ? m->lanes_s : m->lanes; }
 
 /* long-row bin for a lanes-per-row kernel: rows over 16 strides */
-static inline int bn_long_len(int lanes) { return 16 * lanes; }
+/* Rows longer than 16 x this width go to the long-row bin.  The default
+ * kernel covers 4 load rounds of `lanes` entries per pass; the 8- and
+ * 16-round variants cover 2x and 4x as many, so their bin starts 2x and
+ * 4x further out (the precomputed bins reach 16 x 64 entries). */
+static inline int bn_long_lanes(int lanes) {
+  const int l = lanes & BN_LANES_MASK;
+  const int w = (lanes & BN_LANES_DEEP16) ? 4 * l : (lanes & BN_LANES_DEEP8) ? 2 * l : l;
+  return w > 64 ? 64 : w;
+}
+static inline int bn_long_len(int lanes) { return 16 * bn_long_lanes(lanes); }
 static inline int bn_nlong(const bn_dcsr *m, int lanes) {
+  lanes = bn_long_lanes(lanes);
   int k = 0;
   while ((1 << k) < lanes && k < 6) k++;
   return m->nlong_for[k];
 }
-/* y = op(M, x [, b]) with a lanes-per-row kernel and the segmented bin */
+/* lanes for the SpMV of M, with its tuned load-round variant */
+static inline int bn_spmv_lanes(const bn_dcsr *m, char vt) {
+  return vt == 's' ? (m->lanes_s | m->deep_s) : (m->lanes | m->deep);
+}
+/* y = op(M, x [, b]), optionally with the fused w.y partials, by the
+ * tuned lanes kernel (no long-row bin: for the fine solve matrix, whose
+ * rows the lanes kernel covers) */
+static inline void bn_dcsr_apply_dot(const bn_dcsr *m, char vt, int op, const void *x, void *y, const void *b,
+                                     const void *dotw, double *partial, hipStream_t st) {
+  bn_csr_apply(vt, op, m->rows, m->ia, m->ja, vt == 's' ? (const void *)m->vs : (const void *)m->vd, x, y, b,
+               bn_spmv_lanes(m, vt), dotw, partial, st);
+}
+/* y = op(M, x [, b]) with a lanes-per-row kernel and the segmented bin
+ * (lanes may carry a BN_LANES_DEEP* bit) */
 static inline void bn_dcsr_apply(const bn_dcsr *m, char vt, int op, int lanes, const void *x, void *y,
                                  const void *b, hipStream_t st) {
   const int nl = bn_nlong(m, lanes);
@@ -172,6 +200,17 @@ static inline void bn_dcsr_apply(const bn_dcsr *m, char vt, int op, int lanes, c
   bn_csr_apply_seg(vt, op, m->rows, m->ia, m->ja, vt == 's' ? (const void *)m->vs : (const void *)m->vd, x, y, b,
                    lanes, bn_long_len(lanes), nl, m->long_rows, m->seg_ptr, m->seg_row, nseg, nmulti, m->seg_part,
                    st);
+}
+/* y = op(M, x [, b]) with M's tuned kernel: row blocks, or lanes per row
+ * with the long-row bin */
+static inline void bn_op_apply(const bn_dcsr *m, char vt, int op, const void *x, void *y, const void *b,
+                               hipStream_t st) {
+  if (vt == 's' ? m->adapt_s : m->adapt) {
+    bn_csr_apply_blocks(vt, op, m->nblk, m->blk, m->ia, m->ja, vt == 's' ? (const void *)m->vs : (const void *)m->vd,
+                        x, y, b, st);
+    return;
+  }
+  bn_dcsr_apply(m, vt, op, bn_spmv_lanes(m, vt), x, y, b, st);
 }
 
 /* segments.c */

@@ -75,6 +75,7 @@ static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
   d->cols = m->col;
   d->nnz = m->nnz;
   d->lanes = d->lanes_s = lanes_for(m->nnz, m->row);
+  d->deep = d->deep_s = 0;
   d->ia = (int *)dmalloc(s, ((size_t)m->row + 1) * sizeof(int));
   d->ja = (int *)dmalloc(s, (size_t)m->nnz * sizeof(int));
   d->vd = (double *)dmalloc(s, (size_t)m->nnz * sizeof(double));
@@ -244,53 +245,93 @@ static int upload_tri(struct bn_solver *s, const ichol_factor *F, const int *per
   return rc;
 }
 
-/* Autotuner: time the operator's SpMV for a few lanes-per-row choices
- * around the heuristic, in both precisions, and keep the fastest.  x and
- * y are scratch device buffers of at least cols / rows doubles. */
-static void autotune_csr(struct bn_solver *s, bn_dcsr *d, void *x, void *y, int allow_blocks) {
+/* Autotuner: time the operator's SpMV for every lanes-per-row choice
+ * (1..64; from 4 lanes up also with 8 and 16 load rounds) whose long-row
+ * bin stays small, and the row-block kernel, in both precisions, and
+ * keep the fastest.
+ * Every timed launch starts cold: a read of a buffer larger than the
+ * last-level cache first evicts the operator, as the rest of a V-cycle
+ * does between two uses of a coarse operator (timing repeated launches
+ * would serve operators below the cache size from cache and favour
+ * kernels that are slow from HBM).  x and y are scratch device buffers
+ * of at least cols / rows doubles; flush (nflush doubles) may be NULL. */
+#define BN_TUNE_REPS 5
+typedef struct {
+  const double *flush;
+  int nflush;
+  hipEvent_t ev[2 * BN_TUNE_REPS + 2];
+} bn_tune_env;
+
+/* total device time of reps cold launches (one warm-up launch first);
+ * every launch has its own event pair, read after one synchronisation */
+static float time_launches(struct bn_solver *s, const bn_tune_env *env, bn_dcsr *d, char vt, int cand,
+                           void *x, void *y, int reps) {
+  const void *val = vt == 's' ? (const void *)d->vs : (const void *)d->vd;
+  for (int r = -1; r < reps; r++) {
+    if (env->flush) bn_dot_partial(env->nflush, env->flush, env->flush, NULL, NULL, s->part1, NULL, s->stream);
+    if (r >= 0) hipEventRecord(env->ev[2 * r], s->stream);
+    if (cand > 0)
+      bn_dcsr_apply(d, vt, BN_OP_SET, cand, x, y, NULL, s->stream);
+    else
+      bn_csr_apply_blocks(vt, BN_OP_SET, d->nblk, d->blk, d->ia, d->ja, val, x, y, NULL, s->stream);
+    if (r >= 0) hipEventRecord(env->ev[2 * r + 1], s->stream);
+  }
+  hipEventSynchronize(env->ev[2 * reps - 1]);
+  float total = 0.0f;
+  for (int r = 0; r < reps; r++) {
+    float ms = 0.0f;
+    hipEventElapsedTime(&ms, env->ev[2 * r], env->ev[2 * r + 1]);
+    total += ms;
+  }
+  return total;
+}
+
+static void autotune_csr(struct bn_solver *s, const bn_tune_env *env, bn_dcsr *d, void *x, void *y,
+                         int allow_blocks, int allow_deep) {
   if (!d->ia || d->rows <= 0 || d->nnz <= 0) return;
   const int base = d->lanes;
-  int cand[7], nc = 0;
-  const int opts[6] = {1, base / 4, base / 2, base, base * 2, base * 4};
-  for (int k = 0; k < 6; k++) {
+  int cand[24], nc = 0;
+  const int opts[7] = {1, 2, 4, 8, 16, 32, 64};
+  for (int k = 0; k < 7; k++) {
     int l = opts[k];
     if (l < 1 || l > 64) continue;
     int dup = 0;
     for (int q = 0; q < nc; q++) dup |= (cand[q] == l);
     if (!dup) cand[nc++] = l;
   }
-  if (allow_blocks && d->blk) cand[nc++] = 0; /* 0: the row-block kernel */
+  if (allow_deep) {
+    const int n0 = nc;
+    for (int k = 0; k < n0; k++)
+      if (cand[k] >= 4) {
+        cand[nc++] = cand[k] | BN_LANES_DEEP8;
+        cand[nc++] = cand[k] | BN_LANES_DEEP16;
+      }
+  }
   /* a lanes choice that sends more than 1/64 of the rows to the long
    * bin is never competitive: drop it (keeps at least one candidate) */
   {
     int keep = 0;
     for (int k = 0; k < nc; k++)
-      if (cand[k] == 0 || (long long)bn_nlong(d, cand[k]) * 64 <= d->rows || cand[k] == 64) cand[keep++] = cand[k];
+      if ((long long)bn_nlong(d, cand[k]) * 64 <= d->rows || (cand[k] & BN_LANES_MASK) == 64) cand[keep++] = cand[k];
     if (keep > 0) nc = keep;
   }
+  if (allow_blocks && d->blk) cand[nc++] = 0; /* 0: the row-block kernel */
   for (int pv = 0; pv < 2; pv++) {
     const char vt = pv ? 's' : 'd';
-    const void *val = pv ? (const void *)d->vs : (const void *)d->vd;
     float best = 1e30f, best_lanes_ms = 1e30f;
     int bl = base, use_blocks = 0;
-    /* enough repetitions that launch overhead does not decide the
-     * choice for small operators (about 200 MB of traffic per candidate) */
+    /* a few cold launches each; more for small operators, whose times
+     * are short and noisier */
     const double opb = (double)d->nnz * (4.0 + (pv ? 4.0 : 8.0)) + (double)d->rows * 12.0;
-    int reps = (int)(2e8 / (opb > 1.0 ? opb : 1.0));
+    int reps = (int)(1e8 / (opb > 1.0 ? opb : 1.0));
     if (reps < 3) reps = 3;
-    if (reps > 40) reps = 40;
+    if (reps > BN_TUNE_REPS) reps = BN_TUNE_REPS;
     for (int k = 0; k < nc; k++) {
-      for (int r = -1; r < reps; r++) {
-        if (r == 0) hipEventRecord(s->ev0, s->stream);
-        if (cand[k] > 0)
-          bn_dcsr_apply(d, vt, BN_OP_SET, cand[k], x, y, NULL, s->stream);
-        else
-          bn_csr_apply_blocks(vt, BN_OP_SET, d->nblk, d->blk, d->ia, d->ja, val, x, y, NULL, s->stream);
-      }
-      hipEventRecord(s->ev1, s->stream);
-      hipEventSynchronize(s->ev1);
-      float ms = 0.0f;
-      hipEventElapsedTime(&ms, s->ev0, s->ev1);
+      const float ms = time_launches(s, env, d, vt, cand[k], x, y, reps);
+      const char *lgc = getenv("BN_TUNE_LOG");
+      if (lgc && lgc[0] == '2')
+        fprintf(stderr, "  cand %c rows=%d nnz=%d lanes=%d rounds=%d blocks=%d: %.1f us\n", vt, d->rows, d->nnz,
+                cand[k] & BN_LANES_MASK, cand[k] >> 8, cand[k] == 0, 1e3 * ms / reps);
       if (cand[k] > 0 && ms < best_lanes_ms) {
         best_lanes_ms = ms;
         bl = cand[k];
@@ -300,11 +341,19 @@ static void autotune_csr(struct bn_solver *s, bn_dcsr *d, void *x, void *y, int 
         use_blocks = cand[k] == 0;
       }
     }
+    const int lanes = bl & BN_LANES_MASK, deep = bl & ~BN_LANES_MASK;
+    const char *lg = getenv("BN_TUNE_LOG");
+    if (lg && (lg[0] == '1' || lg[0] == '2'))
+      fprintf(stderr, "tune %c rows=%d nnz=%d: %s lanes=%d rounds=%s  (%.1f us per cold launch)\n", vt, d->rows,
+              d->nnz, use_blocks ? "row-block" : "lanes", lanes,
+              deep == BN_LANES_DEEP16 ? "16" : deep == BN_LANES_DEEP8 ? "8" : "default", 1e3 * best / reps);
     if (pv) {
-      d->lanes_s = bl;
+      d->lanes_s = lanes;
+      d->deep_s = deep;
       d->adapt_s = use_blocks;
     } else {
-      d->lanes = bl;
+      d->lanes = lanes;
+      d->deep = deep;
       d->adapt = use_blocks;
     }
   }
@@ -321,6 +370,7 @@ static int apply_mode(void) {
   return 0;
 }
 
+
 static void pin_blocks(struct bn_solver *s) {
   for (int l = 0; l < s->nlev; l++) {
     bn_dcsr *ops[3] = {&s->lev[l].A, &s->lev[l].P, &s->lev[l].R};
@@ -329,23 +379,77 @@ static void pin_blocks(struct bn_solver *s) {
   }
 }
 
-static void autotune_all(struct bn_solver *s) {
-  void *x = s->x, *y = s->q;
-  bn_fill('d', s->lev[0].n, x, 1.0, s->stream);
+/* the tuned operators in a fixed order (NULL entries skipped) */
+static int tune_list(struct bn_solver *s, bn_dcsr **ops, int *flags) {
+  int n = 0;
   for (int l = 0; l < s->nlev; l++) {
     const int allow = apply_mode() == 0;
-    autotune_csr(s, &s->lev[l].A, x, y, allow);
-    autotune_csr(s, &s->lev[l].P, x, y, allow);
-    autotune_csr(s, &s->lev[l].R, x, y, allow);
-    autotune_csr(s, &s->lev[l].Ac, x, y, 0);
+    ops[n] = &s->lev[l].A, flags[n++] = allow ? 3 : 2;
+    ops[n] = &s->lev[l].P, flags[n++] = allow ? 3 : 2;
+    ops[n] = &s->lev[l].R, flags[n++] = allow ? 3 : 2;
+    ops[n] = &s->lev[l].Ac, flags[n++] = 0;
   }
-  /* the outer PCG's q = A p fuses p'q into the lanes kernel */
-  if (!s->asolve_shared) autotune_csr(s, &s->Asolve, x, y, 0);
-  else s->Asolve = s->lev[0].A;
-  autotune_csr(s, &s->fine_ic.lo, x, y, 0);
-  autotune_csr(s, &s->fine_ic.up, x, y, 0);
-  autotune_csr(s, &s->coarse_ic.lo, x, y, 0);
-  autotune_csr(s, &s->coarse_ic.up, x, y, 0);
+  /* the outer PCG's q = A p fuses p'q into the lanes kernel (no row blocks) */
+  ops[n] = s->asolve_shared ? NULL : &s->Asolve, flags[n++] = 2;
+  ops[n] = &s->fine_ic.lo, flags[n++] = 0;
+  ops[n] = &s->fine_ic.up, flags[n++] = 0;
+  ops[n] = &s->coarse_ic.lo, flags[n++] = 0;
+  ops[n] = &s->coarse_ic.up, flags[n++] = 0;
+  return n;
+}
+
+#define BN_TUNE_FIELDS 8
+static void tune_save(const bn_dcsr *d, int *t) {
+  t[0] = d->rows, t[1] = d->nnz, t[2] = d->lanes, t[3] = d->lanes_s;
+  t[4] = d->deep, t[5] = d->deep_s, t[6] = d->adapt, t[7] = d->adapt_s;
+}
+
+static void autotune_all(struct bn_solver *s) {
+  bn_dcsr *ops[4 * AMG_MAX_LEVELS + 8];
+  int flags[4 * AMG_MAX_LEVELS + 8];
+  const int nops = tune_list(s, ops, flags);
+  /* a rebuild on the same pattern (amg_solver_update) keeps the choices */
+  int reuse = s->tune && s->ntune == nops;
+  for (int k = 0; reuse && k < nops; k++) {
+    const int *t = s->tune + BN_TUNE_FIELDS * k;
+    if (ops[k] && (t[0] != ops[k]->rows || t[1] != ops[k]->nnz)) reuse = 0;
+  }
+  if (reuse) {
+    for (int k = 0; k < nops; k++) {
+      const int *t = s->tune + BN_TUNE_FIELDS * k;
+      if (!ops[k] || !ops[k]->ia) continue;
+      ops[k]->lanes = t[2], ops[k]->lanes_s = t[3], ops[k]->deep = t[4], ops[k]->deep_s = t[5];
+      ops[k]->adapt = t[6], ops[k]->adapt_s = t[7];
+    }
+  } else {
+    void *x = s->x, *y = s->q;
+    bn_fill('d', s->lev[0].n, x, 1.0, s->stream);
+    /* eviction buffer: 320 MB, above the 256 MB last-level cache */
+    bn_tune_env env;
+    memset(&env, 0, sizeof(env));
+    env.nflush = 40 << 20;
+    void *fb = NULL;
+    if (hipMalloc(&fb, (size_t)env.nflush * sizeof(double)) == hipSuccess) {
+      bn_fill('d', env.nflush, fb, 0.0, s->stream);
+      env.flush = (const double *)fb;
+    }
+    int evok = 1;
+    for (int k = 0; k < 2 * BN_TUNE_REPS + 2; k++) evok &= hipEventCreate(&env.ev[k]) == hipSuccess;
+    for (int k = 0; evok && k < nops; k++)
+      if (ops[k]) autotune_csr(s, &env, ops[k], x, y, flags[k] & 1, (flags[k] & 2) != 0);
+    hipStreamSynchronize(s->stream);
+    for (int k = 0; k < 2 * BN_TUNE_REPS + 2; k++)
+      if (env.ev[k]) hipEventDestroy(env.ev[k]);
+    if (fb) hipFree(fb);
+    free(s->tune);
+    s->tune = (int *)bn_xmalloc((size_t)nops * BN_TUNE_FIELDS * sizeof(int));
+    s->ntune = nops;
+    for (int k = 0; k < nops; k++) {
+      if (ops[k]) tune_save(ops[k], s->tune + BN_TUNE_FIELDS * k);
+      else memset(s->tune + BN_TUNE_FIELDS * k, 0, BN_TUNE_FIELDS * sizeof(int));
+    }
+  }
+  if (s->asolve_shared) s->Asolve = s->lev[0].A;
   hipStreamSynchronize(s->stream);
 }
 

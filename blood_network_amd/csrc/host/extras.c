@@ -211,7 +211,7 @@ int bn_solver_time_kernel(void *h, int which, int reps, double *ms_avg, double *
     if (r == 1) hipEventRecord(s->ev0, st);
     switch (which) {
       case 0:
-        bn_csr_apply('d', BN_OP_SET, n, A->ia, A->ja, A->vd, s->p, s->q, NULL, A->lanes, NULL, NULL, st);
+        bn_dcsr_apply_dot(A, 'd', BN_OP_SET, s->p, s->q, NULL, NULL, NULL, st);
         b = (double)A->nnz * 12.0 + (n + 1.0) * 4.0 + 2.0 * n * 8.0;
         break;
       case 1: {
@@ -235,9 +235,7 @@ int bn_solver_time_kernel(void *h, int which, int reps, double *ms_avg, double *
         break;
       }
       case 2: {
-        const void *val = vt == 's' ? (const void *)d->A.vs : (const void *)d->A.vd;
-        bn_csr_apply(vt, BN_OP_RESID, n, d->A.ia, d->A.ja, val, s->u1, s->r1, s->g_in, bn_lanes(&d->A, vt),
-                     NULL, NULL, st);
+        bn_op_apply(&d->A, vt, BN_OP_RESID, s->u1, s->r1, s->g_in, st);
         b = (double)d->A.nnz * (4.0 + v) + (n + 1.0) * 4.0 + 3.0 * n * v;
         break;
       }

@@ -55,6 +55,7 @@ static void destroy(entry *e) {
       bn_device_release(e->s);
       if (e->s->host_scal) hipHostFree(e->s->host_scal);
       if (e->s->stream) hipStreamDestroy(e->s->stream);
+      free(e->s->tune);
       free(e->s);
     }
   }
@@ -290,7 +291,7 @@ INT amg_pcg(const dcsr_mat *A, const REAL *b, REAL *x,
   double *hr = e->hv, *hz = e->hv + n, *sc = s->scal, *hs = s->host_scal;
   hipMemcpyAsync(db, b, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st);
   hipMemcpyAsync(dx, x, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st);
-  bn_csr_apply('d', BN_OP_RESID, n, M->ia, M->ja, M->vd, dx, r, db, M->lanes, NULL, NULL, st);
+  bn_csr_apply('d', BN_OP_RESID, n, M->ia, M->ja, M->vd, dx, r, db, M->lanes | M->deep, NULL, NULL, st);
   bn_dot_partial(n, db, db, NULL, NULL, s->part1, NULL, st);
   bn_scalar_finish(3, s->part1, NULL, NULL, sc, 0, st);
 #define CALLBACK()                                                                            \
@@ -317,7 +318,7 @@ INT amg_pcg(const dcsr_mat *A, const REAL *b, REAL *x,
   INT it = 0;
   if (hs[BN_S_RZ0] != 0.0) {
     for (it = 1; it <= maxiter; it++) {
-      bn_csr_apply('d', BN_OP_SET, n, M->ia, M->ja, M->vd, p, q, NULL, M->lanes, p, s->part1, st);
+      bn_csr_apply('d', BN_OP_SET, n, M->ia, M->ja, M->vd, p, q, NULL, M->lanes | M->deep, p, s->part1, st);
       bn_scalar_finish(1, s->part1, NULL, NULL, sc, 0, st);
       bn_pcg_update_xr(n, dx, p, r, q, sc, s->part2, st);
       CALLBACK();

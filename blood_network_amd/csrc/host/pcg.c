@@ -39,7 +39,7 @@ static void enqueue_step(struct bn_solver *s) {
   const int flex = s->opt.flexible;
   bn_pcg_update_p(n, s->p, s->z, sc, st);
   /* q = A p, p'q -> alpha */
-  bn_csr_apply('d', BN_OP_SET, n, A->ia, A->ja, A->vd, s->p, s->q, NULL, A->lanes, s->p, s->part1, st);
+  bn_dcsr_apply_dot(A, 'd', BN_OP_SET, s->p, s->q, NULL, s->p, s->part1, st);
   bn_scalar_finish(1, s->part1, NULL, NULL, sc, 0, st);
   bn_pcg_update_xr(n, s->x, s->p, s->r, s->q, sc, s->part2, st);
   bn_vc_precond(s, s->r, s->z);
@@ -96,7 +96,7 @@ int bn_pcg_run(struct bn_solver *s, const double *d_b, double *d_x, double tol, 
   bn_permute(n, s->lev[0].perm_d, d_b, s->bb, 0, st);
   bn_permute(n, s->lev[0].perm_d, d_x, s->x, 0, st);
   /* r = b - A x ; ||b||^2 */
-  bn_csr_apply('d', BN_OP_RESID, n, A->ia, A->ja, A->vd, s->x, s->r, s->bb, A->lanes, NULL, NULL, st);
+  bn_dcsr_apply_dot(A, 'd', BN_OP_RESID, s->x, s->r, s->bb, NULL, NULL, st);
   bn_dot_partial(n, s->bb, s->bb, NULL, NULL, s->part1, NULL, st);
   bn_scalar_finish(3, s->part1, NULL, NULL, sc, 0, st);
   bn_vc_precond(s, s->r, s->z);

@@ -126,10 +126,9 @@ static bn_segs *segs_for(struct bn_solver *s, int nsys, const int *seg_ptr) {
 
 static void enqueue_step_seg(struct bn_solver *s, bn_segs *g) {
   hipStream_t st = s->stream;
-  const int n = s->lev[0].n;
   const bn_dcsr *A = &s->Asolve;
   bn_update_p_seg(g->nitem, g->item_lo, g->item_seg, s->p, s->z, g->scal, st);
-  bn_csr_apply('d', BN_OP_SET, n, A->ia, A->ja, A->vd, s->p, s->q, NULL, A->lanes, NULL, NULL, st);
+  bn_dcsr_apply_dot(A, 'd', BN_OP_SET, s->p, s->q, NULL, NULL, NULL, st);
   bn_dot_seg(g->nitem, g->item_lo, s->p, s->q, NULL, NULL, g->part1, NULL, st);
   bn_finish_seg(g->nsys, 1, g->item_ptr, g->part1, NULL, NULL, g->scal, st);
   bn_update_xr_seg(g->nitem, g->item_lo, g->item_seg, s->x, s->p, s->r, s->q, g->scal, g->part2, st);
@@ -206,7 +205,7 @@ static int pcg_segments(struct bn_solver *s, int nsys, const int *seg_ptr, const
   hipMemcpyAsync(g->scal, hs, (size_t)BN_S_COUNT * nsys * sizeof(double), hipMemcpyHostToDevice, st);
   bn_copy('d', n, d_b, s->bb, st);
   bn_copy('d', n, d_x, s->x, st);
-  bn_csr_apply('d', BN_OP_RESID, n, A->ia, A->ja, A->vd, s->x, s->r, s->bb, A->lanes, NULL, NULL, st);
+  bn_dcsr_apply_dot(A, 'd', BN_OP_RESID, s->x, s->r, s->bb, NULL, NULL, st);
   bn_dot_seg(g->nitem, g->item_lo, s->bb, s->bb, NULL, NULL, g->part1, NULL, st);
   bn_finish_seg(nsys, 3, g->item_ptr, g->part1, NULL, NULL, g->scal, st);
   bn_vc_precond(s, s->r, s->z);

@@ -31,11 +31,7 @@ static inline const void *vecv(const double *vd, const float *vs, char vt) {
 
 /* y = op(M, x [, b]) in precision vt */
 static void apply(struct bn_solver *s, const bn_dcsr *M, int op, const void *x, void *y, const void *b) {
-  if (s->vt == 's' ? M->adapt_s : M->adapt) {
-    bn_csr_apply_blocks(s->vt, op, M->nblk, M->blk, M->ia, M->ja, opv(M, s->vt), x, y, b, s->stream);
-    return;
-  }
-  bn_dcsr_apply(M, s->vt, op, bn_lanes(M, s->vt), x, y, b, s->stream);
+  bn_op_apply(M, s->vt, op, x, y, b, s->stream);
 }
 
 /* Block smoother: nu sweeps that start from xin (NULL: zero) and end in

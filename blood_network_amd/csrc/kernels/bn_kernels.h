@@ -37,6 +37,15 @@ enum {
 /* reduction partial buffer holds at most this many blocks per dot */
 #define BN_RED_BLOCKS 2048
 
+/* Lanes-per-row SpMV variants: the lanes argument of bn_csr_apply and
+ * bn_csr_apply_seg may carry one of these bits.  Each lane then issues 8
+ * (DEEP8) or 16 (DEEP16) index and value loads before its x gathers
+ * instead of 4 (8 for 2 lanes): fewer dependent memory round trips per
+ * row. */
+#define BN_LANES_DEEP8 0x100
+#define BN_LANES_DEEP16 0x200
+#define BN_LANES_MASK 0xff
+
 /* CSR operator kinds for bn_csr_apply */
 enum {
   BN_OP_SET = 0,    /* y = A x                 */

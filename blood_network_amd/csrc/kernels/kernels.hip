@@ -65,10 +65,9 @@ __device__ __forceinline__ void clear_tail(double *partial) {
 // row then finishes in one round), 4 otherwise (one lane: the extra
 // registers and masked loads of R = 8 measured slower).  SKIP_DIAG drops
 // the entry whose column equals `self`.
-template <typename T, int LANES, bool SKIP_DIAG>
+template <typename T, int LANES, bool SKIP_DIAG, int R = (LANES == 2 ? 8 : 4)>
 __device__ __forceinline__ T row_dot(int q, const int e, const int *__restrict__ ja,
                                      const T *__restrict__ val, const T *__restrict__ x, int self) {
-  constexpr int R = LANES == 2 ? 8 : 4;
   T acc[4] = {0, 0, 0, 0};
   for (; q + (R - 1) * LANES < e; q += R * LANES) {
     int j[R];
@@ -144,7 +143,7 @@ inline int grid_for(long long nrb) {
 // ---------------------------------------------------------------------
 // CSR apply: y = A x | y = b - A x | y += A x, optional fused w.y dot
 // ---------------------------------------------------------------------
-template <typename T, int LANES, int OP, bool DOT>
+template <typename T, int LANES, int OP, bool DOT, int R = (LANES == 2 ? 8 : 4)>
 __global__ __launch_bounds__(BLOCK) void k_csr(int n, int chunk, const int *__restrict__ ia,
                                                const int *__restrict__ ja, const T *__restrict__ val,
                                                const T *__restrict__ x, T *__restrict__ y,
@@ -163,7 +162,7 @@ __global__ __launch_bounds__(BLOCK) void k_csr(int n, int chunk, const int *__re
     if (row < n) {
       const int q0 = ia[row], q1 = ia[row + 1];
       if (q1 - q0 > long_len) continue; /* handled by k_csr_long */
-      T s = row_dot<T, LANES, false>(q0 + sub, q1, ja, val, x, row);
+      T s = row_dot<T, LANES, false, R>(q0 + sub, q1, ja, val, x, row);
       if (LANES > 1) s = group_sum<LANES>(s);
       if (sub == 0) {
         T out;
@@ -182,13 +181,13 @@ __global__ __launch_bounds__(BLOCK) void k_csr(int n, int chunk, const int *__re
   }
 }
 
-template <typename T, int LANES>
+template <typename T, int LANES, int R = (LANES == 2 ? 8 : 4)>
 void launch_csr(int op, int n, const int *ia, const int *ja, const T *val, const T *x, T *y,
                 const T *b, const T *w, double *partial, int long_len, hipStream_t st) {
   constexpr int RPB = BLOCK / LANES;
   const long long nrb = ((long long)n + RPB - 1) / RPB;
 #define BN_CSR_LAUNCH(OPV, DOTV, G, CH) \
-  hipLaunchKernelGGL((k_csr<T, LANES, OPV, DOTV>), dim3(G), dim3(BLOCK), 0, st, n, CH, ia, ja, val, x, y, b, w, partial, long_len)
+  hipLaunchKernelGGL((k_csr<T, LANES, OPV, DOTV, R>), dim3(G), dim3(BLOCK), 0, st, n, CH, ia, ja, val, x, y, b, w, partial, long_len)
   if (partial) {
     // fused dot: capped grid so the partials fit BN_RED_BLOCKS slots
     const int g = grid_for(nrb);
@@ -206,9 +205,34 @@ void launch_csr(int op, int n, const int *ia, const int *ja, const T *val, const
 #undef BN_CSR_LAUNCH
 }
 
+// lanes = 1..64, optionally with a BN_LANES_DEEP* bit
+template <typename T, int LANES>
+void launch_csr_rounds(int deep, int op, int n, const int *ia, const int *ja, const T *val, const T *x, T *y,
+                       const T *b, const T *w, double *partial, int long_len, hipStream_t st) {
+  if (deep == BN_LANES_DEEP16)
+    launch_csr<T, LANES, 16>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st);
+  else if (deep == BN_LANES_DEEP8)
+    launch_csr<T, LANES, 8>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st);
+  else
+    launch_csr<T, LANES>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st);
+}
+
 template <typename T>
 void dispatch_csr(int op, int n, const int *ia, const int *ja, const T *val, const T *x, T *y,
-                  const T *b, int lanes, const T *w, double *partial, int long_len, hipStream_t st) {
+                  const T *b, int lanes_in, const T *w, double *partial, int long_len, hipStream_t st) {
+  const int deep = lanes_in & (BN_LANES_DEEP8 | BN_LANES_DEEP16);
+  const int lanes = lanes_in & BN_LANES_MASK;
+  if (deep) {
+    switch (lanes) {
+      case 1: launch_csr_rounds<T, 1>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, st); return;
+      case 2: launch_csr_rounds<T, 2>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, st); return;
+      case 4: launch_csr_rounds<T, 4>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, st); return;
+      case 8: launch_csr_rounds<T, 8>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, st); return;
+      case 16: launch_csr_rounds<T, 16>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, st); return;
+      case 32: launch_csr_rounds<T, 32>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, st); return;
+      default: launch_csr_rounds<T, 64>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, st); return;
+    }
+  }
   switch (lanes) {
     case 1: launch_csr<T, 1>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
     case 2: launch_csr<T, 2>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;

@@ -47,10 +47,11 @@ def test_csr_ops_against_oracle(oracle, vt):
         ia, ja, val = s.level(lev)["A"]
         n = len(ia) - 1
         lens = np.diff(ia)
-        for lanes in (1, 4, 16, 64):
-            long_len = 16 * lanes
+        for lanes in (1, 4, 16, 64, 1 | L.LANES_DEEP8, 1 | L.LANES_DEEP16, 2 | L.LANES_DEEP16, 4 | L.LANES_DEEP8,
+                      8 | L.LANES_DEEP16, 16 | L.LANES_DEEP8, 16 | L.LANES_DEEP16, 64 | L.LANES_DEEP16):
+            long_len = 16 * (lanes & 0xFF)
             long_rows = np.where(lens > long_len)[0].astype(np.int32)
-            case(f"csr level {lev} n={n} lanes={lanes} long={len(long_rows)} vt={vt}")
+            case(f"csr level {lev} n={n} lanes={lanes & 0xFF} rounds={lanes >> 8} long={len(long_rows)} vt={vt}")
             x = rng.random(n)
             b = rng.random(n)
             y0 = rng.random(n)
@@ -143,9 +144,10 @@ def test_segmented_long_rows_against_oracle(oracle, vt):
         ia = np.ascontiguousarray(ia, np.int32)
         ja = np.ascontiguousarray(ja, np.int32)
         n = len(ia) - 1
-        for lanes in (1, 4, 16):
-            rows, seg_ptr, seg_row, nmulti = seg_bin(ia, 16 * lanes)
-            case(f"segmented csr {name} n={n} lanes={lanes} long={len(rows)} segments={seg_ptr[-1]} multi={nmulti}")
+        for lanes in (1, 4, 16, 16 | L.LANES_DEEP16):
+            rows, seg_ptr, seg_row, nmulti = seg_bin(ia, 16 * (lanes & 0xFF))
+            case(f"segmented csr {name} n={n} lanes={lanes & 0xFF} rounds={lanes >> 8} long={len(rows)} "
+                 f"segments={seg_ptr[-1]} multi={nmulti}")
             x, b, y0 = rng.random(n), rng.random(n), rng.random(n)
             ref_set, ref_res, ref_add = np.zeros(n), np.zeros(n), y0.copy()
             oracle.lib.abyvg(ip(ia), ip(ja), dp(val), dp(x), n, dp(ref_set))
@@ -160,7 +162,8 @@ def test_segmented_long_rows_against_oracle(oracle, vt):
             for op, ref, init in ((0, ref_set, None), (1, ref_res, None), (2, ref_add, y0)):
                 dy = dev((init if init is not None else np.zeros(n)).astype(dt))
                 lib.bn_csr_apply_seg(vt.encode(), op, n, dia.ptr, dja.ptr, dv.ptr, dx.ptr, dy.ptr, db.ptr, lanes,
-                                     16 * lanes, len(rows), drows.ptr, dsp.ptr, dsr.ptr, int(seg_ptr[-1]), nmulti,
+                                     16 * (lanes & 0xFF), len(rows), drows.ptr, dsp.ptr, dsr.ptr, int(seg_ptr[-1]),
+                                     nmulti,
                                      dpart.ptr, None)
                 got = dy.numpy().astype(np.float64)
                 assert relerr(got, ref) < tol, (name, lanes, op)

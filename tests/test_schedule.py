@@ -168,3 +168,4 @@ def test_chunk_order_colours_each_chunk_properly(nchunks):
     same_chunk = chunk_of[A.row] == chunk_of[A.col]
     sel = off & same_chunk
     assert not np.any(colour[A.row[sel]] == colour[A.col[sel]])
+
