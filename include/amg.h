@@ -14,6 +14,15 @@
 #ifndef BN_AMG_H
 #define BN_AMG_H
 
+/* the reference header brings these in for its callers
+ * (C_amg1/include/agg_lib.h:20-24, amg.h:16-17) */
+#include <ctype.h>
+#include <math.h>
+#include <stdio.h>
+#include <stdlib.h>
+#include <string.h>
+#include <time.h>
+
 #ifdef __cplusplus
 extern "C" {
 #endif
@@ -34,6 +43,153 @@ typedef struct {
   INT *ja;
   REAL *val;
 } dcsr_mat;
+
+/* 0-based COO (C_amg1/include/agg_lib.h:47-54) */
+typedef struct {
+  INT row;
+  INT col;
+  INT nnz;
+  INT *icols;
+  INT *irows;
+  REAL *val;
+} dcoo_mat;
+
+/* ---------------------------------------------------------------------
+ * Header-only CSR/COO helpers of C_amg1/include/agg_lib.h:57-242, same
+ * semantics (callers such as test drivers use them unchanged).
+ * ------------------------------------------------------------------- */
+static inline void dcsr_mat_free(dcsr_mat *a) {
+  if (!a) return;
+  free(a->ia);
+  free(a->ja);
+  free(a->val);
+  a->ia = a->ja = NULL;
+  a->val = NULL;
+  a->row = a->col = a->nnz = 0;
+}
+
+static inline void dcoo_mat_free(dcoo_mat *a) {
+  if (!a) return;
+  free(a->irows);
+  free(a->icols);
+  free(a->val);
+  a->irows = a->icols = NULL;
+  a->val = NULL;
+  a->row = a->col = a->nnz = 0;
+}
+
+/* zero-filled arrays for an nrows x ncols matrix with nnz entries */
+static inline void dcsr_mat_alloc(dcsr_mat *a, INT nrows, INT ncols, INT nnz) {
+  a->row = nrows;
+  a->col = ncols;
+  a->nnz = nnz;
+  a->ia = (INT *)calloc((size_t)nrows + 1, sizeof(INT));
+  a->ja = (INT *)calloc((size_t)nnz, sizeof(INT));
+  a->val = (REAL *)calloc((size_t)nnz, sizeof(REAL));
+}
+
+/* at = transpose of A with its rows taken in the order p (row i of the
+ * permuted matrix is row p[i] of A; p NULL: natural order).  Rows of at
+ * list their columns ascending.  at's arrays are pre-allocated; with
+ * A->val or at->val NULL only the structure is formed. */
+static inline void dcsr_transz(const dcsr_mat *a_in, INT *p, dcsr_mat *at_out) {
+  const INT n = a_in->row, m = a_in->col;
+  at_out->row = m;
+  at_out->col = n;
+  at_out->nnz = a_in->nnz;
+  INT *ptr = at_out->ia;
+  for (INT j = 0; j <= m; j++) ptr[j] = 0;
+  for (INT k = 0; k < a_in->nnz; k++) ptr[a_in->ja[k] + 1]++;
+  for (INT j = 0; j < m; j++) ptr[j + 1] += ptr[j];
+  INT *fill = (INT *)calloc((size_t)(m > 0 ? m : 1), sizeof(INT));
+  const int vals = a_in->val != NULL && at_out->val != NULL;
+  for (INT i = 0; i < n; i++) {
+    const INT src = p ? p[i] : i;
+    for (INT k = a_in->ia[src]; k < a_in->ia[src + 1]; k++) {
+      const INT j = a_in->ja[k];
+      const INT at = ptr[j] + fill[j]++;
+      at_out->ja[at] = i;
+      if (vals) at_out->val[at] = a_in->val[k];
+    }
+  }
+  free(fill);
+}
+
+/* c = a + b, allocated here.  Row i lists a's entries in a's order, then
+ * b's columns not yet in the row; a b entry in a column already in the
+ * row adds to it.  Returns -1 when the shapes differ, else 0. */
+static inline INT dcsr_mat_add(dcsr_mat *a, dcsr_mat *b, dcsr_mat *c) {
+  if (a->row != b->row || a->col != b->col) return -1;
+  const INT n = a->row, m = a->col;
+  INT *where = (INT *)malloc((size_t)(m > 0 ? m : 1) * sizeof(INT));
+  REAL *acc = (REAL *)calloc((size_t)(m > 0 ? m : 1), sizeof(REAL));
+  for (INT j = 0; j < m; j++) where[j] = -1;
+  INT cnt = 0;
+  for (INT i = 0; i < n; i++) {
+    for (INT q = a->ia[i]; q < a->ia[i + 1]; q++) where[a->ja[q]] = i, cnt++;
+    for (INT q = b->ia[i]; q < b->ia[i + 1]; q++)
+      if (where[b->ja[q]] != i) where[b->ja[q]] = i, cnt++;
+  }
+  dcsr_mat_alloc(c, n, m, cnt);
+  for (INT j = 0; j < m; j++) where[j] = -1;
+  INT pos = 0;
+  for (INT i = 0; i < n; i++) {
+    const INT row0 = pos;
+    c->ia[i] = pos;
+    for (INT q = a->ia[i]; q < a->ia[i + 1]; q++) {
+      const INT j = a->ja[q];
+      where[j] = pos;
+      c->ja[pos++] = j;
+      acc[j] = a->val[q];
+    }
+    for (INT q = b->ia[i]; q < b->ia[i + 1]; q++) {
+      const INT j = b->ja[q];
+      if (where[j] >= row0) {
+        acc[j] += b->val[q];
+      } else {
+        where[j] = pos;
+        c->ja[pos++] = j;
+        acc[j] = b->val[q];
+      }
+    }
+    for (INT q = row0; q < pos; q++) {
+      c->val[q] = acc[c->ja[q]];
+      acc[c->ja[q]] = 0.0;
+    }
+  }
+  c->ia[n] = pos;
+  c->nnz = pos;
+  free(where);
+  free(acc);
+  return 0;
+}
+
+/* ---------------------------------------------------------------------
+ * Host-array helpers (C_amg1/include/agg_lib.h:251-310).  They work on
+ * host memory on the CPU, as in the reference; each row's sum runs in the
+ * reference's order.  The solve entry points below do not use them.
+ * ------------------------------------------------------------------- */
+void nullv(REAL *u, INT n);                                 /* agg_lib.h:251 */
+void copyv(const REAL *u, REAL *v, INT n);                  /* agg_lib.h:254 */
+void scpro(const REAL *u, const REAL *v, REAL *scpr, INT n); /* :257 */
+void l2norm(const REAL *u, REAL *l2nr, INT n);              /* agg_lib.h:260 */
+void uuplmv(REAL *u, const REAL *v, INT n, REAL smult);     /* agg_lib.h:263 */
+void umuplv(REAL *u, const REAL *v, INT n, REAL smult);     /* agg_lib.h:266 */
+void randv(REAL *u, INT n);                                 /* agg_lib.h:269 */
+void dcsr_cp(const dcsr_mat *src, dcsr_mat *dst);           /* agg_lib.h:276 */
+void abyvg(const INT *ia, const INT *ja, const REAL *a, const REAL *x, INT n,
+           REAL *y);                                        /* agg_lib.h:284 */
+void abyvcs(const INT *ia, const INT *ja, const REAL *a, const REAL *x, INT n,
+            REAL *y);                                       /* agg_lib.h:288 */
+void abyvam(const REAL *x, const INT *ia, const INT *ja, const REAL *a,
+            const REAL *y, INT n, REAL *z);                 /* agg_lib.h:292 */
+void vbya(const INT *ia, const INT *ja, const REAL *a, const REAL *x, INT n,
+          INT m_dim, REAL *y);                              /* agg_lib.h:296 */
+void dcoo2dcsr(dcoo_mat *acoo, dcsr_mat *a);                /* agg_lib.h:307 */
+/* agg_lib.h:310: "nrows ncols nnz" then "row col value" lines, 1-based
+ * indices detected; returns 0, or 1 with a message (entries outside the
+ * stated shape are also rejected) */
+INT dcsr_mat_dcoo(FILE *fp, dcsr_mat *a_out);
 
 #define AMG_MAX_LEVELS 20 /* C_amg1/include/amg.h:19 */
 

@@ -1,0 +1,201 @@
+"""The host-array helpers of the library ABI (C_amg1/include/agg_lib.h:
+COO reader, BLAS-1 and CSR products on host arrays, header-only CSR
+helpers) against the compiled reference (oracle/_ref), bit for bit, and
+the reference's own driver (C_amg1/test_amg.c) building against
+include/amg.h and libbnamg.so."""
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+from blood_network_amd import _lib as L
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+REF = os.path.join(ROOT, "oracle", "_ref")
+REF_LIB = os.path.join(REF, "libamgref.so")
+
+pytestmark = pytest.mark.skipif(not os.path.exists(REF_LIB), reason="compiled reference (oracle/_ref) not built")
+
+libc = C.CDLL(None)
+libc.fopen.restype = C.c_void_p
+libc.fopen.argtypes = [C.c_char_p, C.c_char_p]
+libc.fclose.argtypes = [C.c_void_p]
+libc.free.argtypes = [C.c_void_p]
+libc.srand.argtypes = [C.c_uint]
+
+
+def ref_lib():
+    lib = C.CDLL(REF_LIB)
+    return lib
+
+
+def read_coo(lib, path):
+    """dcsr_mat_dcoo of `lib` on `path`: (rc, ia, ja, val) (arrays None on error)."""
+    lib.dcsr_mat_dcoo.restype = C.c_int
+    lib.dcsr_mat_dcoo.argtypes = [C.c_void_p, C.POINTER(L.CSR)]
+    fp = libc.fopen(path.encode(), b"r")
+    assert fp
+    m = L.CSR()
+    rc = lib.dcsr_mat_dcoo(fp, C.byref(m))
+    libc.fclose(fp)
+    if rc:
+        return rc, None, None, None
+    n, nnz = m.row, m.nnz
+    ia = np.ctypeslib.as_array(m.ia, shape=(n + 1,)).copy()
+    ja = np.ctypeslib.as_array(m.ja, shape=(max(nnz, 1),)).copy()[:nnz]
+    val = np.ctypeslib.as_array(m.val, shape=(max(nnz, 1),)).copy()[:nnz]
+    for p in (m.ia, m.ja, m.val):
+        libc.free(C.cast(p, C.c_void_p))
+    return rc, (m.row, m.col), ia, ja, val
+
+
+def write_coo(path, nr, nc, rows, cols, vals, base):
+    with open(path, "w") as f:
+        f.write(f"{nr} {nc} {len(rows)}\n")
+        for i, j, v in zip(rows, cols, vals):
+            f.write(f"{i + base} {j + base} {v:.17g}\n")
+
+
+@pytest.mark.parametrize("base", [0, 1])
+def test_coo_reader_matches_reference(tmp_path, base):
+    # unsorted entries, duplicates and an empty row; both index bases
+    rng = np.random.default_rng(base)
+    nr, nc, nnz = 300, 280, 2500
+    rows = rng.integers(0, nr, nnz)
+    cols = rng.integers(0, nc, nnz)
+    rows[rows == 17] = 18  # row 17 empty
+    rows[0], cols[0] = 0, 0  # the smallest index decides the base
+    vals = rng.normal(size=nnz)
+    path = str(tmp_path / "a.txt")
+    write_coo(path, nr, nc, rows, cols, vals, base)
+    ref = read_coo(ref_lib(), path)
+    got = read_coo(L.lib, path)
+    assert ref[0] == 0 and got[0] == 0
+    assert ref[1] == got[1] == (nr, nc)
+    for a, b in zip(ref[2:], got[2:]):
+        assert np.array_equal(a, b)
+    ia, ja, val = got[2:]
+    assert ia[18] == ia[17]  # the empty row
+    # each row keeps its entries in file order
+    r0 = np.where(rows == 5)[0]
+    assert np.array_equal(ja[ia[5]:ia[6]], cols[r0]) and np.array_equal(val[ia[5]:ia[6]], vals[r0])
+
+
+def test_coo_reader_errors(tmp_path):
+    bad_header = tmp_path / "h.txt"
+    bad_header.write_text("3 3\n")
+    short = tmp_path / "s.txt"
+    short.write_text("3 3 4\n1 1 2.0\n2 2 3.0\n")
+    for p in (bad_header, short):
+        assert read_coo(ref_lib(), str(p))[0] == 1
+        assert read_coo(L.lib, str(p))[0] == 1
+    # an entry outside the stated shape: rejected (the reference would
+    # write past its arrays)
+    out = tmp_path / "o.txt"
+    out.write_text("2 2 2\n1 1 1.0\n3 1 1.0\n")
+    assert read_coo(L.lib, str(out))[0] == 1
+
+
+def test_vector_and_csr_helpers_match_reference():
+    ref = ref_lib()
+    rng = np.random.default_rng(7)
+    n, m = 4000, 3500
+    lens = rng.integers(0, 12, n)
+    ia = np.zeros(n + 1, np.int32)
+    ia[1:] = np.cumsum(lens)
+    ja = rng.integers(0, m, ia[-1]).astype(np.int32)
+    a = rng.normal(size=ia[-1])
+    x = rng.normal(size=m)
+    xn = rng.normal(size=n)
+    y0 = rng.normal(size=n)
+    ip = lambda v: v.ctypes.data_as(L.IP)  # noqa: E731
+    dp = lambda v: v.ctypes.data_as(L.DP)  # noqa: E731
+    for lib in (ref, L.lib):
+        lib.abyvg.argtypes = [L.IP, L.IP, L.DP, L.DP, C.c_int, L.DP]
+        lib.abyvcs.argtypes = [L.IP, L.IP, L.DP, L.DP, C.c_int, L.DP]
+        lib.abyvam.argtypes = [L.DP, L.IP, L.IP, L.DP, L.DP, C.c_int, L.DP]
+        lib.vbya.argtypes = [L.IP, L.IP, L.DP, L.DP, C.c_int, C.c_int, L.DP]
+        lib.scpro.argtypes = [L.DP, L.DP, L.DP, C.c_int]
+        lib.l2norm.argtypes = [L.DP, L.DP, C.c_int]
+        lib.uuplmv.argtypes = [L.DP, L.DP, C.c_int, C.c_double]
+        lib.umuplv.argtypes = [L.DP, L.DP, C.c_int, C.c_double]
+        lib.randv.argtypes = [L.DP, C.c_int]
+        lib.nullv.argtypes = [L.DP, C.c_int]
+        lib.copyv.argtypes = [L.DP, L.DP, C.c_int]
+
+    def run(lib):
+        out = {}
+        y = np.zeros(n)
+        lib.abyvg(ip(ia), ip(ja), dp(a), dp(x), n, dp(y))
+        out["abyvg"] = y
+        y = y0.copy()
+        lib.abyvcs(ip(ia), ip(ja), dp(a), dp(x), n, dp(y))
+        out["abyvcs"] = y
+        z = np.zeros(n)
+        lib.abyvam(dp(y0), ip(ia), ip(ja), dp(a), dp(x), n, dp(z))
+        out["abyvam"] = z
+        t = np.full(m, np.nan)
+        lib.vbya(ip(ia), ip(ja), dp(a), dp(xn), n, m, dp(t))
+        out["vbya"] = t
+        s = np.zeros(1)
+        lib.scpro(dp(xn), dp(y0), dp(s), n)
+        out["scpro"] = s
+        s = np.zeros(1)
+        lib.l2norm(dp(xn), dp(s), n)
+        out["l2norm"] = s
+        u = y0.copy()
+        lib.uuplmv(dp(u), dp(xn), n, 0.37)
+        out["uuplmv"] = u
+        u = y0.copy()
+        lib.umuplv(dp(u), dp(xn), n, -1.9)
+        out["umuplv"] = u
+        libc.srand(12)
+        r = np.zeros(n)
+        lib.randv(dp(r), n)
+        out["randv"] = r
+        c = np.zeros(n)
+        lib.copyv(dp(y0), dp(c), n)
+        out["copyv"] = c
+        lib.nullv(dp(c), n)
+        out["nullv"] = c
+        return out
+
+    want, got = run(ref), run(L.lib)
+    for k in want:
+        assert np.array_equal(want[k], got[k]), k
+
+
+def test_header_csr_helpers_match_reference(tmp_path):
+    # dcsr_transz (plain and row-permuted) and dcsr_mat_add as defined in
+    # each header, on a square matrix with duplicates and empty rows
+    probes = [os.path.join(REF, p) for p in ("inline_probe_ref", "inline_probe_bn")]
+    if not all(os.path.exists(p) for p in probes):
+        pytest.skip("inline probes not built")
+    rng = np.random.default_rng(3)
+    n, nnz = 400, 3000
+    rows = rng.integers(0, n, nnz)
+    cols = rng.integers(0, n, nnz)
+    rows[rows == 5] = 6
+    path = str(tmp_path / "a.txt")
+    write_coo(path, n, n, rows, cols, rng.normal(size=nnz), 1)
+    perm = str(tmp_path / "p.txt")
+    np.savetxt(perm, rng.permutation(n), fmt="%d")
+    outs = [subprocess.run([p, path, perm], capture_output=True, text=True, timeout=60) for p in probes]
+    for o in outs:
+        assert o.returncode == 0, o.stderr
+    assert outs[0].stdout == outs[1].stdout
+    assert "permuted_transpose" in outs[1].stdout and "sum" in outs[1].stdout
+
+
+def test_reference_driver_links_against_libbnamg():
+    # C_amg1/test_amg.c, unchanged, compiled against include/amg.h and linked
+    # to libbnamg.so; without arguments it prints its usage and exits 1
+    exe = os.path.join(REF, "test_amg_bn")
+    if not os.path.exists(exe):
+        pytest.skip("drop-in driver not built")
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 1 and "Usage" in r.stderr
+    ldd = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
+    assert "libbnamg.so" in ldd
