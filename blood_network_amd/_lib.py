@@ -60,6 +60,19 @@ class Options(C.Structure):
                 ("block_smoother", C.c_int), ("chunk_rows", C.c_int)]
 
 
+class Nwk(C.Structure):
+    """bn_nwk (include/bn_netgen.h): a vessel-network case."""
+    _fields_ = [("np", C.c_int), ("nf", C.c_int), ("nbc", C.c_int), ("pts", C.POINTER(C.c_double)),
+                ("face", C.POINTER(C.c_int)), ("dia", C.POINTER(C.c_double)), ("bc_node", C.POINTER(C.c_int)),
+                ("bc_val", C.POINTER(C.c_double))]
+
+
+class NwkSys(C.Structure):
+    """bn_nwk_sys (include/bn_netgen.h): its interior pressure system."""
+    _fields_ = [("n", C.c_int), ("nnz", C.c_int), ("ia", C.POINTER(C.c_int)), ("ja", C.POINTER(C.c_int)),
+                ("val", C.POINTER(C.c_double)), ("b", C.POINTER(C.c_double)), ("interior", C.POINTER(C.c_int))]
+
+
 class SolveInfo(C.Structure):
     """bn_solve_info (include/bn_amg_ext.h)."""
     _fields_ = [("iterations", C.c_int), ("rel_bnorm", C.c_double),
@@ -152,6 +165,12 @@ def _load():
         "bn_netgen_assemble": (None, [C.POINTER(Network), DP, DP, DP, DP]),
         "bn_netgen_conductance": (None, [C.POINTER(Network), DP, C.c_double, DP]),
         "bn_netgen_free": (None, [C.POINTER(Network)]),
+        "bn_nwk_read": (C.c_int, [C.c_char_p, C.POINTER(Nwk)]),
+        "bn_nwk_free": (None, [C.POINTER(Nwk)]),
+        "bn_nwk_system": (C.c_int, [C.POINTER(Nwk), C.POINTER(NwkSys)]),
+        "bn_nwk_sys_free": (None, [C.POINTER(NwkSys)]),
+        "bn_write_pressures": (C.c_int, [C.c_char_p, DP, C.c_int]),
+        "bn_write_labelled": (C.c_int, [C.c_char_p, IP, DP, C.c_int]),
         "amg_build_hierarchy": (None, [VP, C.POINTER(CSR)]),
         "amg_free": (None, [VP]),
         "ichol_compute": (None, [C.POINTER(CSR), VP]),

@@ -12,7 +12,8 @@ import numpy as np
 
 from . import _lib as L
 
-__all__ = ["VesselNetwork", "make_network", "WORKLOADS", "union_networks", "uneven_batch"]
+__all__ = ["VesselNetwork", "make_network", "WORKLOADS", "union_networks", "uneven_batch", "NetworkCase",
+           "read_network_case", "write_pressures", "write_labelled"]
 
 # name -> (lattice edge, tree depth, trees per kind, coarsening)
 WORKLOADS = {
@@ -149,3 +150,59 @@ def uneven_batch(count=64, seed=0, edge=(12, 40), depth=(6, 10)):
             d -= 1
         nets.append(make_network(int(Le), int(d), 1, seed=seed + g))
     return nets
+
+
+class NetworkCase:
+    """A vessel-network case read from <base>.fMx/.pMx/.dia/.BC and its
+    interior pressure system (libbnamg bn_nwk_read / bn_nwk_system, after
+    Matlab_amg/caseReaderMJ.m and read_nwk.m).  Indices are 0-based except
+    ``interior``, which keeps the files' 1-based node numbers."""
+
+    def __init__(self, base):
+        w = L.Nwk()
+        if L.lib.bn_nwk_read(str(base).encode(), C.byref(w)):
+            raise ValueError(L.last_error())
+        try:
+            self.n_points, self.n_faces, self.n_boundary = w.np, w.nf, w.nbc
+            self.points = np.ctypeslib.as_array(w.pts, shape=(w.np, 3)).copy()
+            self.faces = np.ctypeslib.as_array(w.face, shape=(w.nf, 2)).copy()
+            self.diameters = (np.ctypeslib.as_array(w.dia, shape=(w.nf,)).copy() if bool(w.dia) else None)
+            nb = max(w.nbc, 1)
+            self.bc_nodes = np.ctypeslib.as_array(w.bc_node, shape=(nb,)).copy()[:w.nbc]
+            self.bc_pressures = np.ctypeslib.as_array(w.bc_val, shape=(nb,)).copy()[:w.nbc]
+            sysm = L.NwkSys()
+            if L.lib.bn_nwk_system(C.byref(w), C.byref(sysm)):
+                raise ValueError(L.last_error())
+            try:
+                n, nnz = sysm.n, sysm.nnz
+                self.n = n
+                self.ia = np.ctypeslib.as_array(sysm.ia, shape=(n + 1,)).copy()
+                self.ja = np.ctypeslib.as_array(sysm.ja, shape=(max(nnz, 1),)).copy()[:nnz]
+                self.val = np.ctypeslib.as_array(sysm.val, shape=(max(nnz, 1),)).copy()[:nnz]
+                self.b = np.ctypeslib.as_array(sysm.b, shape=(max(n, 1),)).copy()[:n]
+                self.interior = np.ctypeslib.as_array(sysm.interior, shape=(max(n, 1),)).copy()[:n]
+            finally:
+                L.lib.bn_nwk_sys_free(C.byref(sysm))
+        finally:
+            L.lib.bn_nwk_free(C.byref(w))
+
+
+def read_network_case(base):
+    return NetworkCase(base)
+
+
+def write_pressures(path, p):
+    """One %.16e per line (Matlab_amg/test_amg.m:68-72)."""
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    if L.lib.bn_write_pressures(str(path).encode(), L.dp(p), len(p)):
+        raise OSError(L.last_error())
+
+
+def write_labelled(path, nodes, p):
+    """"node pressure" lines, 1-based nodes (Matlab_amg/label_solution.m:42-47)."""
+    nodes = np.ascontiguousarray(nodes, dtype=np.int32)
+    p = np.ascontiguousarray(p, dtype=np.float64)
+    if len(nodes) != len(p):
+        raise ValueError("one node per pressure")
+    if L.lib.bn_write_labelled(str(path).encode(), L.ip(nodes), L.dp(p), len(p)):
+        raise OSError(L.last_error())

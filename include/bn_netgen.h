@@ -68,6 +68,43 @@ void bn_netgen_conductance(const bn_network *net, const double *radius,
                            double mu, double *g);
 void bn_netgen_free(bn_network *net);
 
+/* ---- vessel-network case files (Matlab_amg/caseReaderMJ.m:1-33) ----
+ * <base>.fMx (faces: columns 2, 3 = 1-based end points), <base>.pMx (x y z
+ * per point), <base>.dia (face diameters, may be empty), <base>.BC (node,
+ * type, pressure), numeric text as MATLAB load() reads it.  Indices here
+ * are 0-based. */
+typedef struct {
+  int np, nf, nbc;
+  double *pts;     /* np x 3 */
+  int *face;       /* nf x 2 end points */
+  double *dia;     /* nf, or NULL when the file is empty */
+  int *bc_node;    /* nbc boundary nodes */
+  double *bc_val;  /* nbc pressures */
+} bn_nwk;
+
+/* the interior pressure system of Matlab_amg/read_nwk.m:15-37 */
+typedef struct {
+  int n, nnz;
+  int *ia, *ja;    /* CSR, columns ascending per row */
+  double *val;
+  double *b;       /* n right-hand side */
+  int *interior;   /* n: 1-based node of each unknown, ascending */
+} bn_nwk_sys;
+
+/* Returns 0, or -1 (bn_last_error() says why). */
+int bn_nwk_read(const char *base, bn_nwk *w);
+void bn_nwk_free(bn_nwk *w);
+/* A = C_I' W C_I, b = -C_I' W C_B p_B with diffusive Cartesian face
+ * conductances (makeDiffusiveConductance.m); duplicate entries summed,
+ * exact zeros dropped.  Rejects a boundary node listed twice and faces
+ * with no positive axis extent (MATLAB would give NaN).  Returns 0 or -1. */
+int bn_nwk_system(const bn_nwk *w, bn_nwk_sys *s);
+void bn_nwk_sys_free(bn_nwk_sys *s);
+/* solution files: one %.16e per line (Matlab_amg/test_amg.m:68-72), and
+ * "%d %.16e" with 1-based node indices (label_solution.m:42-47) */
+int bn_write_pressures(const char *path, const double *p, int n);
+int bn_write_labelled(const char *path, const int *node, const double *p, int n);
+
 #ifdef __cplusplus
 }
 #endif

@@ -70,3 +70,33 @@ def test_reference_driver_on_libbnamg(tmp_path, two_matrix):
     print(f"iterations reference={it_r} libbnamg={it_b}; relres {rr_r:.2e} / {rr_b:.2e}")
     assert it_b > 0 and rr_b < 1e-5
     assert abs(it_b - it_r) <= 3
+
+
+def test_network_case_files_to_solution_files(tmp_path):
+    """A Cartesian case written as <base>.fMx/.pMx/.dia/.BC, read and
+    assembled by libbnamg (read_nwk.m semantics), solved on the GPU, and
+    written as p_interior / labelled files (test_amg.m:68-72,
+    label_solution.m:42-47); the pressures match a direct solve."""
+    import scipy.sparse as sp
+    import scipy.sparse.linalg as spla
+
+    from blood_network_amd import AMGSolver, read_network_case, write_labelled, write_pressures
+    from test_nwk_io import cartesian_case, write_case
+
+    faces, pts, bc = cartesian_case(40, 30, 20, h=(2e-5, 2e-5, 4e-5), seed=2)
+    base = str(tmp_path / "nwk_Cartesian_40x30x20")
+    write_case(base, faces, pts, bc, dia=np.full(len(faces), 5e-6))
+    c = read_network_case(base)
+    case(f"network case files: {c.n_points} points, {c.n_faces} faces, n={c.n}")
+    s = AMGSolver(c.ia, c.ja, c.val, 0.25, 128, 20, 2)
+    p, it = s.solve(c.b, tol=1e-10, maxiter=500)
+    A = sp.csr_matrix((c.val, c.ja, c.ia), shape=(c.n, c.n))
+    ref = spla.spsolve(A.tocsc(), c.b)
+    err = np.linalg.norm(p - ref) / np.linalg.norm(ref)
+    print(f"iterations={it} rel err vs direct solve {err:.2e}")
+    assert it > 0 and err < 1e-8
+    write_pressures(tmp_path / "p_interior_40x30x20.txt", p)
+    write_labelled(tmp_path / "p_interior_40x30x20_labeled.txt", c.interior, p)
+    assert np.array_equal(np.loadtxt(tmp_path / "p_interior_40x30x20.txt"), p)
+    lab = np.loadtxt(tmp_path / "p_interior_40x30x20_labeled.txt")
+    assert np.array_equal(lab[:, 0].astype(int), c.interior) and np.array_equal(lab[:, 1], p)
