@@ -85,7 +85,7 @@ def descent_bench(args, n_obs=10_000, steps=20):
     edge scatter, zero initial guesses, radii fixed) in FP64 and with the
     FP32 V-cycle in flexible FP64 PCG, each against an FP64 gradient solved
     to 1e-13; then 20 descent steps with the hierarchy rebuilt per radius
-    set and with it kept (device value swap).  The CPU leg: the compiled
+    set on the host, rebuilt on the device, and kept (device value swap).  The CPU leg: the compiled
     reference's forward and adjoint PCG solves on one host core."""
     import numpy as np
 
@@ -128,7 +128,7 @@ def descent_bench(args, n_obs=10_000, steps=20):
         log(f"[descent] gradient {name}: {grad[name]}")
         fit.free()
     descents = {}
-    for refresh in ("rebuild", "values"):
+    for refresh in ("rebuild", "device", "values"):
         fit = PressureFit(net, obs, p_obs, tol=args.rtol, refresh=refresh, **modes["fp32_flex"])
         t0 = time.perf_counter()
         _, losses = fit.descend(logr0, steps=steps)

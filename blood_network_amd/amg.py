@@ -202,6 +202,12 @@ class AMGSolver:
         creation order); the preconditioner is kept."""
         self._check(L.lib.bn_solver_set_values_device(self.h, nnz, val_ptr))
 
+    def rebuild_values_device(self, nnz, val_ptr):
+        """New solve-matrix values (device pointer, creation entry order) and
+        a value-only rebuild of the whole hierarchy on the device
+        (amg_solver_update's result, computed on the GPU)."""
+        self._check(L.lib.bn_solver_rebuild_values_device(self.h, nnz, val_ptr))
+
     def solve_batch(self, B, tol=1e-8, maxiter=1000):
         """B: (k, n) right-hand sides; returns X (k, n) and per-column counts."""
         B = np.ascontiguousarray(B, dtype=np.float64)

@@ -84,6 +84,7 @@ void bn_drop_graphs(struct bn_solver *s) {
 
 void bn_device_release(struct bn_solver *s) {
   bn_drop_graphs(s);
+  bn_rebuild_release(s);
   for (int k = 0; k < s->ndptr; k++) hipFree(s->dptr[k]);
   free(s->dptr);
   s->dptr = NULL;

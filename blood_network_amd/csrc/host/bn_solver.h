@@ -156,6 +156,7 @@ struct bn_solver {
   int nhptr, caphptr;
   bn_block blk;         /* block solve work space */
   void *segs;           /* cached segmentations of segmented solves (segments.c) */
+  void *rb;             /* index maps of the device value-only rebuild (rebuild.c) */
 };
 
 /* lanes per row for precision vt */
@@ -216,6 +217,10 @@ static inline void bn_op_apply(const bn_dcsr *m, char vt, int op, const void *x,
 /* segments.c */
 void bn_segments_release(struct bn_solver *s);
 void bn_drop_graphs(struct bn_solver *s);
+/* extras.c: caller value order -> stored solve-matrix slots (vperm_state) */
+int bn_value_map(struct bn_solver *s);
+/* rebuild.c: frees the device rebuild plan's host side */
+void bn_rebuild_release(struct bn_solver *s);
 
 /* device.c */
 int bn_device_upload(struct bn_solver *s);

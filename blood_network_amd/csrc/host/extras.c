@@ -83,7 +83,7 @@ int bn_solver_adjoint_grad_device(void *h, const double *d_p, const double *d_dL
  * the caller's pattern relabelled by the level-0 row order, so relabelling
  * a matrix whose values are the slot numbers (exact in FP64) gives the
  * map.  The same order needs no gather. */
-static int value_map(struct bn_solver *s) {
+int bn_value_map(struct bn_solver *s) {
   const dcsr_mat *A = s->A;
   const int n = A->row, nnz = A->nnz;
   int *inv = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
@@ -139,7 +139,7 @@ int bn_solver_set_values_device(void *h, int nnz_in, const double *d_val) {
   int rc = bn_ensure_device(s);
   if (rc) return rc;
   if (s->Asolve.nnz != s->A->nnz) return BN_ERR_ARG;
-  if (!s->vperm_state && (rc = value_map(s))) return rc;
+  if (!s->vperm_state && (rc = bn_value_map(s))) return rc;
   const size_t nnz = (size_t)s->A->nnz;
   if (s->asolve_shared) {
     /* the level-0 V-cycle operator keeps its values: the solve matrix

@@ -272,6 +272,20 @@ void bn_finish_seg(int nsys, int which, const int *item_ptr, const double *p1, c
  * n x K (columns k..K-1 zero); back = 1 copies the first k columns out */
 void bn_interleave(int n, int k, int K, const double *src, double *dst, int back, hipStream_t stream);
 
+/* ---- value-only rebuild of a hierarchy (rebuild.hip) ---- */
+void bn_rb_symmetrize(int nnz, const int *tpos, const double *v, double *out, hipStream_t st);
+void bn_rb_gather(int n, const int *map, const double *src, double *out, float *outs, hipStream_t st);
+void bn_rb_to_float(int n, const double *src, float *out, hipStream_t st);
+void bn_rb_row_scalars(int n, const int *ia, const int *ja, const double *a, double *dinv, float *dinvs,
+                       double *l1inv, float *l1invs, double *l1, hipStream_t st);
+void bn_rb_dinv_map(int n, const int *ia, const int *ja, const double *a, const int *map, double *dinv, float *dinvs,
+                    hipStream_t st);
+void bn_rb_interp(int n, const int *aia, const int *aja, const double *a, const int *sia, const int *scol,
+                  const int *sslot, const int *cpt, const int *pia, double *pv, hipStream_t st);
+void bn_rb_rap(int nc, const int *ria, const int *rja, const double *rv, const int *aia, const int *aja,
+               const double *av, const int *pia, const int *pja, const double *pv, const int *cia, const int *ccol,
+               const int *cpos, double *out, hipStream_t st);
+
 #ifdef __cplusplus
 }
 #endif

@@ -10,10 +10,12 @@ the same hierarchy (A is symmetric) and the edge-scatter kernel
 libbnamg on device-resident arrays.  The host assembles matrix values from
 conductances (bn_netgen_assemble) and moves the log-radii.
 
-Between radius sets the preconditioner is either rebuilt (``refresh=
-"rebuild"``: amg_solver_update, a host value-only rebuild of the hierarchy
-and the fine IC(0) on the stored splitting and interpolation pattern,
-C_amg1/src/build_hierarchy.c:170-242) or kept from the first radius set
+Between radius sets the preconditioner is rebuilt on the host
+(``refresh="rebuild"``: amg_solver_update, a value-only rebuild of the
+hierarchy and the fine IC(0) on the stored splitting and interpolation
+pattern, C_amg1/src/build_hierarchy.c:170-242, then an upload), rebuilt
+on the device with bit-identical results (``refresh="device"``:
+bn_solver_rebuild_values_device), or kept from the first radius set
 (``refresh="values"``: only the solve matrix's values change on the device,
 bn_solver_set_values_device).
 """
@@ -32,8 +34,8 @@ class PressureFit:
     observations.  ``solver_kw`` are AMGSolver keyword arguments."""
 
     def __init__(self, net, obs, p_obs, *, refresh="rebuild", tol=1e-10, maxiter=1000, **solver_kw):
-        if refresh not in ("rebuild", "values"):
-            raise ValueError("refresh is 'rebuild' or 'values'")
+        if refresh not in ("rebuild", "device", "values"):
+            raise ValueError("refresh is 'rebuild', 'device' or 'values'")
         self.net = net
         self.obs = np.ascontiguousarray(obs, dtype=np.int64)
         self.p_obs = np.ascontiguousarray(p_obs, dtype=np.float64)
@@ -51,7 +53,7 @@ class PressureFit:
         self.d_grad = L.DeviceArray(np.zeros(m))
         self.d_b, self.d_p = L.DeviceArray(np.zeros(n)), L.DeviceArray(np.zeros(n))
         self.d_lam, self.d_dLdp = L.DeviceArray(np.zeros(n)), L.DeviceArray(np.zeros(n))
-        self.d_val = L.DeviceArray(np.zeros(net.nnz)) if refresh == "values" else None
+        self.d_val = L.DeviceArray(np.zeros(net.nnz)) if refresh != "rebuild" else None
         self.p = None
         self.log = []  # one dict per evaluation: loss, iterations, times
 
@@ -65,6 +67,9 @@ class PressureFit:
             self.solver = AMGSolver(self.net.ia, self.net.ja, val, **self.solver_kw)
         elif self.refresh == "rebuild":
             self.solver.update(self.net.ia, self.net.ja, val)
+        elif self.refresh == "device":
+            self.d_val.upload(val)
+            self.solver.rebuild_values_device(len(val), self.d_val.ptr)
         else:
             self.d_val.upload(val)
             self.solver.set_values_device(len(val), self.d_val.ptr)

@@ -167,6 +167,17 @@ int bn_solver_adjoint_grad_device(void *h, const double *d_p, const double *d_dL
  * hierarchy rebuilds; amg_solver_update rebuilds everything. */
 int bn_solver_set_values_device(void *h, int nnz, const double *d_val);
 
+/* ---- value-only rebuild of the hierarchy on the device: new solve-matrix
+ * values as bn_solver_set_values_device takes them, then the interpolation
+ * weights and Galerkin operators of every level recomputed on the stored
+ * C/F split and patterns, bit-identical to amg_solver_update's host
+ * rebuild (C_amg1/src/build_hierarchy.c:170-242, amg_wrapper.c:114-149);
+ * the fine IC(0) and the coarsest solve are refreshed from the new values
+ * on the host, and the host hierarchy follows.  Not for the block
+ * smoother layout or a solver holding a separate solve matrix.  Extension:
+ * the reference rebuilds on the host only. */
+int bn_solver_rebuild_values_device(void *h, int nnz, const double *d_val);
+
 /* ---- batched solves: k right-hand sides stored column-major (b[j*n+i]),
  * each solved to tol; iterations per column into its[k]. */
 int bn_solver_solve_batch(void *h, int k, const double *b, double *x, double tol, int maxiter,

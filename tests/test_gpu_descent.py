@@ -201,3 +201,13 @@ def test_twenty_descent_steps(problem):
         fit2.free()
     print(f"values-only refresh losses {['%.6e' % x for x in losses2]}; iterations {its}", flush=True)
     np.testing.assert_allclose(losses2, losses[:6], rtol=1e-7)
+
+    case("5 steps rebuilding the hierarchy on the device follow the host rebuild's losses")
+    fit3 = PressureFit(net, obs, p_obs, tol=1e-11, coarsening="pmis", refresh="device")
+    try:
+        _, losses3 = fit3.descend(logr0, steps=5)
+        refresh_ms = [round(r["refresh_ms"], 1) for r in fit3.log[1:]]
+    finally:
+        fit3.free()
+    print(f"device-rebuild losses {['%.6e' % x for x in losses3]}; refresh ms {refresh_ms}", flush=True)
+    np.testing.assert_allclose(losses3, losses[:6], rtol=1e-9)
