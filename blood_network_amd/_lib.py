@@ -153,6 +153,8 @@ def _load():
                                                     C.c_int, VP, VP, VP]),
         "bn_solver_set_values_device": (C.c_int, [VP, C.c_int, VP]),
         "bn_solver_rebuild_values_device": (C.c_int, [VP, C.c_int, VP]),
+        "bn_solver_coarse_inverse": (C.c_int, [VP, IP, DP]),
+        "bn_dense_inverse": (C.c_int, [C.POINTER(CSR), DP]),
         "bn_solver_solve_batch": (C.c_int, [VP, C.c_int, DP, DP, C.c_double, C.c_int, IP, C.POINTER(SolveInfo)]),
         "bn_solver_solve_segments": (C.c_int, [VP, C.c_int, IP, DP, DP, C.c_double, C.c_int, IP,
                                                C.POINTER(SolveInfo)]),

@@ -208,6 +208,14 @@ class AMGSolver:
         (amg_solver_update's result, computed on the GPU)."""
         self._check(L.lib.bn_solver_rebuild_values_device(self.h, nnz, val_ptr))
 
+    def coarse_inverse(self):
+        """The solver's dense inverse of the coarsest operator (n_c x n_c)."""
+        n = C.c_int(0)
+        self._check(L.lib.bn_solver_coarse_inverse(self.h, C.byref(n), None))
+        out = np.zeros((n.value, n.value))
+        self._check(L.lib.bn_solver_coarse_inverse(self.h, C.byref(n), out.ctypes.data_as(L.DP)))
+        return out
+
     def solve_batch(self, B, tol=1e-8, maxiter=1000):
         """B: (k, n) right-hand sides; returns X (k, n) and per-column counts."""
         B = np.ascontiguousarray(B, dtype=np.float64)

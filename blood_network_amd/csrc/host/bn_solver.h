@@ -221,6 +221,7 @@ void bn_drop_graphs(struct bn_solver *s);
 int bn_value_map(struct bn_solver *s);
 /* rebuild.c: frees the device rebuild plan's host side */
 void bn_rebuild_release(struct bn_solver *s);
+int bn_coarse_inverse_device(struct bn_solver *s, const dcsr_mat *A, double *cinv_d, float *cinv_s, int *clamped);
 
 /* device.c */
 int bn_device_upload(struct bn_solver *s);

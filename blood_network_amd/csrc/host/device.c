@@ -693,11 +693,17 @@ int bn_device_upload(struct bn_solver *s) {
     if (nlev == 1 && s->opt.coarse_solver == BN_COARSE_AUTO) dense = (nc <= BN_DENSE_MAX);
     s->coarse_dense = dense;
     if (dense) {
-      double *inv_h = (double *)bn_xmalloc((size_t)nc * (size_t)nc * sizeof(double));
-      if (bn_dense_inverse(&C->A, inv_h) && s->opt.verbose)
+      /* the entries of bn_dense_inverse, formed on the device */
+      int clamped = 0;
+      s->cinv_d = (double *)dmalloc(s, (size_t)nc * (size_t)nc * sizeof(double) + 8);
+      s->cinv_s = (float *)dmalloc(s, (size_t)nc * (size_t)nc * sizeof(float) + 8);
+      if (!s->cinv_d || !s->cinv_s) {
+        bn_set_error("device out of memory (coarsest inverse of %d rows)", nc);
+        rc = BN_ERR_HIP;
+      }
+      if (!rc) rc = bn_coarse_inverse_device(s, &C->A, s->cinv_d, s->cinv_s, &clamped);
+      if (!rc && clamped && s->opt.verbose)
         fprintf(stderr, "bnamg: coarsest operator not numerically SPD (pivot clamped)\n");
-      rc = upload_vec2(s, inv_h, nc * nc, &s->cinv_d, &s->cinv_s);
-      free(inv_h);
     } else {
       rc = upload_tri(s, &C->L_ichol, NULL, NULL, &s->coarse_ic);
     }

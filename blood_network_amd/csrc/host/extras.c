@@ -167,6 +167,22 @@ int bn_solver_set_values_device(void *h, int nnz_in, const double *d_val) {
 /* k right-hand sides, column-major; every column is solved to tol with
  * the shared hierarchy.  its[j] gets column j's iteration count; the
  * return value is the largest count (negative if any column failed). */
+int bn_solver_coarse_inverse(void *h, int *n, double *out) {
+  struct bn_solver *s = (struct bn_solver *)h;
+  if (!s || !n) return BN_ERR_ARG;
+  int rc = bn_ensure_device(s);
+  if (rc) return rc;
+  if (!s->coarse_dense) {
+    bn_set_error("the coarsest solve is not the dense inverse");
+    return BN_ERR_ARG;
+  }
+  const int nc = s->lev[s->nlev - 1].n;
+  *n = nc;
+  if (out && hipMemcpy(out, s->cinv_d, (size_t)nc * (size_t)nc * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
+    return BN_ERR_HIP;
+  return 0;
+}
+
 int bn_solver_solve_batch(void *h, int k, const double *b, double *x, double tol, int maxiter, int *its,
                           bn_solve_info *info) {
   struct bn_solver *s = (struct bn_solver *)h;

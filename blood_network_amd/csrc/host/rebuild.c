@@ -15,13 +15,21 @@
  * hierarchy is brought up to date as well, so a later re-upload gives the
  * same state.
  */
+#include <limits.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
 
 #include "bn_solver.h"
 #include "csr.h"
 #include "schedule.h"
+
+static double rb_ms(void) {
+  struct timespec t;
+  clock_gettime(CLOCK_MONOTONIC, &t);
+  return 1e3 * (double)t.tv_sec + 1e-6 * (double)t.tv_nsec;
+}
 
 void bn_set_error(const char *fmt, ...);
 void bn_track_device(struct bn_solver *s, void *p);
@@ -29,17 +37,31 @@ int bn_ensure_device(struct bn_solver *s);
 int bn_dense_inverse(const dcsr_mat *A, double *inv);
 
 typedef struct {
-  int *s_ia, *s_col, *s_slot, *cpt; /* strength rows sorted, P slot of each strong C point */
-  int *rmap;                        /* R entry -> P entry */
-  int *c_col, *c_pos, *tpos;        /* sorted row copy and transpose positions (levels >= 1) */
-  int *acmap;                       /* colour-ordered GS row entry -> level entry */
+  /* standard interpolation (every level but the last): per A entry class
+   * (0 strong, 1 weak, 2 diagonal), C points, the (F row, strong F
+   * neighbour) pairs of each row with the A entry of each and the row-f
+   * entries its den sums, and the terms of each P entry */
+  unsigned char *cls;
+  int *cpt, *rp, *pair_k, *doff, *dq;
+  int *prow, *poff, *pa, *pq;
+  int npairs;
+  int *rmap; /* R entry -> P entry */
+  /* Galerkin product of this level: per coarse entry triple lists (pull),
+   * or one thread per coarse row over the sorted coarse rows below */
+  int pull, nlong;
+  int *toff, *t1, *t2, *t3, *elist;
+  int *c_col, *c_pos, *tpos; /* sorted row copy and transpose positions (levels >= 1) */
+  int *acmap;                /* colour-ordered GS row entry -> level entry */
 } rb_level;
 
 typedef struct {
   rb_level lev[AMG_MAX_LEVELS];
   int *tpos0;    /* fine level: transpose positions (input symmetrisation) */
   double *work;  /* product values / gathered input, max nnz over levels */
+  double *iw;    /* interpolation scratch: den per pair, dhat per row */
+  double *dn_c, *dn_r; /* coarsest inverse: the factor column- and row-major */
   int maxnnz;
+  long long maxiw;
 } rb_plan;
 
 void bn_rebuild_release(struct bn_solver *s) {
@@ -121,11 +143,283 @@ static int transpose_positions(const dcsr_mat *A, const int *col, const int *pos
   return bad ? -1 : 0;
 }
 
+static unsigned char *rb_upload_bytes(struct bn_solver *s, const unsigned char *h, size_t n) {
+  unsigned char *d = (unsigned char *)rb_dev(s, n);
+  if (d && n && hipMemcpy(d, h, n, hipMemcpyHostToDevice) != hipSuccess) return NULL;
+  return d;
+}
+
+static int *xcalloc_int(size_t n) {
+  int *p = (int *)bn_xmalloc((n > 0 ? n : 1) * sizeof(int));
+  memset(p, 0, (n > 0 ? n : 1) * sizeof(int));
+  return p;
+}
+
+/* exclusive prefix sums of cnt[0..n) into off[0..n]; -1 past INT_MAX */
+static int scan_counts(const int *cnt, int n, int *off) {
+  long long t = 0;
+  for (int i = 0; i < n; i++) {
+    off[i] = (int)t;
+    t += cnt[i];
+    if (t > INT_MAX) return -1;
+  }
+  off[n] = (int)t;
+  return 0;
+}
+
+/* strong neighbours of F row i in mk: the P slot of a strong C point (slots
+ * in the order the interpolation's sparsity pass assigns them), -2 for a
+ * strong F point; returns the slot past the last */
+static int mark_strong(const amg_level *L, int i, int *mk) {
+  const dcsr_mat *S = &L->A_filtered;
+  int slot = L->P.ia[i];
+  for (int k = S->ia[i]; k < S->ia[i + 1]; k++) {
+    const int j = S->ja[k];
+    mk[j] = L->mis[j] ? slot++ : -2;
+  }
+  return slot;
+}
+
+static void unmark(const amg_level *L, int i, int *mk) {
+  const dcsr_mat *S = &L->A_filtered;
+  for (int k = S->ia[i]; k < S->ia[i + 1]; k++) mk[S->ja[k]] = -1;
+}
+
+/* Contribution lists of the standard interpolation of one level
+ * (C_amg1/src/rs_classical.c, rs_standard_interpolation_values): for F
+ * row i, the A entries (a_ij with j a strong C point) that start each
+ * weight, the strong F neighbours f in row order with the entries of row
+ * f that their den sums (columns that are strong C points of i), and for
+ * each weight the terms a_if a_fj / den that follow, in the order the
+ * host adds them. */
+static int plan_interp(struct bn_solver *s, const amg_level *L, rb_level *r, long long *iw) {
+  const dcsr_mat *A = &L->A, *P = &L->P;
+  const int n = A->row, np = P->nnz;
+  int *npr = xcalloc_int((size_t)n + 1), *ndr = xcalloc_int((size_t)n + 1), *pcnt = xcalloc_int((size_t)np + 1);
+  unsigned char *cls = (unsigned char *)bn_xmalloc((size_t)(A->nnz > 0 ? A->nnz : 1));
+  int *cpt = xcalloc_int((size_t)n);
+  int bad = 0, maxp = 1, rc = 0;
+  for (int i = 0; i < n; i++)
+    if (P->ia[i + 1] - P->ia[i] > maxp) maxp = P->ia[i + 1] - P->ia[i];
+#pragma omp parallel reduction(| : bad)
+  {
+    int *mk = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+    for (int j = 0; j < n; j++) mk[j] = -1;
+#pragma omp for schedule(dynamic, 256)
+    for (int i = 0; i < n; i++) {
+      cpt[i] = L->mis[i] ? 1 : 0;
+      if (cpt[i]) {
+        for (int k = A->ia[i]; k < A->ia[i + 1]; k++) cls[k] = 1;
+        continue;
+      }
+      if (mark_strong(L, i, mk) != P->ia[i + 1]) {
+        bad = 1;
+        unmark(L, i, mk);
+        continue;
+      }
+      for (int k = A->ia[i]; k < A->ia[i + 1]; k++) {
+        const int j = A->ja[k];
+        if (j == i) {
+          cls[k] = 2;
+          continue;
+        }
+        const int m = mk[j];
+        cls[k] = m == -1 ? 1 : 0;
+        if (m >= 0) {
+          pcnt[m]++;
+        } else if (m == -2) {
+          npr[i]++;
+          for (int q = A->ia[j]; q < A->ia[j + 1]; q++) {
+            const int m2 = mk[A->ja[q]];
+            if (m2 >= 0) ndr[i]++, pcnt[m2]++;
+          }
+        }
+      }
+      unmark(L, i, mk);
+    }
+    free(mk);
+  }
+  int *rp = xcalloc_int((size_t)n + 1), *dbase = xcalloc_int((size_t)n + 1), *poff = xcalloc_int((size_t)np + 1);
+  if (bad) {
+    bn_set_error("interpolation pattern does not match the strong C points");
+    rc = BN_ERR_ARG;
+  } else if (scan_counts(npr, n, rp) || scan_counts(ndr, n, dbase) || scan_counts(pcnt, np, poff)) {
+    bn_set_error("interpolation lists exceed 2^31 entries");
+    rc = BN_ERR_ARG;
+  }
+  const int npairs = rc ? 0 : rp[n], nd = rc ? 0 : dbase[n], nt = rc ? 0 : poff[np];
+  int *pair_k = xcalloc_int((size_t)npairs), *doff = xcalloc_int((size_t)npairs + 1), *dq = xcalloc_int((size_t)nd);
+  int *prow = xcalloc_int((size_t)np), *pa = xcalloc_int((size_t)nt), *pq = xcalloc_int((size_t)nt);
+  if (!rc) {
+#pragma omp parallel
+    {
+      int *mk = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+      int *cur = (int *)bn_xmalloc((size_t)maxp * sizeof(int));
+      for (int j = 0; j < n; j++) mk[j] = -1;
+#pragma omp for schedule(dynamic, 256)
+      for (int i = 0; i < n; i++) {
+        const int p0 = P->ia[i];
+        for (int p = p0; p < P->ia[i + 1]; p++) prow[p] = i;
+        if (cpt[i]) continue;
+        mark_strong(L, i, mk);
+        for (int p = p0; p < P->ia[i + 1]; p++) cur[p - p0] = poff[p];
+        /* the direct entries first: the host adds them in a pass of its own */
+        for (int k = A->ia[i]; k < A->ia[i + 1]; k++) {
+          const int j = A->ja[k];
+          if (j == i || mk[j] < 0) continue;
+          const int t = cur[mk[j] - p0]++;
+          pa[t] = k;
+          pq[t] = -1;
+        }
+        int pair = rp[i], dpos = dbase[i];
+        for (int k = A->ia[i]; k < A->ia[i + 1]; k++) {
+          const int j = A->ja[k];
+          if (j == i || mk[j] != -2) continue;
+          pair_k[pair] = k;
+          doff[pair] = dpos;
+          for (int q = A->ia[j]; q < A->ia[j + 1]; q++) {
+            const int m2 = mk[A->ja[q]];
+            if (m2 < 0) continue;
+            dq[dpos++] = q;
+            const int t = cur[m2 - p0]++;
+            pa[t] = pair;
+            pq[t] = q;
+          }
+          pair++;
+        }
+        unmark(L, i, mk);
+      }
+      free(cur);
+      free(mk);
+    }
+    doff[npairs] = nd;
+    r->npairs = npairs;
+    r->cls = rb_upload_bytes(s, cls, (size_t)A->nnz);
+    r->cpt = rb_upload_int(s, cpt, (size_t)n);
+    r->rp = rb_upload_int(s, rp, (size_t)n + 1);
+    r->pair_k = rb_upload_int(s, pair_k, (size_t)npairs);
+    r->doff = rb_upload_int(s, doff, (size_t)npairs + 1);
+    r->dq = rb_upload_int(s, dq, (size_t)nd);
+    r->prow = rb_upload_int(s, prow, (size_t)np);
+    r->poff = rb_upload_int(s, poff, (size_t)np + 1);
+    r->pa = rb_upload_int(s, pa, (size_t)nt);
+    r->pq = rb_upload_int(s, pq, (size_t)nt);
+    if ((long long)npairs + n > *iw) *iw = (long long)npairs + n;
+  }
+  free(npr);
+  free(ndr);
+  free(pcnt);
+  free(cls);
+  free(cpt);
+  free(rp);
+  free(dbase);
+  free(poff);
+  free(pair_k);
+  free(doff);
+  free(dq);
+  free(prow);
+  free(pa);
+  free(pq);
+  return rc;
+}
+
+/* Per coarse entry lists of the Galerkin product R A P (the host's
+ * row-by-row product: R row, then A row, then P row), for a level whose
+ * busiest coarse row has more than thresh terms (thresh < 0: never); a
+ * lighter level keeps one thread per coarse row. */
+static int plan_rap(struct bn_solver *s, const dcsr_mat *R, const dcsr_mat *A, const dcsr_mat *P, const dcsr_mat *C,
+                    rb_level *r, long long thresh) {
+  r->pull = 0;
+  if (thresh < 0) return 0;
+  const int nf = A->row, nc = C->row, nnz = C->nnz;
+  long long *ap = (long long *)bn_xmalloc((size_t)(nf > 0 ? nf : 1) * sizeof(long long));
+#pragma omp parallel for schedule(static)
+  for (int i = 0; i < nf; i++) {
+    long long t = 0;
+    for (int q = A->ia[i]; q < A->ia[i + 1]; q++) t += P->ia[A->ja[q] + 1] - P->ia[A->ja[q]];
+    ap[i] = t;
+  }
+  long long mx = 0, tot = 0;
+#pragma omp parallel for schedule(static) reduction(max : mx) reduction(+ : tot)
+  for (int I = 0; I < nc; I++) {
+    long long t = 0;
+    for (int q = R->ia[I]; q < R->ia[I + 1]; q++) t += ap[R->ja[q]];
+    if (t > mx) mx = t;
+    tot += t;
+  }
+  free(ap);
+  if (mx <= thresh || tot > INT_MAX) return 0;
+  int *cnt = xcalloc_int((size_t)nnz + 1), *toff = xcalloc_int((size_t)nnz + 1);
+  for (int pass = 0; pass < 2; pass++) {
+    if (pass == 1) {
+      scan_counts(cnt, nnz, toff); /* below tot <= INT_MAX */
+      memcpy(cnt, toff, (size_t)nnz * sizeof(int));
+    }
+    const int T = toff[nnz];
+    int *t1 = NULL, *t2 = NULL, *t3 = NULL;
+    if (pass == 1) t1 = xcalloc_int((size_t)T), t2 = xcalloc_int((size_t)T), t3 = xcalloc_int((size_t)T);
+#pragma omp parallel
+    {
+      int *mk = (int *)bn_xmalloc((size_t)(nc > 0 ? nc : 1) * sizeof(int));
+      for (int j = 0; j < nc; j++) mk[j] = -1;
+#pragma omp for schedule(dynamic, 16)
+      for (int I = 0; I < nc; I++) {
+        for (int q = C->ia[I]; q < C->ia[I + 1]; q++) mk[C->ja[q]] = q;
+        for (int q1 = R->ia[I]; q1 < R->ia[I + 1]; q1++) {
+          const int i1 = R->ja[q1];
+          for (int q2 = A->ia[i1]; q2 < A->ia[i1 + 1]; q2++) {
+            const int i2 = A->ja[q2];
+            for (int q3 = P->ia[i2]; q3 < P->ia[i2 + 1]; q3++) {
+              const int m = mk[P->ja[q3]];
+              if (m < 0) continue;
+              if (pass == 0) {
+                cnt[m]++;
+              } else {
+                const int t = cnt[m]++;
+                t1[t] = q1, t2[t] = q2, t3[t] = q3;
+              }
+            }
+          }
+        }
+        for (int q = C->ia[I]; q < C->ia[I + 1]; q++) mk[C->ja[q]] = -1;
+      }
+      free(mk);
+    }
+    if (pass == 1) {
+      int nlong = 0;
+      for (int e = 0; e < nnz; e++) nlong += toff[e + 1] - toff[e] > BN_RB_LONG;
+      int *el = xcalloc_int((size_t)nlong);
+      nlong = 0;
+      for (int e = 0; e < nnz; e++)
+        if (toff[e + 1] - toff[e] > BN_RB_LONG) el[nlong++] = e;
+      r->toff = rb_upload_int(s, toff, (size_t)nnz + 1);
+      r->t1 = rb_upload_int(s, t1, (size_t)T);
+      r->t2 = rb_upload_int(s, t2, (size_t)T);
+      r->t3 = rb_upload_int(s, t3, (size_t)T);
+      r->elist = rb_upload_int(s, el, (size_t)nlong);
+      r->nlong = nlong;
+      r->pull = 1;
+      free(el);
+      free(t1);
+      free(t2);
+      free(t3);
+    }
+  }
+  free(cnt);
+  free(toff);
+  return 0;
+}
+
 static int plan_build(struct bn_solver *s) {
   rb_plan *pl = (rb_plan *)calloc(1, sizeof(rb_plan));
   const amg_data *amg = s->amg;
   const int nlev = s->nlev;
   int rc = 0;
+  /* product lists for levels whose busiest coarse row has more terms than
+   * this (BN_RB_PULL; negative: never) */
+  long long thresh = 2048;
+  const char *env = getenv("BN_RB_PULL");
+  if (env && *env) thresh = atoll(env);
   for (int l = 0; l < nlev && !rc; l++) {
     const amg_level *L = &amg->levels[l];
     const dcsr_mat *A = &L->A;
@@ -141,8 +435,10 @@ static int plan_build(struct bn_solver *s) {
     }
     if (!rc && l == 0) pl->tpos0 = rb_upload_int(s, tp, (size_t)A->nnz);
     if (!rc && l > 0) {
-      r->c_col = rb_upload_int(s, col, (size_t)A->nnz);
-      r->c_pos = rb_upload_int(s, pos, (size_t)A->nnz);
+      if (!pl->lev[l - 1].pull) {
+        r->c_col = rb_upload_int(s, col, (size_t)A->nnz);
+        r->c_pos = rb_upload_int(s, pos, (size_t)A->nnz);
+      }
       r->tpos = rb_upload_int(s, tp, (size_t)A->nnz);
     }
     free(col);
@@ -167,45 +463,11 @@ static int plan_build(struct bn_solver *s) {
       free(cperm);
     }
     if (rc || l == nlev - 1) continue;
-    /* strength rows sorted, with the P slot of each strong C point (slots
-     * in the order the interpolation's sparsity pass assigns them) */
-    const dcsr_mat *S = &L->A_filtered;
-    const dcsr_mat *P = &L->P;
-    int *scol = (int *)bn_xmalloc((size_t)(S->nnz > 0 ? S->nnz : 1) * sizeof(int));
-    int *sslot = (int *)bn_xmalloc((size_t)(S->nnz > 0 ? S->nnz : 1) * sizeof(int));
-    int *cpt = (int *)bn_xmalloc((size_t)(L->n > 0 ? L->n : 1) * sizeof(int));
-    int bad = 0;
-#pragma omp parallel for schedule(dynamic, 1024) reduction(| : bad)
-    for (int i = 0; i < S->row; i++) {
-      cpt[i] = L->mis[i] ? 1 : 0;
-      const int lo = S->ia[i], len = S->ia[i + 1] - lo;
-      pair_t *pp = (pair_t *)malloc((size_t)(len > 0 ? len : 1) * sizeof(pair_t));
-      int slot = P->ia[i];
-      for (int k = 0; k < len; k++) {
-        const int j = S->ja[lo + k];
-        pp[k].col = j;
-        pp[k].val = L->mis[j] ? slot++ : -1;
-      }
-      if (!L->mis[i] && slot != P->ia[i + 1]) bad = 1;
-      sort_row(pp, len);
-      for (int k = 0; k < len; k++) scol[lo + k] = pp[k].col, sslot[lo + k] = pp[k].val;
-      free(pp);
-    }
-    if (bad) {
-      bn_set_error("level %d: interpolation pattern does not match the strong C points", l);
-      rc = BN_ERR_ARG;
-    }
+    if (!rc) rc = plan_interp(s, L, r, &pl->maxiw);
+    /* R = P^T as the upload built it (stable transpose): entry -> P entry,
+     * and the product lists on that R */
     if (!rc) {
-      r->s_ia = rb_upload_int(s, S->ia, (size_t)S->row + 1);
-      r->s_col = rb_upload_int(s, scol, (size_t)S->nnz);
-      r->s_slot = rb_upload_int(s, sslot, (size_t)S->nnz);
-      r->cpt = rb_upload_int(s, cpt, (size_t)L->n);
-    }
-    free(scol);
-    free(sslot);
-    free(cpt);
-    /* R = P^T as the upload built it (stable transpose): entry -> P entry */
-    if (!rc) {
+      const dcsr_mat *P = &L->P;
       dcsr_mat Pi = *P, Ri;
       Pi.val = (double *)bn_xmalloc((size_t)(P->nnz > 0 ? P->nnz : 1) * sizeof(double));
       for (int k = 0; k < P->nnz; k++) Pi.val[k] = (double)k;
@@ -214,25 +476,47 @@ static int plan_build(struct bn_solver *s) {
       for (int k = 0; k < Ri.nnz; k++) m[k] = (int)Ri.val[k];
       r->rmap = rb_upload_int(s, m, (size_t)Ri.nnz);
       free(m);
+      rc = plan_rap(s, &Ri, A, P, &amg->levels[l + 1].A, r, thresh);
       bn_csr_free(&Ri);
       free(Pi.val);
     }
   }
   if (!rc) pl->work = (double *)rb_dev(s, (size_t)(pl->maxnnz > 0 ? pl->maxnnz : 1) * sizeof(double));
+  if (!rc) pl->iw = (double *)rb_dev(s, (size_t)(pl->maxiw > 0 ? pl->maxiw : 1) * sizeof(double));
   for (int l = 0; l < nlev && !rc; l++) {
     const rb_level *r = &pl->lev[l];
     const int last = (l == nlev - 1);
-    if ((l == 0 && !pl->tpos0) || (l > 0 && (!r->c_col || !r->c_pos || !r->tpos)) ||
+    if ((l == 0 && !pl->tpos0) || (l > 0 && !r->tpos) || (l > 0 && !pl->lev[l - 1].pull && (!r->c_col || !r->c_pos)) ||
         (s->lev[l].use_gs && !r->acmap) ||
-        (!last && (!r->s_ia || !r->s_col || !r->s_slot || !r->cpt || !r->rmap)))
+        (!last && (!r->cls || !r->cpt || !r->rp || !r->pair_k || !r->doff || !r->dq || !r->prow || !r->poff ||
+                   !r->pa || !r->pq || !r->rmap)) ||
+        (r->pull && (!r->toff || !r->t1 || !r->t2 || !r->t3 || !r->elist)))
       rc = BN_ERR_HIP;
   }
-  if (!rc && !pl->work) rc = BN_ERR_HIP;
+  if (!rc && s->coarse_dense) {
+    const size_t nc = (size_t)s->lev[nlev - 1].n;
+    pl->dn_c = (double *)rb_dev(s, nc * nc * sizeof(double));
+    pl->dn_r = (double *)rb_dev(s, nc * nc * sizeof(double));
+    if (!pl->dn_c || !pl->dn_r) rc = BN_ERR_HIP;
+  }
+  if (!rc && (!pl->work || !pl->iw)) rc = BN_ERR_HIP;
   if (rc) {
     free(pl);
     if (rc == BN_ERR_HIP) bn_set_error("device memory for the rebuild plan");
     return rc;
   }
+  const char *lg = getenv("BN_RB_LOG");
+  if (lg && *lg && *lg != '0')
+    for (int l = 0; l + 1 < nlev; l++) {
+      const rb_level *r = &pl->lev[l];
+      int t = 0;
+      if (r->pull && hipMemcpy(&t, r->toff + amg->levels[l + 1].A.nnz, sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+        t = -1;
+      fprintf(stderr, "[rebuild plan] level %d: %d strong F pairs; product %s", l, r->npairs,
+              r->pull ? "per-entry lists" : "per coarse row");
+      if (r->pull) fprintf(stderr, " (%d terms, %d entries over %d terms)", t, r->nlong, BN_RB_LONG);
+      fprintf(stderr, "\n");
+    }
   s->rb = pl;
   return 0;
 }
@@ -297,9 +581,14 @@ int bn_solver_rebuild_values_device(void *h, int nnz_in, const double *d_val) {
     bn_set_error("rebuild: the solver holds a solve matrix apart from its hierarchy");
     return BN_ERR_ARG;
   }
+  const char *lg = getenv("BN_RB_LOG");
+  const int log = lg && *lg && *lg != '0';
+  double tm[6];
+  tm[0] = rb_ms();
   if (!s->vperm_state && (rc = bn_value_map(s))) return rc;
   if (!s->rb && (rc = plan_build(s))) return rc;
   rb_plan *pl = (rb_plan *)s->rb;
+  tm[1] = rb_ms();
   hipStream_t st = s->stream;
   const int n0 = s->lev[0].n, nz0 = s->lev[0].A.nnz;
 
@@ -332,11 +621,19 @@ int bn_solver_rebuild_values_device(void *h, int nnz_in, const double *d_val) {
   for (int l = 0; l < nlev - 1; l++) {
     bn_dlevel *d = &s->lev[l], *c = &s->lev[l + 1];
     const rb_level *r = &pl->lev[l], *rc1 = &pl->lev[l + 1];
-    bn_rb_interp(d->n, d->A.ia, d->A.ja, d->A.vd, r->s_ia, r->s_col, r->s_slot, r->cpt, d->P.ia, d->P.vd, st);
+    double *den = pl->iw, *dhat = pl->iw + r->npairs;
+    bn_rb_den(r->npairs, r->doff, r->dq, d->A.vd, den, st);
+    bn_rb_dhat(d->n, d->A.ia, d->A.vd, r->cls, r->cpt, r->rp, r->pair_k, den, dhat, st);
+    bn_rb_pval(d->P.nnz, r->prow, r->cpt, d->P.ia, r->poff, r->pa, r->pq, r->pair_k, d->A.vd, den, dhat, d->P.vd,
+               st);
     bn_rb_to_float(d->P.nnz, d->P.vd, d->P.vs, st);
     bn_rb_gather(d->R.nnz, r->rmap, d->P.vd, d->R.vd, d->R.vs, st);
-    bn_rb_rap(c->n, d->R.ia, d->R.ja, d->R.vd, d->A.ia, d->A.ja, d->A.vd, d->P.ia, d->P.ja, d->P.vd, c->A.ia,
-              rc1->c_col, rc1->c_pos, pl->work, st);
+    if (r->pull)
+      bn_rb_rap_pull(c->A.nnz, r->toff, r->t1, r->t2, r->t3, r->nlong, r->elist, d->R.vd, d->A.vd, d->P.vd,
+                     pl->work, st);
+    else
+      bn_rb_rap(c->n, d->R.ia, d->R.ja, d->R.vd, d->A.ia, d->A.ja, d->A.vd, d->P.ia, d->P.ja, d->P.vd, c->A.ia,
+                rc1->c_col, rc1->c_pos, pl->work, st);
     bn_rb_symmetrize(c->A.nnz, rc1->tpos, pl->work, c->A.vd, st);
   }
   /* derived per-level arrays */
@@ -357,6 +654,7 @@ int bn_solver_rebuild_values_device(void *h, int nnz_in, const double *d_val) {
     bn_set_error("device failure in the value-only rebuild");
     return BN_ERR_HIP;
   }
+  tm[2] = rb_ms();
 
   /* the host hierarchy follows (level and interpolation values, l1 norms) */
   hipError_t e = hipSuccess;
@@ -370,27 +668,71 @@ int bn_solver_rebuild_values_device(void *h, int nnz_in, const double *d_val) {
   if (e != hipSuccess) return BN_ERR_HIP;
   memcpy(s->A_own.val, amg->levels[0].A.val, (size_t)nz0 * sizeof(double));
   (void)n0;
+  tm[3] = rb_ms();
 
   /* fine IC(0) and the coarsest solve, as amg_solver_update forms them */
   ichol_free(&s->L_own);
   ichol_compute(s->A, &s->L_own);
   if (s->fine_ic.lo.ia && (rc = refresh_tri(s, &s->L_own, &s->fine_ic))) return rc;
+  tm[4] = rb_ms();
   amg_level *C = &amg->levels[nlev - 1];
   ichol_free(&C->L_ichol);
   ichol_compute(&C->A, &C->L_ichol);
   if (s->coarse_dense) {
-    const int nc = C->A.row;
-    double *inv = (double *)bn_xmalloc((size_t)nc * (size_t)nc * sizeof(double));
-    float *invs = (float *)bn_xmalloc((size_t)nc * (size_t)nc * sizeof(float));
-    bn_dense_inverse(&C->A, inv);
-    for (size_t k = 0; k < (size_t)nc * (size_t)nc; k++) invs[k] = (float)inv[k];
-    e = hipMemcpy(s->cinv_d, inv, (size_t)nc * (size_t)nc * sizeof(double), hipMemcpyHostToDevice);
-    if (e == hipSuccess) e = hipMemcpy(s->cinv_s, invs, (size_t)nc * (size_t)nc * sizeof(float), hipMemcpyHostToDevice);
-    free(inv);
-    free(invs);
-    if (e != hipSuccess) return BN_ERR_HIP;
+    /* the entries of bn_dense_inverse, formed on the device */
+    const bn_dlevel *dc = &s->lev[nlev - 1];
+    if (bn_dn_inverse(dc->n, dc->A.ia, dc->A.ja, dc->A.vd, pl->dn_c, pl->dn_r, s->cinv_d, s->cinv_s, NULL, st) ||
+        hipStreamSynchronize(st) != hipSuccess) {
+      bn_set_error("device failure in the coarsest inverse");
+      return BN_ERR_HIP;
+    }
   } else if ((rc = refresh_tri(s, &C->L_ichol, &s->coarse_ic))) {
     return rc;
   }
+  tm[5] = rb_ms();
+  if (log)
+    fprintf(stderr,
+            "[rebuild] plan %.1f ms, device levels %.1f ms, host copy of the hierarchy %.1f ms, fine IC(0) %.1f ms, "
+            "coarsest solve (%d rows, %s) %.1f ms\n",
+            tm[1] - tm[0], tm[2] - tm[1], tm[3] - tm[2], tm[4] - tm[3], C->A.row, s->coarse_dense ? "dense" : "IC(0)",
+            tm[5] - tm[4]);
   return 0;
 }
+
+/* The coarsest inverse of a host CSR matrix formed on the device into
+ * cinv_d / cinv_s (entries of bn_dense_inverse); the matrix copy and the
+ * dense factor scratch are released before returning.  *clamped as
+ * bn_dense_inverse returns it. */
+int bn_coarse_inverse_device(struct bn_solver *s, const dcsr_mat *A, double *cinv_d, float *cinv_s, int *clamped) {
+  const int n = A->row, nnz = A->nnz;
+  const size_t nn = (size_t)n * (size_t)n;
+  double *Lc = NULL, *Lr = NULL, *va = NULL;
+  int *ia = NULL, *ja = NULL, *flag = NULL, zero = 0, rc = 0;
+  hipError_t e = hipMalloc((void **)&Lc, nn * sizeof(double) + 8);
+  if (e == hipSuccess) e = hipMalloc((void **)&Lr, nn * sizeof(double) + 8);
+  if (e == hipSuccess) e = hipMalloc((void **)&ia, ((size_t)n + 1) * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void **)&ja, (size_t)(nnz > 0 ? nnz : 1) * sizeof(int));
+  if (e == hipSuccess) e = hipMalloc((void **)&va, (size_t)(nnz > 0 ? nnz : 1) * sizeof(double));
+  if (e == hipSuccess) e = hipMalloc((void **)&flag, sizeof(int));
+  if (e == hipSuccess) e = hipMemcpy(ia, A->ia, ((size_t)n + 1) * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && nnz) e = hipMemcpy(ja, A->ja, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && nnz) e = hipMemcpy(va, A->val, (size_t)nnz * sizeof(double), hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMemcpy(flag, &zero, sizeof(int), hipMemcpyHostToDevice);
+  if (e == hipSuccess && bn_dn_inverse(n, ia, ja, va, Lc, Lr, cinv_d, cinv_s, flag, s->stream)) rc = BN_ERR_ARG;
+  if (e == hipSuccess && !rc) e = hipStreamSynchronize(s->stream);
+  if (e == hipSuccess && !rc) e = hipMemcpy(&zero, flag, sizeof(int), hipMemcpyDeviceToHost);
+  if (clamped) *clamped = zero;
+  hipFree(Lc);
+  hipFree(Lr);
+  hipFree(ia);
+  hipFree(ja);
+  hipFree(va);
+  hipFree(flag);
+  if (e != hipSuccess) {
+    bn_set_error("device failure in the coarsest inverse (%d rows)", n);
+    return BN_ERR_HIP;
+  }
+  if (rc) bn_set_error("coarsest inverse: %d rows is beyond the dense limit", n);
+  return rc;
+}
+

@@ -280,8 +280,31 @@ void bn_rb_row_scalars(int n, const int *ia, const int *ja, const double *a, dou
                        double *l1inv, float *l1invs, double *l1, hipStream_t st);
 void bn_rb_dinv_map(int n, const int *ia, const int *ja, const double *a, const int *map, double *dinv, float *dinvs,
                     hipStream_t st);
-void bn_rb_interp(int n, const int *aia, const int *aja, const double *a, const int *sia, const int *scol,
-                  const int *sslot, const int *cpt, const int *pia, double *pv, hipStream_t st);
+/* standard interpolation from contribution lists: den per (F row, strong F
+ * neighbour) pair, dhat per F row, then every P entry */
+void bn_rb_den(int npairs, const int *doff, const int *dq, const double *a, double *den, hipStream_t st);
+void bn_rb_dhat(int n, const int *aia, const double *a, const unsigned char *cls, const int *cpt, const int *rp,
+                const int *pair_k, const double *den, double *dhat, hipStream_t st);
+void bn_rb_pval(int nnzp, const int *prow, const int *cpt, const int *pia, const int *poff, const int *pa,
+                const int *pq, const int *pair_k, const double *a, const double *den, const double *dhat, double *pv,
+                hipStream_t st);
+/* lists longer than this take a wavefront each */
+#define BN_RB_LONG 64
+/* Galerkin entries from per-entry triple lists (toff, (t1, t2, t3) = R, A,
+ * P entry); the nlong entries in elist, whose lists exceed 64 terms, take
+ * one wavefront each */
+void bn_rb_rap_pull(int nnz, const int *toff, const int *t1, const int *t2, const int *t3, int nlong,
+                    const int *elist, const double *rv, const double *av, const double *pv, double *out,
+                    hipStream_t st);
+/* Dense inverse of an SPD CSR matrix of n <= 8192 rows on the device,
+ * entry for entry as dense.c's bn_dense_inverse forms it: Lc and Lr are
+ * n x n scratch (the factor column-major and its row-major copy), inv the
+ * row-major inverse and invs (may be NULL) its FP32 copy; *clamped (may
+ * be NULL) is set to 1 if a pivot had to be clamped.  Returns -1 if n is
+ * out of range. */
+int bn_dn_inverse(int n, const int *ia, const int *ja, const double *a, double *Lc, double *Lr, double *inv,
+                  float *invs, int *clamped, hipStream_t st);
+/* Galerkin entries, one thread per coarse row */
 void bn_rb_rap(int nc, const int *ria, const int *rja, const double *rv, const int *aia, const int *aja,
                const double *av, const int *pia, const int *pja, const double *pv, const int *cia, const int *ccol,
                const int *cpos, double *out, hipStream_t st);

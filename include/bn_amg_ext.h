@@ -178,6 +178,15 @@ int bn_solver_set_values_device(void *h, int nnz, const double *d_val);
  * the reference rebuilds on the host only. */
 int bn_solver_rebuild_values_device(void *h, int nnz, const double *d_val);
 
+/* ---- coarsest dense inverse: the solver's copy (row-major, n x n) into
+ * out when out != NULL, its order in *n; BN_ERR_ARG if the coarsest
+ * solve is not the dense inverse.  bn_dense_inverse is the host form of
+ * the same inverse (A SPD, n = A->row; returns 1 if a pivot was clamped);
+ * the solver forms it on the device entry for entry.  Extension: the
+ * reference solves its coarsest level by an inner IC-PCG. */
+int bn_solver_coarse_inverse(void *h, int *n, double *out);
+int bn_dense_inverse(const dcsr_mat *A, double *inv);
+
 /* ---- batched solves: k right-hand sides stored column-major (b[j*n+i]),
  * each solved to tol; iterations per column into its[k]. */
 int bn_solver_solve_batch(void *h, int k, const double *b, double *x, double tol, int maxiter,

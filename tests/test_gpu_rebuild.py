@@ -4,6 +4,9 @@ amg_solver_update (C_amg1/src/build_hierarchy.c:170-242 semantics): every
 level operator, interpolation operator and IC(0) factor bit for bit, and
 the solves that follow."""
 import sys
+import time
+
+import ctypes as C
 
 import numpy as np
 import pytest
@@ -30,6 +33,24 @@ def new_values(net, seed, spread=0.3):
     return net.assemble(g)
 
 
+def host_dense_inverse(A):
+    """bn_dense_inverse on the host of a level's (ia, ja, val)."""
+    ia, ja, val = (np.ascontiguousarray(x) for x in A)
+    n = len(ia) - 1
+    csr = L.CSR(n, n, len(ja), ia.ctypes.data_as(L.IP), ja.ctypes.data_as(L.IP), val.ctypes.data_as(L.DP))
+    inv = np.zeros((n, n))
+    L.lib.bn_dense_inverse(C.byref(csr), inv.ctypes.data_as(L.DP))
+    return inv
+
+
+def same_coarse_inverse(s):
+    """the device-formed coarsest inverse equals the host form bit for bit"""
+    dev = s.coarse_inverse()
+    host = host_dense_inverse(s.level(s.num_levels() - 1)["A"])
+    assert dev.shape == host.shape
+    assert np.array_equal(dev, host), np.abs(dev - host).max()
+
+
 def same_hierarchy(a, b):
     assert a.num_levels() == b.num_levels()
     for lev in range(a.num_levels()):
@@ -42,10 +63,17 @@ def same_hierarchy(a, b):
         assert np.array_equal(x, y), "fine IC(0)"
 
 
-@pytest.mark.parametrize("coarsening,coarse", [("rs", "auto"), ("pmis", "auto"), ("hmis", "auto"),
-                                               ("pmis", "icpcg")])
-def test_device_rebuild_equals_host_rebuild(net, coarsening, coarse):
-    case(f"device rebuild vs host rebuild, {coarsening}, coarse={coarse}, n={net.n}")
+@pytest.mark.parametrize("coarsening,coarse,pull", [("rs", "auto", None), ("pmis", "auto", None),
+                                                    ("hmis", "auto", None), ("pmis", "icpcg", None),
+                                                    ("rs", "auto", "0"), ("pmis", "auto", "0"),
+                                                    ("hmis", "auto", "-1")])
+def test_device_rebuild_equals_host_rebuild(net, coarsening, coarse, pull, monkeypatch):
+    """pull: BN_RB_PULL, the busiest-row term count above which a level's
+    Galerkin product uses per-entry lists ("0": every level, "-1": none)."""
+    case(f"device rebuild vs host rebuild, {coarsening}, coarse={coarse}, BN_RB_PULL={pull}, n={net.n}")
+    if pull is not None:
+        monkeypatch.setenv("BN_RB_PULL", pull)
+    monkeypatch.setenv("BN_RB_LOG", "1")
     kw = dict(coarsening=coarsening, min_size=300, coarse=coarse)
     dev_s = AMGSolver(net.ia, net.ja, net.val, **kw)
     host_s = AMGSolver(net.ia, net.ja, net.val, **kw)
@@ -56,6 +84,9 @@ def test_device_rebuild_equals_host_rebuild(net, coarsening, coarse):
     dev_s.rebuild_values_device(len(val), d_val.ptr)
     host_s.update(net.ia, net.ja, val)
     same_hierarchy(dev_s, host_s)
+    if coarse == "auto":
+        same_coarse_inverse(dev_s)
+        same_coarse_inverse(host_s)
     x1, it1 = dev_s.solve(b, tol=1e-10)
     x2, it2 = host_s.solve(b, tol=1e-10)
     err = np.linalg.norm(x1 - x2) / np.linalg.norm(x2)
@@ -91,6 +122,37 @@ def test_device_rebuild_fp32_fast_path(net):
     err = np.linalg.norm(x1 - x2) / np.linalg.norm(x2)
     print(f"iterations {it1} / {it2}, rel diff {err:.1e}")
     assert it1 == it2 and err < 1e-6
+
+
+def test_device_rebuild_full_network(monkeypatch):
+    """The 1,065,532-unknown network (100^3 lattice, depth-15 trees): its
+    coarse levels hold hub rows whose Galerkin entries sum up to ~10^5
+    terms each (one wavefront per long list)."""
+    case("device rebuild vs host rebuild, 100^3 lattice + depth-15 trees, pmis")
+    monkeypatch.setenv("BN_RB_LOG", "1")
+    big = make_network(100, 15, 1, seed=0)
+    dev_s = AMGSolver(big.ia, big.ja, big.val, coarsening="pmis")
+    host_s = AMGSolver(big.ia, big.ja, big.val, coarsening="pmis")
+    try:
+        val, b = new_values(big, 7, 0.4)
+        d_val = L.DeviceArray(val)
+        t0 = time.perf_counter()
+        dev_s.rebuild_values_device(len(val), d_val.ptr)
+        t1 = time.perf_counter()
+        dev_s.rebuild_values_device(len(val), d_val.ptr)
+        t2 = time.perf_counter()
+        host_s.update(big.ia, big.ja, val)
+        t3 = time.perf_counter()
+        print(f"device rebuild {1e3 * (t1 - t0):.0f} ms with plan, {1e3 * (t2 - t1):.0f} ms again; "
+              f"host rebuild + upload {1e3 * (t3 - t2):.0f} ms", flush=True)
+        same_hierarchy(dev_s, host_s)
+        same_coarse_inverse(dev_s)
+        x1, it1 = dev_s.solve(b, tol=1e-10)
+        x2, it2 = host_s.solve(b, tol=1e-10)
+        assert it1 == it2 and np.linalg.norm(x1 - x2) / np.linalg.norm(x2) < 1e-9
+    finally:
+        dev_s.free()
+        host_s.free()
 
 
 def test_device_rebuild_errors(net):

@@ -199,3 +199,47 @@ def test_reference_driver_links_against_libbnamg():
     assert r.returncode == 1 and "Usage" in r.stderr
     ldd = subprocess.run(["ldd", exe], capture_output=True, text=True).stdout
     assert "libbnamg.so" in ldd
+
+
+def dense_inverse(ia, ja, val):
+    """bn_dense_inverse (the host form of the solver's coarsest inverse)."""
+    ia = np.ascontiguousarray(ia, np.int32)
+    ja = np.ascontiguousarray(ja, np.int32)
+    val = np.ascontiguousarray(val, np.float64)
+    n = len(ia) - 1
+    csr = L.CSR(n, n, len(ja), ia.ctypes.data_as(L.IP), ja.ctypes.data_as(L.IP), val.ctypes.data_as(L.DP))
+    inv = np.zeros((n, n))
+    clamped = L.lib.bn_dense_inverse(C.byref(csr), inv.ctypes.data_as(L.DP))
+    return inv, clamped
+
+
+def test_dense_inverse_against_lapack_and_closed_forms():
+    """Cholesky-based inverse against numpy (LAPACK) on a random SPD
+    sparse matrix, the closed-form inverse of the 1-D Dirichlet Laplacian
+    tridiag(-1, 2, -1) (A^-1_ij = min(i,j) (n+1-max(i,j)) / (n+1), 1-based),
+    and the clamp flag on an indefinite matrix."""
+    rng = np.random.default_rng(11)
+    n = 300
+    M = np.where(rng.random((n, n)) < 0.02, rng.normal(size=(n, n)), 0.0)
+    A = M @ M.T + n * 0.05 * np.eye(n)
+    rows, cols = np.nonzero(A)
+    ia = np.searchsorted(rows, np.arange(n + 1)).astype(np.int32)
+    inv, clamped = dense_inverse(ia, cols, A[rows, cols])
+    assert clamped == 0
+    ref = np.linalg.inv(A)
+    assert np.abs(inv - ref).max() <= 1e-11 * np.abs(ref).max()
+
+    m = 64
+    T = 2 * np.eye(m) - np.eye(m, k=1) - np.eye(m, k=-1)
+    rows, cols = np.nonzero(T)
+    ia = np.searchsorted(rows, np.arange(m + 1)).astype(np.int32)
+    inv, clamped = dense_inverse(ia, cols, T[rows, cols])
+    i = np.arange(1, m + 1)
+    exact = np.minimum.outer(i, i) * (m + 1 - np.maximum.outer(i, i)) / (m + 1)
+    assert clamped == 0
+    assert np.abs(inv - exact).max() <= 1e-12 * exact.max()
+
+    B = np.array([[1.0, 2.0], [2.0, 1.0]])
+    rows, cols = np.nonzero(B)
+    inv, clamped = dense_inverse(np.array([0, 2, 4]), cols, B[rows, cols])
+    assert clamped == 1
