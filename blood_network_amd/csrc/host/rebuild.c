@@ -60,6 +60,10 @@ typedef struct {
   double *work;  /* product values / gathered input, max nnz over levels */
   double *iw;    /* interpolation scratch: den per pair, dhat per row */
   double *dn_c, *dn_r; /* coarsest inverse: the factor column- and row-major */
+  int *ic_lmap, *ic_umap; /* fine IC(0) on the device: factor entry -> operator
+                           * entry, transposed row entry -> factor entry */
+  double *l1;              /* l1 row norms of every level (host set-up's l1_diag) */
+  long long l1off[AMG_MAX_LEVELS + 1];
   int maxnnz;
   long long maxiw;
 } rb_plan;
@@ -410,6 +414,53 @@ static int plan_rap(struct bn_solver *s, const dcsr_mat *R, const dcsr_mat *A, c
   return 0;
 }
 
+/* The fine IC(0) is formed on the device when the device factor exists
+ * and every factor row is strictly ascending with its diagonal last (the
+ * host factor is the operator's lower part in stored order); otherwise
+ * the host forms it.  Maps: factor entry -> operator entry, and the
+ * transposed rows' entries -> factor entry (as the upload transposed). */
+static int plan_ic(struct bn_solver *s, rb_plan *pl) {
+  const ichol_factor *F = &s->L_own;
+  const dcsr_mat *A = s->A;
+  if (!s->fine_ic.lo.ia || F->n != A->row || s->fine_ic.lo.nnz != F->nnz) return 0;
+  for (int i = 0; i < F->n; i++) {
+    const int lo = F->ia[i], hi = F->ia[i + 1];
+    if (hi <= lo || F->ja[hi - 1] != i) return 0;
+    for (int q = lo + 1; q < hi; q++)
+      if (F->ja[q] <= F->ja[q - 1]) return 0;
+  }
+  int *lmap = (int *)bn_xmalloc((size_t)(F->nnz > 0 ? F->nnz : 1) * sizeof(int));
+  int bad = 0;
+  for (int i = 0; i < A->row && !bad; i++) {
+    int p = F->ia[i];
+    for (int k = A->ia[i]; k < A->ia[i + 1]; k++)
+      if (A->ja[k] <= i) {
+        if (p >= F->ia[i + 1] || F->ja[p] != A->ja[k]) bad = 1;
+        else lmap[p++] = k;
+      }
+    if (p != F->ia[i + 1]) bad = 1;
+  }
+  if (!bad) {
+    dcsr_mat Li = {F->n, F->n, F->nnz, F->ia, F->ja, NULL}, Ui;
+    Li.val = (double *)bn_xmalloc((size_t)(F->nnz > 0 ? F->nnz : 1) * sizeof(double));
+    for (int k = 0; k < F->nnz; k++) Li.val[k] = (double)k;
+    bn_csr_transpose(&Li, &Ui);
+    int *umap = (int *)bn_xmalloc((size_t)(Ui.nnz > 0 ? Ui.nnz : 1) * sizeof(int));
+    for (int k = 0; k < Ui.nnz; k++) umap[k] = (int)Ui.val[k];
+    pl->ic_lmap = rb_upload_int(s, lmap, (size_t)F->nnz);
+    pl->ic_umap = rb_upload_int(s, umap, (size_t)Ui.nnz);
+    free(umap);
+    bn_csr_free(&Ui);
+    free(Li.val);
+  }
+  free(lmap);
+  if (!bad && (!pl->ic_lmap || !pl->ic_umap)) {
+    bn_set_error("device memory for the rebuild plan");
+    return BN_ERR_HIP;
+  }
+  return 0;
+}
+
 static int plan_build(struct bn_solver *s) {
   rb_plan *pl = (rb_plan *)calloc(1, sizeof(rb_plan));
   const amg_data *amg = s->amg;
@@ -499,7 +550,12 @@ static int plan_build(struct bn_solver *s) {
     pl->dn_r = (double *)rb_dev(s, nc * nc * sizeof(double));
     if (!pl->dn_c || !pl->dn_r) rc = BN_ERR_HIP;
   }
-  if (!rc && (!pl->work || !pl->iw)) rc = BN_ERR_HIP;
+  if (!rc) {
+    for (int l = 0; l < nlev; l++) pl->l1off[l + 1] = pl->l1off[l] + s->lev[l].n;
+    pl->l1 = (double *)rb_dev(s, (size_t)(pl->l1off[nlev] > 0 ? pl->l1off[nlev] : 1) * sizeof(double));
+  }
+  if (!rc && (!pl->work || !pl->iw || !pl->l1)) rc = BN_ERR_HIP;
+  if (!rc) rc = plan_ic(s, pl);
   if (rc) {
     free(pl);
     if (rc == BN_ERR_HIP) bn_set_error("device memory for the rebuild plan");
@@ -636,11 +692,12 @@ int bn_solver_rebuild_values_device(void *h, int nnz_in, const double *d_val) {
                 rc1->c_col, rc1->c_pos, pl->work, st);
     bn_rb_symmetrize(c->A.nnz, rc1->tpos, pl->work, c->A.vd, st);
   }
-  /* derived per-level arrays */
+  /* derived per-level arrays, and the l1 row norms of the host set-up */
   for (int l = 0; l < nlev; l++) {
     bn_dlevel *d = &s->lev[l];
     bn_rb_to_float(d->A.nnz, d->A.vd, d->A.vs, st);
-    bn_rb_row_scalars(d->n, d->A.ia, d->A.ja, d->A.vd, d->dinv_d, d->dinv_s, d->l1inv_d, d->l1inv_s, NULL, st);
+    bn_rb_row_scalars(d->n, d->A.ia, d->A.ja, d->A.vd, d->dinv_d, d->dinv_s, d->l1inv_d, d->l1inv_s,
+                      pl->l1 + pl->l1off[l], st);
     if (d->use_gs) {
       bn_rb_gather(d->Ac.nnz, pl->lev[l].acmap, d->A.vd, d->Ac.vd, d->Ac.vs, st);
       bn_rb_dinv_map(d->n, d->Ac.ia, d->Ac.ja, d->Ac.vd, d->cmap_d, d->dinvc_d, d->dinvc_s, st);
@@ -650,52 +707,82 @@ int bn_solver_rebuild_values_device(void *h, int nnz_in, const double *d_val) {
       hipMemcpyAsync(s->Asolve.vd, s->lev[0].A.vd, (size_t)nz0 * sizeof(double), hipMemcpyDeviceToDevice, st) !=
           hipSuccess)
     return BN_ERR_HIP;
-  if (hipStreamSynchronize(st) != hipSuccess) {
+  hipEvent_t ev;
+  if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return BN_ERR_HIP;
+  hipError_t e = hipEventRecord(ev, st);
+
+  /* then, on the device, the fine IC(0) (ichol_compute's entries) and the
+   * coarsest inverse (bn_dense_inverse's entries) */
+  bn_dtri *t = &s->fine_ic;
+  if (pl->ic_lmap) {
+    const int nzl = t->lo.nnz;
+    bn_rb_gather(nzl, pl->ic_lmap, s->lev[0].A.vd, t->lo.vd, NULL, st);
+    for (int lv = 0; lv < t->nlev_f; lv++)
+      bn_ic0_rows(t->ptr_f[lv + 1] - t->ptr_f[lv], t->rows_f + t->ptr_f[lv], t->lo.ia, t->lo.ja, t->lo.vd, st);
+    bn_rb_to_float(nzl, t->lo.vd, t->lo.vs, st);
+    bn_rb_gather(t->up.nnz, pl->ic_umap, t->lo.vd, t->up.vd, t->up.vs, st);
+    bn_rb_row_scalars(t->n, t->lo.ia, t->lo.ja, t->lo.vd, t->dinv_d, t->dinv_s, NULL, NULL, NULL, st);
+  }
+  const bn_dlevel *dc = &s->lev[nlev - 1];
+  if (s->coarse_dense &&
+      bn_dn_inverse(dc->n, dc->A.ia, dc->A.ja, dc->A.vd, pl->dn_c, pl->dn_r, s->cinv_d, s->cinv_s, NULL, st))
+    e = hipErrorInvalidValue;
+
+  /* meanwhile the host hierarchy follows: level and interpolation values,
+   * l1 norms */
+  if (e == hipSuccess) e = hipEventSynchronize(ev);
+  hipEventDestroy(ev);
+  if (e != hipSuccess) {
+    hipStreamSynchronize(st);
     bn_set_error("device failure in the value-only rebuild");
     return BN_ERR_HIP;
   }
   tm[2] = rb_ms();
-
-  /* the host hierarchy follows (level and interpolation values, l1 norms) */
-  hipError_t e = hipSuccess;
   for (int l = 0; l < nlev && e == hipSuccess; l++) {
     amg_level *L = &amg->levels[l];
     e = hipMemcpy(L->A.val, s->lev[l].A.vd, (size_t)L->A.nnz * sizeof(double), hipMemcpyDeviceToHost);
     if (e == hipSuccess && l < nlev - 1)
       e = hipMemcpy(L->P.val, s->lev[l].P.vd, (size_t)L->P.nnz * sizeof(double), hipMemcpyDeviceToHost);
-    if (L->l1_diag) bn_csr_l1(&L->A, L->l1_diag);
+    if (e == hipSuccess && L->l1_diag)
+      e = hipMemcpy(L->l1_diag, pl->l1 + pl->l1off[l], (size_t)L->A.row * sizeof(double), hipMemcpyDeviceToHost);
   }
-  if (e != hipSuccess) return BN_ERR_HIP;
+  if (e != hipSuccess) {
+    hipStreamSynchronize(st);
+    return BN_ERR_HIP;
+  }
   memcpy(s->A_own.val, amg->levels[0].A.val, (size_t)nz0 * sizeof(double));
   (void)n0;
   tm[3] = rb_ms();
 
-  /* fine IC(0) and the coarsest solve, as amg_solver_update forms them */
-  ichol_free(&s->L_own);
-  ichol_compute(s->A, &s->L_own);
-  if (s->fine_ic.lo.ia && (rc = refresh_tri(s, &s->L_own, &s->fine_ic))) return rc;
-  tm[4] = rb_ms();
+  /* host-side remainder: the coarsest IC(0) factor (the coarsest solve
+   * when it is not the dense inverse), the fine IC(0) where the device
+   * does not form it */
   amg_level *C = &amg->levels[nlev - 1];
   ichol_free(&C->L_ichol);
   ichol_compute(&C->A, &C->L_ichol);
-  if (s->coarse_dense) {
-    /* the entries of bn_dense_inverse, formed on the device */
-    const bn_dlevel *dc = &s->lev[nlev - 1];
-    if (bn_dn_inverse(dc->n, dc->A.ia, dc->A.ja, dc->A.vd, pl->dn_c, pl->dn_r, s->cinv_d, s->cinv_s, NULL, st) ||
-        hipStreamSynchronize(st) != hipSuccess) {
-      bn_set_error("device failure in the coarsest inverse");
+  if (!pl->ic_lmap) {
+    ichol_free(&s->L_own);
+    ichol_compute(s->A, &s->L_own);
+  }
+  if (hipStreamSynchronize(st) != hipSuccess) {
+    bn_set_error("device failure in the fine IC(0) or the coarsest inverse");
+    return BN_ERR_HIP;
+  }
+  tm[4] = rb_ms();
+  if (pl->ic_lmap) {
+    if (hipMemcpy(s->L_own.val, t->lo.vd, (size_t)t->lo.nnz * sizeof(double), hipMemcpyDeviceToHost) != hipSuccess)
       return BN_ERR_HIP;
-    }
-  } else if ((rc = refresh_tri(s, &C->L_ichol, &s->coarse_ic))) {
+  } else if (t->lo.ia && (rc = refresh_tri(s, &s->L_own, t))) {
     return rc;
   }
+  if (!s->coarse_dense && (rc = refresh_tri(s, &C->L_ichol, &s->coarse_ic))) return rc;
   tm[5] = rb_ms();
   if (log)
     fprintf(stderr,
-            "[rebuild] plan %.1f ms, device levels %.1f ms, host copy of the hierarchy %.1f ms, fine IC(0) %.1f ms, "
-            "coarsest solve (%d rows, %s) %.1f ms\n",
-            tm[1] - tm[0], tm[2] - tm[1], tm[3] - tm[2], tm[4] - tm[3], C->A.row, s->coarse_dense ? "dense" : "IC(0)",
-            tm[5] - tm[4]);
+            "[rebuild] plan %.1f ms, device levels %.1f ms, host copy of the hierarchy %.1f ms, fine IC(0) (%s) and "
+            "coarsest %s (%d rows) %.1f ms more, factor copies %.1f ms\n",
+            tm[1] - tm[0], tm[2] - tm[1], tm[3] - tm[2], pl->ic_lmap ? "device" : "host",
+            s->coarse_dense ? "inverse" : "IC(0)", C->A.row, tm[4] - tm[3], tm[5] - tm[4]);
   return 0;
 }
 

@@ -296,6 +296,11 @@ void bn_rb_pval(int nnzp, const int *prow, const int *cpt, const int *pia, const
 void bn_rb_rap_pull(int nnz, const int *toff, const int *t1, const int *t2, const int *t3, int nlong,
                     const int *elist, const double *rv, const double *av, const double *pv, double *out,
                     hipStream_t st);
+/* IC(0) of the rows rows[0..cnt) of a lower factor (strictly ascending
+ * columns, diagonal last), in place on lv holding the operator's lower
+ * entries; every row a row depends on must be final (one schedule level
+ * per call) */
+void bn_ic0_rows(int cnt, const int *rows, const int *lia, const int *lja, double *lv, hipStream_t st);
 /* Dense inverse of an SPD CSR matrix of n <= 8192 rows on the device,
  * entry for entry as dense.c's bn_dense_inverse forms it: Lc and Lr are
  * n x n scratch (the factor column-major and its row-major copy), inv the

@@ -246,6 +246,37 @@ __global__ void __launch_bounds__(BLOCK) k_rb_rap_pull_wave(int nlong, const int
 }
 
 
+// ---- IC(0) of the fine operator (ichol_compute, setup.c, entry for
+// entry): rows of one level of the lower factor's schedule, one thread per
+// row.  Rows are strictly ascending with the diagonal last (checked when
+// the plan is built), so "column j is in row i" of the host's column
+// stamps is a merge of the two sorted rows, and the row's own values
+// serve as the host's work row. ----
+__global__ void k_ic0_rows(int cnt, const int *__restrict__ rows, const int *__restrict__ lia,
+                           const int *__restrict__ lja, double *__restrict__ lv) {
+  const int x = blockIdx.x * BLOCK + threadIdx.x;
+  if (x >= cnt) return;
+  const int i = rows[x], lo = lia[i], hi = lia[i + 1];
+  for (int q = lo; q < hi; q++) {
+    const int k = lja[q];
+    if (k >= i) continue;
+    double v = lv[q];
+    int at = lo;
+    const int k1 = lia[k + 1];
+    for (int t = lia[k]; t < k1; t++) {
+      const int j = lja[t];
+      if (j >= k) break;
+      while (at < hi && lja[at] < j) at++;
+      if (at < hi && lja[at] == j) v -= lv[at] * lv[t];
+    }
+    lv[q] = v / lv[k1 - 1];
+  }
+  double d = lv[hi - 1];
+  for (int q = lo; q < hi - 1; q++) d -= lv[q] * lv[q];
+  if (d <= 0.0) d = 1e-14;
+  lv[hi - 1] = sqrt(d);
+}
+
 // ---- dense inverse of the coarsest operator (dense.c's bn_dense_inverse,
 // entry for entry) ----
 // The factor is kept column-major (Lc: L(i,p) at Lc[p n + i]) for the
@@ -533,6 +564,10 @@ void bn_rb_rap(int nc, const int *ria, const int *rja, const double *rv, const i
   if (nc > 0)
     hipLaunchKernelGGL(k_rb_rap, g1(nc), dim3(BLOCK), 0, st, nc, ria, rja, rv, aia, aja, av, pia, pja, pv, cia,
                        ccol, cpos, out);
+}
+
+void bn_ic0_rows(int cnt, const int *rows, const int *lia, const int *lja, double *lv, hipStream_t st) {
+  if (cnt > 0) hipLaunchKernelGGL(k_ic0_rows, g1(cnt), dim3(BLOCK), 0, st, cnt, rows, lia, lja, lv);
 }
 
 int bn_dn_inverse(int n, const int *ia, const int *ja, const double *a, double *Lc, double *Lr, double *inv,
