@@ -57,7 +57,7 @@ class Options(C.Structure):
                 ("max_colours", C.c_int), ("flexible", C.c_int),
                 ("stop", C.c_int), ("symmetrize", C.c_int), ("device", C.c_int),
                 ("verbose", C.c_int), ("graph", C.c_int),
-                ("block_smoother", C.c_int), ("chunk_rows", C.c_int)]
+                ("block_smoother", C.c_int), ("chunk_rows", C.c_int), ("strength", C.c_int)]
 
 
 class Nwk(C.Structure):
@@ -111,6 +111,7 @@ IC_EXACT, IC_JACOBI, IC_NONE = 0, 1, 2
 # lanes-argument bits of the SpMV launchers: 8 or 16 load rounds per lane
 LANES_DEEP8, LANES_DEEP16 = 0x100, 0x200
 STOP_BNORM, STOP_RELRES = 0, 1
+STRENGTH_C, STRENGTH_MEX = 0, 1
 
 
 def _load():

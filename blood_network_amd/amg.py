@@ -50,7 +50,9 @@ class AMGSolver:
     "hmis"), ``precision`` ("fp64" | "fp32"), ``coarse`` ("auto" | "dense"
     | "icpcg"), ``fine_ic`` ("exact" | "jacobi"), ``flexible``, ``stop``
     ("bnorm" | "relres"), ``smoother``, ``block_smoother`` (chunked l1 block
-    Gauss-Seidel on every level, one launch per sweep) with ``chunk_rows``.
+    Gauss-Seidel on every level, one launch per sweep) with ``chunk_rows``,
+    ``strength`` ("c": C_amg1's rule | "mex": the MATLAB MEX rule, which
+    clamps -a_ij at 0).
     """
 
     _COARSEN = {"rs": L.COARSEN_RS, "pmis": L.COARSEN_PMIS, "hmis": L.COARSEN_HMIS}
@@ -62,13 +64,14 @@ class AMGSolver:
     def __init__(self, ia, ja, val, threshold=0.25, min_size=4096, max_levels=20, nu=1, *,
                  coarsening=None, precision="fp64", coarse="auto", fine_ic="exact",
                  ic_sweeps=3, flexible=None, stop="bnorm", smoother="cf_gs", max_colours=32,
-                 symmetrize=True, verbose=False, graph=True, block_smoother=False, chunk_rows=0):
+                 symmetrize=True, verbose=False, graph=True, block_smoother=False, chunk_rows=0,
+                 strength="c"):
         self.ia, self.ja, self.val = _csr(ia, ja, val)
         self.n = len(self.ia) - 1
         plain = (coarsening is None and precision == "fp64" and coarse == "auto"
                  and fine_ic == "exact" and stop == "bnorm" and smoother == "cf_gs"
                  and flexible is None and symmetrize and not verbose and max_colours == 32
-                 and graph and not block_smoother)
+                 and graph and not block_smoother and strength == "c")
         if plain:
             h = L.lib.amg_solver_create(self.n, len(self.ja), L.ip(self.ia), L.ip(self.ja),
                                         L.dp(self.val), threshold, min_size, max_levels, nu)
@@ -96,6 +99,7 @@ class AMGSolver:
             o.block_smoother = int(bool(block_smoother))
             if chunk_rows:
                 o.chunk_rows = int(chunk_rows)
+            o.strength = {"c": L.STRENGTH_C, "mex": L.STRENGTH_MEX}[strength]
             h = L.lib.bn_solver_create(self.n, len(self.ja), L.ip(self.ia), L.ip(self.ja),
                                        L.dp(self.val), C.byref(o))
         if not h:

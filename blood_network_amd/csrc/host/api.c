@@ -208,7 +208,7 @@ void *bn_solver_create(int n, int nnz, const int *ia, const int *ja, const doubl
   a->smoother = s->opt.smoother;
   a->omega_jacobi = s->opt.omega;
   a->coarsening = s->opt.coarsening;
-  amg_build_hierarchy(a, s->A);
+  bn_build_hierarchy(a, s->A, s->opt.strength);
   ichol_compute(s->A, &s->L_own);
   s->last.setup_ms = now_ms() - t0;
   s->last.num_levels = a->num_levels;

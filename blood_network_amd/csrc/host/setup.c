@@ -17,6 +17,7 @@
 #endif
 
 #include "amg.h"
+#include "bn_amg_ext.h"
 #include "csr.h"
 
 void bn_cache_drop(const void *p);
@@ -37,7 +38,20 @@ static inline int thread_id(void) {
  * S keeps A's values and order.  max_lam = longest A row (>= 1).
  * C_amg1/src/rs_classical.c:30-88.
  * ------------------------------------------------------------------- */
+static void strength_rule(const dcsr_mat *A, REAL theta, int clamp, dcsr_mat *S, INT *max_lam);
+
 void rs_strength(const dcsr_mat *A, REAL theta, dcsr_mat *S, INT *max_lam) {
+  strength_rule(A, theta, 0, S, max_lam);
+}
+
+/* clamp = 1: the MATLAB MEX rule, max(0, -a_ij) >= theta * m_i
+ * (Matlab_amg/rs_strength_mex.c:58-60,79-81) */
+static inline int is_strong(REAL a, REAL t, int clamp) {
+  const REAL v = -a;
+  return clamp ? ((v < 0.0 ? 0.0 : v) >= t) : (v >= t);
+}
+
+static void strength_rule(const dcsr_mat *A, REAL theta, int clamp, dcsr_mat *S, INT *max_lam) {
   const INT n = A->row;
   REAL *thr = (REAL *)bn_xmalloc((size_t)n * sizeof(REAL));
   INT *cnt = (INT *)bn_xmalloc((size_t)n * sizeof(INT));
@@ -55,7 +69,7 @@ void rs_strength(const dcsr_mat *A, REAL theta, dcsr_mat *S, INT *max_lam) {
     REAL t = theta * mx;
     INT c = 0;
     for (INT k = A->ia[i]; k < A->ia[i + 1]; k++)
-      if (A->ja[k] != i && -A->val[k] >= t) c++;
+      if (A->ja[k] != i && is_strong(A->val[k], t, clamp)) c++;
     thr[i] = t;
     cnt[i] = c;
   }
@@ -71,7 +85,7 @@ void rs_strength(const dcsr_mat *A, REAL theta, dcsr_mat *S, INT *max_lam) {
   for (INT i = 0; i < n; i++) {
     INT p = S->ia[i];
     for (INT k = A->ia[i]; k < A->ia[i + 1]; k++)
-      if (A->ja[k] != i && -A->val[k] >= thr[i]) {
+      if (A->ja[k] != i && is_strong(A->val[k], thr[i], clamp)) {
         S->ja[p] = A->ja[k];
         S->val[p] = A->val[k];
         p++;
@@ -620,7 +634,10 @@ static void galerkin_level(const amg_level *cur, dcsr_mat *Ac) {
   bn_csr_free(&rap);
 }
 
-void amg_build_hierarchy(amg_data *amg, const dcsr_mat *A) {
+void amg_build_hierarchy(amg_data *amg, const dcsr_mat *A) { bn_build_hierarchy(amg, A, 0); }
+
+/* amg_build_hierarchy with a choice of strength rule (BN_STRENGTH_*) */
+void bn_build_hierarchy(amg_data *amg, const dcsr_mat *A, int strength) {
   INT max_levels = amg->max_levels;
   if (max_levels > AMG_MAX_LEVELS) max_levels = AMG_MAX_LEVELS;
   amg_level *l0 = &amg->levels[0];
@@ -634,7 +651,7 @@ void amg_build_hierarchy(amg_data *amg, const dcsr_mat *A) {
     amg_level *cur = &amg->levels[nlev - 1];
     const INT nf = cur->A.row;
     INT ml = 0, nc = 0;
-    rs_strength(&cur->A, amg->threshold, &cur->A_filtered, &ml);
+    strength_rule(&cur->A, amg->threshold, strength == 1, &cur->A_filtered, &ml);
     INT *cf = (INT *)bn_xcalloc((size_t)nf, sizeof(INT));
     if (amg->coarsening == COARSEN_PMIS) pmis_coarsening(&cur->A_filtered, cf, &nc);
     else if (amg->coarsening == COARSEN_HMIS) hmis_coarsening(&cur->A_filtered, cf, &nc);

@@ -39,6 +39,19 @@ extern "C" {
 #define BN_IC_NONE 2    /* no fine IC(0) step: the PCG preconditioner is     */
                         /* the plain V-cycle (amg_vcycle_precond)           */
 
+/* strength test of the set-up: j strong in row i (j != i) iff
+ *   C   (rs_classical.c:38-58):   -a_ij        >= theta * m_i
+ *   MEX (rs_strength_mex.c:58-60,79-81): max(0, -a_ij) >= theta * m_i
+ * with m_i = max(0, max_{j != i} -a_ij).  They differ only on positive
+ * off-diagonals of rows whose off-diagonals are all >= 0 (m_i = 0): the
+ * MEX rule makes every off-diagonal of such a row strong. */
+#define BN_STRENGTH_C 0
+#define BN_STRENGTH_MEX 1
+
+/* amg_build_hierarchy (C_amg1/src/build_hierarchy.c:29-156) with a choice
+ * of strength rule; amg_build_hierarchy is strength = BN_STRENGTH_C. */
+void bn_build_hierarchy(amg_data *amg, const dcsr_mat *A, int strength);
+
 /* outer stopping test */
 #define BN_STOP_BNORM 0  /* sqrt(r'z / r0'z0) < tol (reference, amg_pcg.c:75) */
 #define BN_STOP_RELRES 1 /* ||r||_2 / ||b||_2 < tol                          */
@@ -67,6 +80,7 @@ typedef struct {
                       /*    of multicolour GS / l1-Jacobi                 */
   int chunk_rows;     /* largest chunk of the block smoother (<= 8192;    */
                       /*    0: 8192)                                      */
+  int strength;       /* BN_STRENGTH_*                                    */
 } bn_options;
 
 typedef struct {

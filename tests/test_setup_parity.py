@@ -111,3 +111,67 @@ def test_hmis_on_hub_level_fixture(oracle):
         assert nc.value == int(d["mis"].sum())
     assert np.array_equal(out["product"], out["oracle"])
     assert np.array_equal(out["product"], d["mis"])
+
+
+def _strong_sets(S):
+    ia, ja, _ = S
+    return [set(ja[ia[i]:ia[i + 1]].tolist()) for i in range(len(ia) - 1)]
+
+
+def test_mex_strength_rule():
+    """The MATLAB MEX strength rule, max(0, -a_ij) >= theta m_i
+    (Matlab_amg/rs_strength_mex.c:58-60,79-81), against C_amg1's
+    -a_ij >= theta m_i, with m_i = max(0, max_j -a_ij):
+    - a row whose off-diagonals are all >= 0 has m_i = 0: the MEX rule makes
+      every off-diagonal strong, the C rule only its stored zeros;
+    - a row with a negative off-diagonal: a positive entry is strong under
+      neither rule, and the negative ones are strong under both alike;
+    - on a graph Laplacian (no positive off-diagonal) the two hierarchies
+      are identical."""
+    import scipy.sparse as sp
+    n = 400
+    rng = np.random.default_rng(5)
+    A = random_graph_laplacian(n, 5, seed=3).tolil()
+    pos_rows = [7, 50, 123]
+    for i in pos_rows:
+        for j in A.rows[i][:]:
+            if j != i:
+                A[i, j] = abs(A[i, j]) + 0.5  # all off-diagonals of row i positive
+                A[j, i] = A[i, j]
+    A[200, 201] = 0.75                       # one positive entry in an ordinary row
+    A[201, 200] = 0.75
+    A = sp.csr_matrix(A)
+    A.setdiag(np.abs(A).sum(axis=1).A1 + 1.0)
+    A.sort_indices()
+    mex = AMGSolver(A.indptr, A.indices, A.data, coarsening="pmis", min_size=50, strength="mex")
+    cee = AMGSolver(A.indptr, A.indices, A.data, coarsening="pmis", min_size=50, strength="c")
+    Sm, Sc = _strong_sets(mex.level(0)["S"]), _strong_sets(cee.level(0)["S"])
+    for i in range(n):
+        row = A.indices[A.indptr[i]:A.indptr[i + 1]]
+        vals = A.data[A.indptr[i]:A.indptr[i + 1]]
+        off = {int(j): v for j, v in zip(row, vals) if j != i}
+        if all(v >= 0 for v in off.values()):
+            assert Sm[i] == set(off), i
+            assert Sc[i] == {j for j, v in off.items() if v == 0.0}, i
+        else:
+            assert Sm[i] == Sc[i], i
+            assert all(off[j] < 0 for j in Sc[i]), i
+    assert any(Sm[i] != Sc[i] for i in pos_rows)
+    assert 201 not in Sm[200] and 201 not in Sc[200]
+
+    L = random_graph_laplacian(n, 5, seed=4)
+    a = AMGSolver(L.indptr, L.indices, L.data, coarsening="hmis", min_size=50, strength="mex")
+    b = AMGSolver(L.indptr, L.indices, L.data, coarsening="hmis", min_size=50, strength="c")
+    compare_solvers(a, b)
+
+
+def compare_solvers(s, t):
+    assert s.num_levels() == t.num_levels()
+    for lev in range(s.num_levels()):
+        a, b = s.level(lev), t.level(lev)
+        for k in b:
+            if isinstance(b[k], tuple):
+                for p, q in zip(a[k], b[k]):
+                    assert np.array_equal(p, q), (lev, k)
+            else:
+                assert np.array_equal(a[k], b[k]), (lev, k)
