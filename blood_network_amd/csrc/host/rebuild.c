@@ -660,12 +660,12 @@ int bn_solver_rebuild_values_device(void *h, int nnz_in, const double *d_val) {
       if (hipMemcpyAsync(s->lev[0].A.vd, in, (size_t)nz0 * sizeof(double), hipMemcpyDeviceToDevice, st) !=
           hipSuccess)
         return BN_ERR_HIP;
-      bn_rb_symmetrize(nz0, pl->tpos0, s->lev[0].A.vd, pl->work, st);
+      bn_rb_symmetrize(nz0, pl->tpos0, s->lev[0].A.vd, pl->work, 0, st);
       if (hipMemcpyAsync(s->lev[0].A.vd, pl->work, (size_t)nz0 * sizeof(double), hipMemcpyDeviceToDevice, st) !=
           hipSuccess)
         return BN_ERR_HIP;
     } else {
-      bn_rb_symmetrize(nz0, pl->tpos0, in, s->lev[0].A.vd, st);
+      bn_rb_symmetrize(nz0, pl->tpos0, in, s->lev[0].A.vd, 0, st);
     }
   } else if (hipMemcpyAsync(s->lev[0].A.vd, in, (size_t)nz0 * sizeof(double), hipMemcpyDeviceToDevice, st) !=
              hipSuccess) {
@@ -690,7 +690,7 @@ int bn_solver_rebuild_values_device(void *h, int nnz_in, const double *d_val) {
     else
       bn_rb_rap(c->n, d->R.ia, d->R.ja, d->R.vd, d->A.ia, d->A.ja, d->A.vd, d->P.ia, d->P.ja, d->P.vd, c->A.ia,
                 rc1->c_col, rc1->c_pos, pl->work, st);
-    bn_rb_symmetrize(c->A.nnz, rc1->tpos, pl->work, c->A.vd, st);
+    bn_rb_symmetrize(c->A.nnz, rc1->tpos, pl->work, c->A.vd, 1, st);
   }
   /* derived per-level arrays, and the l1 row norms of the host set-up */
   for (int l = 0; l < nlev; l++) {

@@ -625,9 +625,19 @@ void ichol_free(ichol_factor *L) {
 /* ---------------------------------------------------------------------
  * Hierarchy (C_amg1/src/build_hierarchy.c:29-260)
  * ------------------------------------------------------------------- */
+/* the same product on the device (galerkin.hip), identical entry for
+ * entry; absent from builds without the device code, and it declines (-1)
+ * when no device is present */
+int bn_galerkin_device(const dcsr_mat *A, const dcsr_mat *P, const dcsr_mat *R, dcsr_mat *Ac)
+    __attribute__((weak));
+
 static void galerkin_level(const amg_level *cur, dcsr_mat *Ac) {
   dcsr_mat Pt, rap;
   bn_csr_transpose(&cur->P, &Pt);
+  if (bn_galerkin_device && bn_galerkin_device(&cur->A, &cur->P, &Pt, Ac) == 0) {
+    bn_csr_free(&Pt);
+    return;
+  }
   dcsr_rap(&Pt, (dcsr_mat *)&cur->A, (dcsr_mat *)&cur->P, &rap);
   bn_csr_free(&Pt);
   bn_csr_symmetrize(&rap, Ac);
@@ -636,8 +646,12 @@ static void galerkin_level(const amg_level *cur, dcsr_mat *Ac) {
 
 void amg_build_hierarchy(amg_data *amg, const dcsr_mat *A) { bn_build_hierarchy(amg, A, 0); }
 
+static double setup_ms(void) { return 1e3 * omp_get_wtime(); }
+
 /* amg_build_hierarchy with a choice of strength rule (BN_STRENGTH_*) */
 void bn_build_hierarchy(amg_data *amg, const dcsr_mat *A, int strength) {
+  const char *lg = getenv("BN_SETUP_LOG");
+  const int log = lg && *lg && *lg != '0';
   INT max_levels = amg->max_levels;
   if (max_levels > AMG_MAX_LEVELS) max_levels = AMG_MAX_LEVELS;
   amg_level *l0 = &amg->levels[0];
@@ -651,11 +665,15 @@ void bn_build_hierarchy(amg_data *amg, const dcsr_mat *A, int strength) {
     amg_level *cur = &amg->levels[nlev - 1];
     const INT nf = cur->A.row;
     INT ml = 0, nc = 0;
+    double t[5];
+    t[0] = setup_ms();
     strength_rule(&cur->A, amg->threshold, strength == 1, &cur->A_filtered, &ml);
+    t[1] = setup_ms();
     INT *cf = (INT *)bn_xcalloc((size_t)nf, sizeof(INT));
     if (amg->coarsening == COARSEN_PMIS) pmis_coarsening(&cur->A_filtered, cf, &nc);
     else if (amg->coarsening == COARSEN_HMIS) hmis_coarsening(&cur->A_filtered, cf, &nc);
     else rs_coarsening(&cur->A_filtered, ml, cf, &nc);
+    t[2] = setup_ms();
     cur->n_coarse = nc;
     if (nc == 0 || nc >= nf) {
       /* stalled (build_hierarchy.c:78-83): this level becomes coarsest */
@@ -675,9 +693,15 @@ void bn_build_hierarchy(amg_data *amg, const dcsr_mat *A, int strength) {
       if (cf[i] != CPT) cur->cf_order[q++] = i;
     rs_standard_interpolation(&cur->A, &cur->A_filtered, cf, &cur->P);
     free(cf);
+    t[3] = setup_ms();
     amg_level *nx = &amg->levels[nlev];
     memset(nx, 0, sizeof(*nx));
     galerkin_level(cur, &nx->A);
+    t[4] = setup_ms();
+    if (log)
+      fprintf(stderr, "[setup] level %d (%d rows, %d nnz): strength %.1f ms, split %.1f ms, interpolation %.1f ms, "
+              "Galerkin %.1f ms -> %d rows, %d nnz\n", nlev - 1, nf, cur->A.nnz, t[1] - t[0], t[2] - t[1],
+              t[3] - t[2], t[4] - t[3], nc, nx->A.nnz);
     nx->n = nc;
     nx->l1_diag = (REAL *)bn_xmalloc((size_t)nc * sizeof(REAL));
     bn_csr_l1(&nx->A, nx->l1_diag);

@@ -273,7 +273,9 @@ void bn_finish_seg(int nsys, int which, const int *item_ptr, const double *p1, c
 void bn_interleave(int n, int k, int K, const double *src, double *dst, int back, hipStream_t stream);
 
 /* ---- value-only rebuild of a hierarchy (rebuild.hip) ---- */
-void bn_rb_symmetrize(int nnz, const int *tpos, const double *v, double *out, hipStream_t st);
+/* (v + v^T) / 2 through transpose positions; diag0 = 1 for Galerkin sums
+ * formed from -0.0 (a zero diagonal is taken as +0.0) */
+void bn_rb_symmetrize(int nnz, const int *tpos, const double *v, double *out, int diag0, hipStream_t st);
 void bn_rb_gather(int n, const int *map, const double *src, double *out, float *outs, hipStream_t st);
 void bn_rb_to_float(int n, const double *src, float *out, hipStream_t st);
 void bn_rb_row_scalars(int n, const int *ia, const int *ja, const double *a, double *dinv, float *dinvs,

@@ -39,12 +39,22 @@ __device__ __forceinline__ int find_col(const int *__restrict__ cols, int lo, in
 }
 
 // out[q] = (v[q] + v[tpos[q]]) * 0.5  (symmetrisation of a structurally
-// symmetric matrix: tpos[q] holds the position of the transposed entry)
+// symmetric matrix: tpos[q] holds the position of the transposed entry).
+// diag0: v holds Galerkin sums that start from -0.0, so that an
+// off-diagonal entry is its first term exactly, as the host product stores
+// it; the host starts the diagonal from +0.0 instead, which differs only
+// in the sign of a zero sum: such a diagonal is taken as +0.0.
 __global__ void k_rb_symmetrize(int nnz, const int *__restrict__ tpos, const double *__restrict__ v,
-                                double *__restrict__ out) {
+                                double *__restrict__ out, int diag0) {
   const int q = blockIdx.x * BLOCK + threadIdx.x;
   if (q >= nnz) return;
-  out[q] = (v[q] + v[tpos[q]]) * 0.5;
+  const int t = tpos[q];
+  if (diag0 && t == q) {
+    const double d = v[q] == 0.0 ? 0.0 : v[q];
+    out[q] = (d + d) * 0.5;
+    return;
+  }
+  out[q] = (v[q] + v[t]) * 0.5;
 }
 
 // out[q] = src[map[q]]
@@ -167,8 +177,9 @@ __global__ void k_rb_pval(int nnzp, const int *__restrict__ prow, const int *__r
 
 // Galerkin product values on the fixed pattern of the coarse operator, one
 // thread per coarse row, in the order of the host's row-by-row product
-// (R row, then A row, then P row).  ccol/cpos: the coarse row's columns
-// sorted, with their positions in the stored row.
+// (R row, then A row, then P row), from -0.0 (see k_rb_symmetrize).
+// ccol/cpos: the coarse row's columns sorted, with their positions in the
+// stored row.
 __global__ void k_rb_rap(int nc, const int *__restrict__ ria, const int *__restrict__ rja,
                          const double *__restrict__ rv, const int *__restrict__ aia, const int *__restrict__ aja,
                          const double *__restrict__ av, const int *__restrict__ pia, const int *__restrict__ pja,
@@ -177,7 +188,7 @@ __global__ void k_rb_rap(int nc, const int *__restrict__ ria, const int *__restr
   const int ic = blockIdx.x * BLOCK + threadIdx.x;
   if (ic >= nc) return;
   const int c0 = cia[ic], c1 = cia[ic + 1];
-  for (int q = c0; q < c1; q++) out[q] = 0.0;
+  for (int q = c0; q < c1; q++) out[q] = -0.0;
   for (int q1 = ria[ic]; q1 < ria[ic + 1]; q1++) {
     const int i1 = rja[q1];
     const double r = rv[q1];
@@ -196,7 +207,8 @@ __global__ void k_rb_rap(int nc, const int *__restrict__ ria, const int *__restr
 // Galerkin product values from per-entry lists: entry e of the coarse
 // operator (stored order) is the sum over its triples t in toff[e] ..
 // toff[e+1], in the host product's (R row, A row, P row) order, of
-// (r a) p.  Lists longer than BN_RB_LONG go to k_rb_rap_pull_wave.
+// (r a) p, from -0.0 (see k_rb_symmetrize).  Lists longer than BN_RB_LONG
+// go to k_rb_rap_pull_wave.
 __global__ void k_rb_rap_pull(int nnz, const int *__restrict__ toff, const int *__restrict__ t1,
                               const int *__restrict__ t2, const int *__restrict__ t3, const double *__restrict__ rv,
                               const double *__restrict__ av, const double *__restrict__ pv, double *__restrict__ out) {
@@ -204,7 +216,7 @@ __global__ void k_rb_rap_pull(int nnz, const int *__restrict__ toff, const int *
   if (e >= nnz) return;
   const int lo = toff[e], hi = toff[e + 1];
   if (hi - lo > BN_RB_LONG) return;
-  double acc = 0.0;
+  double acc = -0.0;
   for (int t = lo; t < hi; t++) {
     const double ra = rv[t1[t]] * av[t2[t]];
     acc += ra * pv[t3[t]];
@@ -231,7 +243,7 @@ __global__ void __launch_bounds__(BLOCK) k_rb_rap_pull_wave(int nlong, const int
   if (w >= nlong) return;
   const int e = __builtin_amdgcn_readfirstlane(elist[w]);
   const int lo = __builtin_amdgcn_readfirstlane(toff[e]), hi = __builtin_amdgcn_readfirstlane(toff[e + 1]);
-  double acc = 0.0;
+  double acc = -0.0;
   for (int base = lo; base < hi; base += 64) {
     const int t = base + lane;
     double c = 0.0;
@@ -509,8 +521,8 @@ __global__ void k_dn_to_float(long long m, const double *__restrict__ src, float
 
 extern "C" {
 
-void bn_rb_symmetrize(int nnz, const int *tpos, const double *v, double *out, hipStream_t st) {
-  if (nnz > 0) hipLaunchKernelGGL(k_rb_symmetrize, g1(nnz), dim3(BLOCK), 0, st, nnz, tpos, v, out);
+void bn_rb_symmetrize(int nnz, const int *tpos, const double *v, double *out, int diag0, hipStream_t st) {
+  if (nnz > 0) hipLaunchKernelGGL(k_rb_symmetrize, g1(nnz), dim3(BLOCK), 0, st, nnz, tpos, v, out, diag0);
 }
 
 void bn_rb_gather(int n, const int *map, const double *src, double *out, float *outs, hipStream_t st) {
