@@ -379,12 +379,19 @@ extern "C" {
 /* A_c = (R A P + (R A P)^T) / 2 of the set-up (setup.c's galerkin_level)
  * on the device, identical to the host product; R = P^T as the host
  * transposes it.  Returns 0 with Ac allocated on the host, or -1 (no
- * device, BN_HOST_GALERKIN set, sizes beyond int32 items, a pattern that
- * is not symmetric, a device failure): the caller then forms it on the
- * host. */
+ * device, BN_HOST_GALERKIN set, fewer than BN_DEVICE_GALERKIN_MIN operator
+ * entries, sizes beyond int32 items, a pattern that is not symmetric, a
+ * device failure): the caller then forms it on the host. */
 int bn_galerkin_device(const dcsr_mat *A, const dcsr_mat *P, const dcsr_mat *R, dcsr_mat *Ac) {
   const char *env = getenv("BN_HOST_GALERKIN");
   if ((env && *env && *env != '0') || !bn_device_ok()) return -1;
+  /* below ~4M operator entries the device's fixed cost (allocation, sort
+   * set-up, copies: ~25 ms) exceeds the host product's time
+   * (BN_DEVICE_GALERKIN_MIN overrides) */
+  long long min_nnz = 4000000;
+  const char *mn = getenv("BN_DEVICE_GALERKIN_MIN");
+  if (mn && *mn) min_nnz = atoll(mn);
+  if (A->nnz < min_nnz) return -1;
   const int rc = galerkin(A, P, R, Ac);
   if (rc) (void)hipGetLastError();
   return rc;

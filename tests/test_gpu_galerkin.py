@@ -36,6 +36,9 @@ def same(a, b):
 
 
 def build(ia, ja, val, host, monkeypatch, **kw):
+    """host: the product on the host; otherwise on the device at every
+    level (BN_DEVICE_GALERKIN_MIN=0 lifts the size floor)"""
+    monkeypatch.setenv("BN_DEVICE_GALERKIN_MIN", "0")
     if host:
         monkeypatch.setenv("BN_HOST_GALERKIN", "1")
     else:
