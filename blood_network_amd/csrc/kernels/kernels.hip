@@ -79,24 +79,33 @@ __device__ __forceinline__ T row_dot(int q, const int e, const int *__restrict__
     }
 #pragma unroll
     for (int u = 0; u < R; u++) {
-      const T xv = (SKIP_DIAG && j[u] == self) ? T(0) : x[j[u]];
-      acc[u & 3] += a[u] * xv;
+      // select the product, not the gathered value: x[self] may be stale
+      // (a triangular solve's own unknown) and is never used
+      const T p = a[u] * x[j[u]];
+      acc[u & 3] += (SKIP_DIAG && j[u] == self) ? T(0) : p;
     }
   }
   if (q < e) {
+    // the tail loads every slot unconditionally (a slot past the row end
+    // re-reads the row's last entry and is then zeroed), so all its loads
+    // and gathers are in flight at once instead of one branch at a time
     int j[R - 1];
     T a[R - 1];
 #pragma unroll
     for (int u = 0; u < R - 1; u++) {
-      const bool h = q + u * LANES < e;
-      j[u] = h ? ja[q + u * LANES] : self;
-      a[u] = h ? val[q + u * LANES] : T(0);
+      const int k = q + u * LANES < e ? q + u * LANES : e - 1;
+      j[u] = ja[k];
+      a[u] = val[k];
     }
 #pragma unroll
     for (int u = 0; u < R - 1; u++) {
-      const bool h = q + u * LANES < e;
-      const T xv = (!h || (SKIP_DIAG && j[u] == self)) ? T(0) : x[j[u]];
-      acc[u & 3] += a[u] * xv;
+      // a padding slot's coefficient is scaled to 0 (x there is a real
+      // neighbour's value, finite); only x[self], which may be stale, is
+      // selected away - on the product, since a load feeding a select
+      // becomes a branch around the load
+      const T hm = q + u * LANES < e ? T(1) : T(0);
+      const T p = (a[u] * hm) * x[j[u]];
+      acc[u & 3] += (SKIP_DIAG && j[u] == self) ? T(0) : p;
     }
   }
   return (acc[0] + acc[1]) + (acc[2] + acc[3]);
@@ -162,13 +171,15 @@ __global__ __launch_bounds__(BLOCK) void k_csr(int n, int chunk, const int *__re
     if (row < n) {
       const int q0 = ia[row], q1 = ia[row + 1];
       if (q1 - q0 > long_len) continue; /* handled by k_csr_long */
+      // b / y ahead of the row's gathers: one memory round trip fewer
+      const T base = OP == BN_OP_RESID ? b[row] : OP == BN_OP_ADD ? y[row] : T(0);
       T s = row_dot<T, LANES, false, R>(q0 + sub, q1, ja, val, x, row);
       if (LANES > 1) s = group_sum<LANES>(s);
       if (sub == 0) {
         T out;
         if (OP == BN_OP_SET) out = s;
-        else if (OP == BN_OP_RESID) out = b[row] - s;
-        else out = y[row] + s;
+        else if (OP == BN_OP_RESID) out = base - s;
+        else out = base + s;
         y[row] = out;
         if (DOT) acc += (double)w[row] * (double)out;
       }
@@ -438,10 +449,12 @@ __global__ __launch_bounds__(BLOCK) void k_rowsolve(int lo, int cnt, const int *
   if (k >= cnt) return;
   const int r = MODE == RS_LIST ? rows[k] : lo + k;
   const int i = MODE == RS_MAP ? rows[r] : r;
+  // b_i and dinv_r ahead of the row's gathers: one memory round trip fewer
+  const T bi = b[i], dr = dinv[r];
   T s = row_dot<T, LANES, true>(ia[r] + sub, ia[r + 1], ja, val, y, i);
   if (LANES > 1) s = group_sum<LANES>(s);
   if (sub == 0) {
-    const T v = (b[i] - s) * dinv[r];
+    const T v = (bi - s) * dr;
     y[i] = v;
     if (zsum) zsum[i] = zbase[i] + v;
   }
@@ -584,10 +597,11 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_rows(int n, const int *__restr
   const int i = logical_block() * RPB + threadIdx.x / LANES;
   const int sub = threadIdx.x % LANES;
   if (i >= n) return;
+  const T bi = b[i], di = dinv[i];
   T s = row_dot<T, LANES, true>(ia[i] + sub, ia[i + 1], ja, val, xin, i);
   if (LANES > 1) s = group_sum<LANES>(s);
   if (sub == 0) {
-    const T v = (b[i] - s) * dinv[i];
+    const T v = (bi - s) * di;
     if (zsum) zsum[i] = zbase[i] + v;
     else xout[i] = v;
   }
