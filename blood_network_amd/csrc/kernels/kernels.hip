@@ -130,7 +130,23 @@ __device__ __forceinline__ T row_dot_deep(int q, const int e, const int *__restr
 #pragma unroll
     for (int u = 0; u < U; u++) acc[u & 3] += a[u] * x[j[u]];
   }
-  for (; q < e; q += BLOCK) acc[0] += val[q] * x[ja[q]];
+  if (q < e) {
+    // at most U - 1 strides left: one round, padding slots re-read the
+    // row's last entry with a zero coefficient (no branch per load)
+    int j[U - 1];
+    T a[U - 1];
+#pragma unroll
+    for (int u = 0; u < U - 1; u++) {
+      const int k = q + u * BLOCK < e ? q + u * BLOCK : e - 1;
+      j[u] = ja[k];
+      a[u] = val[k];
+    }
+#pragma unroll
+    for (int u = 0; u < U - 1; u++) {
+      const T hm = q + u * BLOCK < e ? T(1) : T(0);
+      acc[u & 3] += (a[u] * hm) * x[j[u]];
+    }
+  }
   return (acc[0] + acc[1]) + (acc[2] + acc[3]);
 }
 
@@ -360,7 +376,7 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__res
                                                       const T *__restrict__ val, const T *__restrict__ x,
                                                       T *__restrict__ y, const T *__restrict__ b) {
   constexpr int PER = BN_ROWBLOCK_NNZ / BLOCK;
-  __shared__ T prod[BN_ROWBLOCK_NNZ];
+  __shared__ T prod[BN_ROWBLOCK_NNZ + 1];  // + a spare word padding slots write
   const int bid = logical_block();
   if (bid >= nblk) return;
   const int r0 = blocks[bid], r1 = blocks[bid + 1];
@@ -377,18 +393,26 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__res
     }
     return;
   }
-  int jj[PER];
-  T aa[PER];
+  if (cnt > 0) {
+    // every slot loads (past the block's entries: its last entry, with a
+    // zero coefficient), so all loads and gathers are in flight together
+    int jj[PER];
+    T aa[PER];
 #pragma unroll
-  for (int u = 0; u < PER; u++) {
-    const int k = threadIdx.x + u * BLOCK;
-    jj[u] = k < cnt ? ja[q0 + k] : 0;
-    aa[u] = k < cnt ? val[q0 + k] : T(0);
-  }
+    for (int u = 0; u < PER; u++) {
+      const int k = threadIdx.x + u * BLOCK;
+      const int kk = k < cnt ? k : cnt - 1;
+      jj[u] = ja[q0 + kk];
+      aa[u] = val[q0 + kk];
+    }
 #pragma unroll
-  for (int u = 0; u < PER; u++) {
-    const int k = threadIdx.x + u * BLOCK;
-    if (k < cnt) prod[k] = aa[u] * x[jj[u]];
+    for (int u = 0; u < PER; u++) {
+      // an unconditional store (padding slots to the spare word, never
+      // read) keeps every gather out of a branch
+      const int k = threadIdx.x + u * BLOCK;
+      const T hm = k < cnt ? T(1) : T(0);
+      prod[k < cnt ? k : BN_ROWBLOCK_NNZ] = (aa[u] * hm) * x[jj[u]];
+    }
   }
   __syncthreads();
   // tpr = largest power of two <= 256 / rows, at most one wave
