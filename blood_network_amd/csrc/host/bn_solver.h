@@ -67,6 +67,8 @@ typedef struct {
   int *cmap_d;          /* device: row r of Ac is level row cmap_d[r] */
   double *dinvc_d;      /* 1/a_ii in Ac row order */
   float *dinvc_s;
+  int gs_lanes, gs_lanes_s; /* lanes per row of the colour kernels, timed on
+                               whole sweeps at upload (0: Ac's SpMV choice) */
   int *perm_h;          /* host perm[new] = old */
   int *perm_d;
   void *b, *x, *r;      /* level work vectors in V-cycle precision */
@@ -124,6 +126,7 @@ struct bn_solver {
   int *tune;            /* tuned kernel choices of the last upload (6 ints
                            per operator), reused while the shapes match */
   int ntune;
+  int tune_gs[AMG_MAX_LEVELS][2]; /* the levels' sweep lanes (FP64, FP32) */
   int vperm_state;      /* caller value order -> Asolve slots: 0 not yet
                            known, 1 the same order, 2 gathered by vperm_d */
   int *vperm_d;         /* device: Asolve slot k takes caller slot vperm_d[k] */

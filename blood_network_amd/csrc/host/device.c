@@ -286,6 +286,49 @@ static float time_launches(struct bn_solver *s, const bn_tune_env *env, bn_dcsr 
   return total;
 }
 
+/* Lanes per row of a GS level's colour kernels: each option times whole
+ * cold forward sweeps (all colours back to back, as the V-cycle runs
+ * them), FP64 and FP32 */
+static void tune_gs(struct bn_solver *s, const bn_tune_env *env, bn_dlevel *d, void *x, void *b) {
+  const int opts[5] = {1, 2, 4, 8, 16};
+  for (int p = 0; p < 2; p++) {
+    const char vt = p ? 's' : 'd';
+    const void *val = p ? (const void *)d->Ac.vs : (const void *)d->Ac.vd;
+    const void *dinvc = p ? (const void *)d->dinvc_s : (const void *)d->dinvc_d;
+    int best_l = 0;
+    float best = 1e30f;
+    for (int o = 0; o < 5; o++) {
+      const int lanes = opts[o];
+      for (int r = -1; r < BN_TUNE_REPS; r++) {
+        if (env->flush) bn_dot_partial(env->nflush, env->flush, env->flush, NULL, NULL, s->part1, NULL, s->stream);
+        if (r >= 0) hipEventRecord(env->ev[2 * r], s->stream);
+        for (int c = 0; c < d->ncol; c++)
+          bn_gs_color_map(vt, d->col_ptr[c], d->col_ptr[c + 1], d->cmap_d, d->Ac.ia, d->Ac.ja, val, dinvc, b, x,
+                          lanes, NULL, NULL, s->stream);
+        if (r >= 0) hipEventRecord(env->ev[2 * r + 1], s->stream);
+      }
+      hipEventSynchronize(env->ev[2 * BN_TUNE_REPS - 1]);
+      float total = 0.0f;
+      for (int r = 0; r < BN_TUNE_REPS; r++) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, env->ev[2 * r], env->ev[2 * r + 1]);
+        total += ms;
+      }
+      const char *lg = getenv("BN_TUNE_LOG");
+      if (lg && lg[0] == '2')
+        fprintf(stderr, "  gs %c rows=%d colours=%d lanes=%d: %.1f us per cold sweep\n", vt, d->n, d->ncol, lanes,
+                1e3 * total / BN_TUNE_REPS);
+      if (total < best) best = total, best_l = lanes;
+    }
+    const char *lg = getenv("BN_TUNE_LOG");
+    if (lg && (lg[0] == '1' || lg[0] == '2'))
+      fprintf(stderr, "tune gs %c rows=%d colours=%d: lanes=%d (%.1f us per cold sweep)\n", vt, d->n, d->ncol, best_l,
+              1e3 * best / BN_TUNE_REPS);
+    if (p) d->gs_lanes_s = best_l;
+    else d->gs_lanes = best_l;
+  }
+}
+
 static void autotune_csr(struct bn_solver *s, const bn_tune_env *env, bn_dcsr *d, void *x, void *y,
                          int allow_blocks, int allow_deep) {
   if (!d->ia || d->rows <= 0 || d->nnz <= 0) return;
@@ -421,6 +464,10 @@ static void autotune_all(struct bn_solver *s) {
       ops[k]->lanes = t[2], ops[k]->lanes_s = t[3], ops[k]->deep = t[4], ops[k]->deep_s = t[5];
       ops[k]->adapt = t[6], ops[k]->adapt_s = t[7];
     }
+    for (int l = 0; l < s->nlev; l++) {
+      s->lev[l].gs_lanes = s->tune_gs[l][0];
+      s->lev[l].gs_lanes_s = s->tune_gs[l][1];
+    }
   } else {
     void *x = s->x, *y = s->q;
     bn_fill('d', s->lev[0].n, x, 1.0, s->stream);
@@ -437,6 +484,12 @@ static void autotune_all(struct bn_solver *s) {
     for (int k = 0; k < 2 * BN_TUNE_REPS + 2; k++) evok &= hipEventCreate(&env.ev[k]) == hipSuccess;
     for (int k = 0; evok && k < nops; k++)
       if (ops[k]) autotune_csr(s, &env, ops[k], x, y, flags[k] & 1, (flags[k] & 2) != 0);
+    for (int l = 0; evok && l < s->nlev; l++) {
+      bn_dlevel *d = &s->lev[l];
+      if (d->use_gs && !d->block && d->ncol > 0 && d->Ac.ia) tune_gs(s, &env, d, x, y);
+      s->tune_gs[l][0] = d->gs_lanes;
+      s->tune_gs[l][1] = d->gs_lanes_s;
+    }
     hipStreamSynchronize(s->stream);
     for (int k = 0; k < 2 * BN_TUNE_REPS + 2; k++)
       if (env.ev[k]) hipEventDestroy(env.ev[k]);

@@ -246,7 +246,10 @@ int bn_solver_time_kernel(void *h, int which, int reps, double *ms_avg, double *
             bn_gs_color_map(vt, d->col_ptr[c], d->col_ptr[c + 1], d->cmap_d, d->Ac.ia, d->Ac.ja,
                             vt == 's' ? (const void *)d->Ac.vs : (const void *)d->Ac.vd,
                             vt == 's' ? (const void *)d->dinvc_s : (const void *)d->dinvc_d, s->r1, s->u1,
-                            bn_lanes(&d->Ac, vt), NULL, NULL, st);
+                            (vt == 's' ? d->gs_lanes_s : d->gs_lanes) > 0
+                                ? (vt == 's' ? d->gs_lanes_s : d->gs_lanes)
+                                : bn_lanes(&d->Ac, vt),
+                            NULL, NULL, st);
         b = (double)d->A.nnz * (4.0 + v) + (n + 1.0) * 4.0 + 3.0 * n * v + n * v; /* + dinv */
         break;
       }

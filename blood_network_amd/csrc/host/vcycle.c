@@ -91,8 +91,9 @@ static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int bac
           bn_gs_color_zero_map(vt, lo, hi, n, d->cmap_d, dinvc, b, x, s->stream);
           continue;
         }
+        const int gl = vt == 's' ? d->gs_lanes_s : d->gs_lanes;
         bn_gs_color_map(vt, lo, hi, d->cmap_d, d->Ac.ia, d->Ac.ja, opv(&d->Ac, vt), dinvc, b, x,
-                        bn_lanes(&d->Ac, vt), last ? zbase : NULL, last ? zsum : NULL, s->stream);
+                        gl > 0 ? gl : bn_lanes(&d->Ac, vt), last ? zbase : NULL, last ? zsum : NULL, s->stream);
       }
     }
   } else {
