@@ -191,6 +191,13 @@ static inline int bn_spmv_lanes(const bn_dcsr *m, char vt) {
  * rows the lanes kernel covers) */
 static inline void bn_dcsr_apply_dot(const bn_dcsr *m, char vt, int op, const void *x, void *y, const void *b,
                                      const void *dotw, double *partial, hipStream_t st) {
+  if (vt == 'd' && m->adapt && m->blk) {
+    /* the row-block kernel was timed fastest: run it, then the dot (the
+     * same deterministic block partials the fused form writes) */
+    bn_csr_apply_blocks('d', op, m->nblk, m->blk, m->ia, m->ja, m->vd, x, y, b, st);
+    if (partial) bn_dot_partial(m->rows, (const double *)dotw, (const double *)y, NULL, NULL, partial, NULL, st);
+    return;
+  }
   bn_csr_apply(vt, op, m->rows, m->ia, m->ja, vt == 's' ? (const void *)m->vs : (const void *)m->vd, x, y, b,
                bn_spmv_lanes(m, vt), dotw, partial, st);
 }

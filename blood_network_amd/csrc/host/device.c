@@ -432,8 +432,9 @@ static int tune_list(struct bn_solver *s, bn_dcsr **ops, int *flags) {
     ops[n] = &s->lev[l].R, flags[n++] = allow ? 3 : 2;
     ops[n] = &s->lev[l].Ac, flags[n++] = 0;
   }
-  /* the outer PCG's q = A p fuses p'q into the lanes kernel (no row blocks) */
-  ops[n] = s->asolve_shared ? NULL : &s->Asolve, flags[n++] = 2;
+  /* the outer PCG's q = A p: a lanes kernel with p'q fused, or the row
+   * blocks followed by the dot */
+  ops[n] = s->asolve_shared ? NULL : &s->Asolve, flags[n++] = apply_mode() == 0 ? 3 : 2;
   ops[n] = &s->fine_ic.lo, flags[n++] = 0;
   ops[n] = &s->fine_ic.up, flags[n++] = 0;
   ops[n] = &s->coarse_ic.lo, flags[n++] = 0;
