@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "PCG time to 1e-8 rel. residual, 4M-node net: FP32-AMG vs FP64 GPU; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_IC_JACOBI = {"lattice160_trees17": 237249095}  # measured HBM bytes per launch
+TRAFFIC_IC_JACOBI = {"lattice160_trees17": 238058915}  # measured HBM bytes per launch
 FAST_COLOURS = 16    # GS on levels with <= 16 colours, l1-Jacobi above
 FAST_IC_SWEEPS = 4   # Jacobi sweeps per IC(0) triangle
 
