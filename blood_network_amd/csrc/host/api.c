@@ -209,8 +209,12 @@ void *bn_solver_create(int n, int nnz, const int *ia, const int *ja, const doubl
   a->omega_jacobi = s->opt.omega;
   a->coarsening = s->opt.coarsening;
   bn_build_hierarchy(a, s->A, s->opt.strength);
+  const double th = now_ms();
   ichol_compute(s->A, &s->L_own);
   s->last.setup_ms = now_ms() - t0;
+  const char *lg = getenv("BN_SETUP_LOG");
+  if (lg && *lg && *lg != '0')
+    fprintf(stderr, "[setup] hierarchy %.1f ms, fine IC(0) %.1f ms\n", th - t0, now_ms() - th);
   s->last.num_levels = a->num_levels;
   if (s->opt.verbose) {
     fprintf(stderr, "bnamg hierarchy (%s, theta=%.2f, min_size=%d): %d levels\n",

@@ -70,6 +70,12 @@ static int cmp_desc_ll(const void *a, const void *b) {
   return (x < y) - (x > y);
 }
 
+static int is_identity(const int *perm, int n) {
+  for (int i = 0; i < n; i++)
+    if (perm[i] != i) return 0;
+  return 1;
+}
+
 static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
   d->rows = m->row;
   d->cols = m->col;
@@ -149,13 +155,12 @@ static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
     free(bk);
     d->adapt = d->adapt_s = 0;
   }
-  float *tmp = (float *)bn_xmalloc((size_t)m->nnz * sizeof(float));
-  for (int k = 0; k < m->nnz; k++) tmp[k] = (float)m->val[k];
   CK(hipMemcpy(d->ia, m->ia, ((size_t)m->row + 1) * sizeof(int), hipMemcpyHostToDevice));
   CK(hipMemcpy(d->ja, m->ja, (size_t)m->nnz * sizeof(int), hipMemcpyHostToDevice));
   CK(hipMemcpy(d->vd, m->val, (size_t)m->nnz * sizeof(double), hipMemcpyHostToDevice));
-  CK(hipMemcpy(d->vs, tmp, (size_t)m->nnz * sizeof(float), hipMemcpyHostToDevice));
-  free(tmp);
+  /* the FP32 copy is rounded on the device (same round-to-nearest cast) */
+  bn_rb_to_float(m->nnz, d->vd, d->vs, s->stream);
+  CK(hipStreamSynchronize(s->stream));
   return 0;
 }
 
@@ -218,8 +223,8 @@ static int upload_tri(struct bn_solver *s, const ichol_factor *F, const int *per
       rows_b[k] = inv[rows_b[k]];
     }
   } else {
-    bn_csr_copy(&Lh, &Lp);
-    bn_csr_copy(&Uh, &Up);
+    Lp = Lh; /* read only: no copy */
+    Up = Uh;
     memcpy(dinv_p, dinv, (size_t)n * sizeof(double));
   }
   int rc = upload_csr(s, &Lp, &t->lo);
@@ -235,9 +240,11 @@ static int upload_tri(struct bn_solver *s, const ichol_factor *F, const int *per
     CK(hipMemcpy(t->rows_b, rows_b, (size_t)n * sizeof(int), hipMemcpyHostToDevice));
   }
   t->n = n;
+  if (perm) {
+    bn_csr_free(&Lp);
+    bn_csr_free(&Up);
+  }
   bn_csr_free(&Uh);
-  bn_csr_free(&Lp);
-  bn_csr_free(&Up);
   free(dinv);
   free(dinv_p);
   free(rows_f);
@@ -335,9 +342,14 @@ static void autotune_csr(struct bn_solver *s, const bn_tune_env *env, bn_dcsr *d
   const int base = d->lanes;
   int cand[24], nc = 0;
   const int opts[7] = {1, 2, 4, 8, 16, 32, 64};
+  /* lanes far beyond the mean row length only idle, unless the operator
+   * is small enough that more lanes are what fills the device */
+  int np2 = 1;
+  while (np2 < 64 && (long long)np2 * d->rows < (long long)d->nnz) np2 *= 2;
   for (int k = 0; k < 7; k++) {
     int l = opts[k];
     if (l < 1 || l > 64) continue;
+    if (l > 4 && l > 2 * np2 && (long long)l * d->rows > (1LL << 20)) continue;
     int dup = 0;
     for (int q = 0; q < nc; q++) dup |= (cand[q] == l);
     if (!dup) cand[nc++] = l;
@@ -635,6 +647,8 @@ int bn_device_upload(struct bn_solver *s) {
   int *chunk_h[AMG_MAX_LEVELS], *ccol_h[AMG_MAX_LEVELS], *cbnd_h[AMG_MAX_LEVELS], *cperm_h[AMG_MAX_LEVELS];
   int rc = 0;
 
+  double tph[7];
+  tph[0] = now_ms();
   /* colour orders and inverse permutations */
   for (int l = 0; l < nlev; l++) {
     const amg_level *L = &amg->levels[l];
@@ -672,13 +686,17 @@ int bn_device_upload(struct bn_solver *s) {
     inv[l] = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
     for (int k = 0; k < n; k++) inv[l][d->perm_h[k]] = k;
   }
+  tph[6] = now_ms();
 
   for (int l = 0; l < nlev && !rc; l++) {
     const amg_level *L = &amg->levels[l];
     bn_dlevel *d = &s->lev[l];
     const int n = d->n;
     dcsr_mat Ap;
-    bn_csr_permute(&L->A, d->perm_h, inv[l], &Ap);
+    /* levels outside the block layout keep their numbering: no copy */
+    const int same = is_identity(d->perm_h, n);
+    if (same) Ap = L->A;
+    else bn_csr_permute(&L->A, d->perm_h, inv[l], &Ap);
     rc = upload_csr(s, &Ap, &d->A);
     double *dinv = (double *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(double));
     double *l1inv = (double *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(double));
@@ -689,14 +707,16 @@ int bn_device_upload(struct bn_solver *s) {
     if (!rc && d->use_gs) rc = upload_colour_rows(s, d, &Ap, cperm_h[l]);
     free(dinv);
     free(l1inv);
-    bn_csr_free(&Ap);
+    if (!same) bn_csr_free(&Ap);
     if (!rc && l < nlev - 1) {
       dcsr_mat Pp, Rp;
-      bn_csr_permute(&L->P, d->perm_h, inv[l + 1], &Pp);
+      const int psame = same && is_identity(s->lev[l + 1].perm_h, s->lev[l + 1].n);
+      if (psame) Pp = L->P;
+      else bn_csr_permute(&L->P, d->perm_h, inv[l + 1], &Pp);
       bn_csr_transpose(&Pp, &Rp);
       rc = upload_csr(s, &Pp, &d->P);
       if (!rc) rc = upload_csr(s, &Rp, &d->R);
-      bn_csr_free(&Pp);
+      if (!psame) bn_csr_free(&Pp);
       bn_csr_free(&Rp);
     }
     if (!rc) {
@@ -735,8 +755,12 @@ int bn_device_upload(struct bn_solver *s) {
       bn_csr_free(&Ap);
     }
   }
+  tph[1] = now_ms();
   /* fine IC(0) (absent for hierarchy-only device mirrors) */
-  if (!rc && s->L) rc = upload_tri(s, s->L, s->lev[0].perm_h, inv[0], &s->fine_ic);
+  if (!rc && s->L)
+    rc = upload_tri(s, s->L, is_identity(s->lev[0].perm_h, s->lev[0].n) ? NULL : s->lev[0].perm_h, inv[0],
+                    &s->fine_ic);
+  tph[2] = now_ms();
 
   /* coarsest solver */
   if (!rc) {
@@ -764,6 +788,7 @@ int bn_device_upload(struct bn_solver *s) {
     for (int k = 0; k < 6 && !rc; k++) rc = alloc_vec(s, (void **)&s->cvec[k], nc);
   }
 
+  tph[3] = now_ms();
   /* PCG and preconditioner work space */
   if (!rc) {
     const int n = s->lev[0].n;
@@ -780,6 +805,7 @@ int bn_device_upload(struct bn_solver *s) {
     if (!rc && !s->host_scal) CK(hipHostMalloc((void **)&s->host_scal, BN_S_COUNT * 2 * sizeof(double), 0));
   }
   for (int l = 0; l < nlev; l++) free(inv[l]);
+  tph[4] = now_ms();
   if (!rc) {
     const char *env = getenv("BN_NO_AUTOTUNE");
     if (!(env && env[0] && env[0] != '0')) autotune_all(s);
@@ -787,6 +813,14 @@ int bn_device_upload(struct bn_solver *s) {
     CK(hipDeviceSynchronize());
     s->dev_ready = 1;
   }
+  tph[5] = now_ms();
   s->last.upload_ms = now_ms() - t0;
+  const char *lg = getenv("BN_SETUP_LOG");
+  if (lg && *lg && *lg != '0')
+    fprintf(stderr,
+            "[upload] %.1f ms: colour orders %.1f, level operators %.1f, fine IC(0) factor and schedules %.1f, "
+            "coarsest solver %.1f, work space %.1f, autotuning %.1f\n",
+            tph[5] - t0, tph[6] - tph[0], tph[1] - tph[6], tph[2] - tph[1], tph[3] - tph[2], tph[4] - tph[3],
+            tph[5] - tph[4]);
   return rc;
 }
