@@ -649,8 +649,13 @@ int bn_device_upload(struct bn_solver *s) {
 
   double tph[7];
   tph[0] = now_ms();
-  /* colour orders and inverse permutations */
-  for (int l = 0; l < nlev; l++) {
+  /* colour orders and inverse permutations: one task per level (each
+   * level's result is its own, whatever runs first) */
+#pragma omp parallel
+#pragma omp single
+  for (int l = 0; l < nlev; l++)
+#pragma omp task firstprivate(l) shared(chunk_h, ccol_h, cbnd_h, cperm_h, inv)
+  {
     const amg_level *L = &amg->levels[l];
     bn_dlevel *d = &s->lev[l];
     const int n = L->A.row;

@@ -55,7 +55,16 @@ int bn_colour_order(const dcsr_mat *A, const int *mis, int split, int *perm, int
     nclass = 2;
   }
   int ncol_cls[2] = {0, 0};
-  for (int c = 0; c < nclass; c++) ncol_cls[c] = greedy_class(A, cls, c, col);
+  /* the classes colour independently (a node looks only at neighbours of
+   * its own class): the F class as a task beside the C class */
+  if (nclass == 2) {
+#pragma omp task shared(ncol_cls, col) firstprivate(A, cls) if (n > 65536)
+    ncol_cls[1] = greedy_class(A, cls, 1, col);
+    ncol_cls[0] = greedy_class(A, cls, 0, col);
+#pragma omp taskwait
+  } else {
+    ncol_cls[0] = greedy_class(A, cls, 0, col);
+  }
   const int ncol = ncol_cls[0] + ncol_cls[1];
   /* global colour id: C colours first, then F colours */
   int *cnt = (int *)bn_xcalloc((size_t)ncol + 1, sizeof(int));
