@@ -784,7 +784,22 @@ int bn_device_upload(struct bn_solver *s) {
         bn_set_error("device out of memory (coarsest inverse of %d rows)", nc);
         rc = BN_ERR_HIP;
       }
-      if (!rc) rc = bn_coarse_inverse_device(s, &C->A, s->cinv_d, s->cinv_s, &clamped);
+      if (!rc && nc <= BN_DENSE_MAX) {
+        rc = bn_coarse_inverse_device(s, &C->A, s->cinv_d, s->cinv_s, &clamped);
+      } else if (!rc) {
+        /* a forced dense solve above the device kernel's limit: the host
+         * inverse (same entries, slower) */
+        double *inv_h = (double *)bn_xmalloc((size_t)nc * (size_t)nc * sizeof(double));
+        float *inv_f = (float *)bn_xmalloc((size_t)nc * (size_t)nc * sizeof(float));
+        clamped = bn_dense_inverse(&C->A, inv_h);
+        for (size_t k = 0; k < (size_t)nc * (size_t)nc; k++) inv_f[k] = (float)inv_h[k];
+        if (hipMemcpy(s->cinv_d, inv_h, (size_t)nc * (size_t)nc * sizeof(double), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(s->cinv_s, inv_f, (size_t)nc * (size_t)nc * sizeof(float), hipMemcpyHostToDevice) != hipSuccess)
+          rc = BN_ERR_HIP;
+        free(inv_h);
+        free(inv_f);
+      }
       if (!rc && clamped && s->opt.verbose)
         fprintf(stderr, "bnamg: coarsest operator not numerically SPD (pivot clamped)\n");
     } else {

@@ -544,6 +544,11 @@ static int plan_build(struct bn_solver *s) {
         (r->pull && (!r->toff || !r->t1 || !r->t2 || !r->t3 || !r->elist)))
       rc = BN_ERR_HIP;
   }
+  if (!rc && s->coarse_dense && s->lev[nlev - 1].n > BN_DENSE_MAX) {
+    bn_set_error("rebuild on the device: a dense coarsest solve of %d rows is beyond its %d-row limit",
+                 s->lev[nlev - 1].n, BN_DENSE_MAX);
+    rc = BN_ERR_ARG;
+  }
   if (!rc && s->coarse_dense) {
     const size_t nc = (size_t)s->lev[nlev - 1].n;
     pl->dn_c = (double *)rb_dev(s, nc * nc * sizeof(double));

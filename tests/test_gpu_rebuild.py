@@ -155,6 +155,26 @@ def test_device_rebuild_full_network(monkeypatch):
         host_s.free()
 
 
+def test_device_rebuild_single_level():
+    """A system below min_size: one level, whose coarsest solve is the dense
+    inverse of the whole operator, and the fine IC(0) of the same matrix."""
+    case("device rebuild, one level (dense inverse of the whole operator)")
+    small = make_network(8, 5, seed=3)
+    dev_s = AMGSolver(small.ia, small.ja, small.val, coarsening="pmis", min_size=100000)
+    host_s = AMGSolver(small.ia, small.ja, small.val, coarsening="pmis", min_size=100000)
+    assert dev_s.num_levels() == 1
+    val, b = new_values(small, 9)
+    d_val = L.DeviceArray(val)
+    dev_s.rebuild_values_device(len(val), d_val.ptr)
+    host_s.update(small.ia, small.ja, val)
+    same_hierarchy(dev_s, host_s)
+    same_coarse_inverse(dev_s)
+    x1, it1 = dev_s.solve(b, tol=1e-10)
+    x2, it2 = host_s.solve(b, tol=1e-10)
+    print(f"n={small.n}: iterations {it1} / {it2}")
+    assert it1 == it2 and np.linalg.norm(x1 - x2) <= 1e-9 * np.linalg.norm(x2)
+
+
 def test_device_rebuild_errors(net):
     case("device rebuild: rejected inputs")
     val, _ = new_values(net, 4)
