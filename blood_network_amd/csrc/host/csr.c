@@ -77,9 +77,56 @@ void bn_prefix(const INT *counts, INT n, INT *ptr) {
   }
 }
 
-/* Counting sort by column.  Column-major fill visits source rows in
- * ascending order, so each row of A^T is sorted by source row. */
+/* Stable transpose (column j of a lists its rows ascending, with their
+ * values).  Large matrices split the rows into contiguous ranges, one per
+ * thread: per-range column counts, then each column's entries laid out
+ * range by range, so every thread scatters its rows in order and the
+ * result is the sequential one. */
 void bn_csr_transpose(const dcsr_mat *a, dcsr_mat *at) {
+#ifdef _OPENMP
+  int nt = omp_get_max_threads();
+  if (nt > 8) nt = 8;
+  if (a->nnz >= (1 << 20) && nt > 1) {
+    const INT nc = a->col, nr = a->row;
+    bn_csr_alloc(at, nc, nr, a->nnz);
+    INT *cnt = (INT *)bn_xcalloc((size_t)nt * ((size_t)nc + 1), sizeof(INT));
+#pragma omp parallel num_threads(nt)
+    {
+      const int t = omp_get_thread_num();
+      const INT r0 = (INT)((long long)nr * t / nt), r1 = (INT)((long long)nr * (t + 1) / nt);
+      INT *c = cnt + (size_t)t * ((size_t)nc + 1);
+      for (INT k = a->ia[r0]; k < a->ia[r1]; k++) c[a->ja[k]]++;
+#pragma omp barrier
+#pragma omp single
+      {
+        INT run = 0;
+        for (INT j = 0; j < nc; j++) {
+          at->ia[j] = run;
+          for (int u = 0; u < nt; u++) run += cnt[(size_t)u * ((size_t)nc + 1) + j];
+        }
+        at->ia[nc] = run;
+      }
+#pragma omp for schedule(static)
+      for (INT j = 0; j < nc; j++) {
+        INT pos = at->ia[j];
+        for (int u = 0; u < nt; u++) {
+          INT *cu = cnt + (size_t)u * ((size_t)nc + 1);
+          const INT m = cu[j];
+          cu[j] = pos;
+          pos += m;
+        }
+      }
+      for (INT i = r0; i < r1; i++)
+        for (INT k = a->ia[i]; k < a->ia[i + 1]; k++) {
+          const INT q = c[a->ja[k]]++;
+          at->ja[q] = i;
+          if (a->val && at->val) at->val[q] = a->val[k];
+        }
+    }
+    free(cnt);
+    return;
+  }
+#endif
   bn_csr_alloc(at, a->col, a->row, a->nnz);
   INT *cnt = (INT *)bn_xcalloc((size_t)a->col + 1, sizeof(INT));
   for (INT k = 0; k < a->nnz; k++) cnt[a->ja[k]]++;
