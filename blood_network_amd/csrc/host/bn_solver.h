@@ -246,6 +246,8 @@ void bn_vc_ichol_precond(struct bn_solver *s, const double *g, double *z);
  * fine_ic == BN_IC_NONE */
 void bn_vc_precond(struct bn_solver *s, const double *g, double *z);
 void bn_vc_vcycle_precond(struct bn_solver *s, const double *g, double *z);
+/* FP32 V-cycle: FP32 input in s->g_in, FP32 result left in s->zout */
+void bn_vc_precond_s(struct bn_solver *s);
 void bn_vc_fine_ic(struct bn_solver *s, const void *b, void *x);
 
 /* pcg.c */

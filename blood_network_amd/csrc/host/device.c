@@ -298,6 +298,27 @@ static float time_launches(struct bn_solver *s, const bn_tune_env *env, bn_dcsr 
  * them), FP64 and FP32 */
 static void tune_gs(struct bn_solver *s, const bn_tune_env *env, bn_dlevel *d, void *x, void *b) {
   const int opts[5] = {1, 2, 4, 8, 16};
+  /* BN_GS_LANES pins the colour kernels' lanes per row (1, 2, 4, 8 or
+   * 16): one value for every level, or a comma-separated list by level
+   * (levels past the list, or with 0, are timed) */
+  const char *pin = getenv("BN_GS_LANES");
+  if (pin && pin[0]) {
+    const int lev = (int)(d - s->lev);
+    int l = atoi(pin);
+    if (strchr(pin, ',')) {
+      const char *c = pin;
+      for (int k = 0; k < lev && c; k++) {
+        c = strchr(c, ',');
+        if (c) c++;
+      }
+      l = c ? atoi(c) : 0;
+    }
+    for (int o = 0; o < 5; o++)
+      if (opts[o] == l) {
+        d->gs_lanes = d->gs_lanes_s = l;
+        return;
+      }
+  }
   for (int p = 0; p < 2; p++) {
     const char vt = p ? 's' : 'd';
     const void *val = p ? (const void *)d->Ac.vs : (const void *)d->Ac.vd;

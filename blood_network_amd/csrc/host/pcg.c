@@ -37,13 +37,24 @@ static void enqueue_step(struct bn_solver *s) {
   const bn_dcsr *A = &s->Asolve;
   double *sc = s->scal;
   const int flex = s->opt.flexible;
-  bn_pcg_update_p(n, s->p, s->z, sc, st);
+  /* FP32 V-cycle: r's FP32 copy comes out of the x/r update and z stays
+   * FP32 (s->zout, widened where it is read), so no conversion pass runs;
+   * the first step's z is the one the start-up preconditioning left there */
+  const int f32 = (s->vt == 's');
+  if (f32) bn_pcg_update_p_s(n, s->p, (const float *)s->zout, sc, st);
+  else bn_pcg_update_p(n, s->p, s->z, sc, st);
   /* q = A p, p'q -> alpha */
   bn_dcsr_apply_dot(A, 'd', BN_OP_SET, s->p, s->q, NULL, s->p, s->part1, st);
   bn_scalar_finish(1, s->part1, NULL, NULL, sc, 0, st);
-  bn_pcg_update_xr(n, s->x, s->p, s->r, s->q, sc, s->part2, st);
-  bn_vc_precond(s, s->r, s->z);
-  bn_dot_partial(n, s->r, s->z, flex ? s->z : NULL, flex ? s->q : NULL, s->part1, s->part3, st);
+  if (f32) {
+    bn_pcg_update_xr_s(n, s->x, s->p, s->r, s->q, sc, s->part2, (float *)s->g_in, st);
+    bn_vc_precond_s(s);
+    bn_dot_rz_s(n, s->r, (const float *)s->zout, flex ? s->q : NULL, s->part1, s->part3, st);
+  } else {
+    bn_pcg_update_xr(n, s->x, s->p, s->r, s->q, sc, s->part2, st);
+    bn_vc_precond(s, s->r, s->z);
+    bn_dot_partial(n, s->r, s->z, flex ? s->z : NULL, flex ? s->q : NULL, s->part1, s->part3, st);
+  }
   bn_scalar_finish(flex ? 4 : 2, s->part1, s->part2, flex ? s->part3 : NULL, sc, 0, st);
 }
 

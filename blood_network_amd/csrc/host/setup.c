@@ -409,63 +409,88 @@ void rs_standard_interpolation_values(const dcsr_mat *A, const dcsr_mat *S,
                                       const INT *cf, dcsr_mat *P) {
   const INT n = A->row;
   const int nt = bn_num_threads();
-  INT *owner_all = (INT *)bn_xmalloc((size_t)nt * (size_t)n * sizeof(INT));
+  INT maxrow = 1;
+  for (INT i = 0; i < n; i++)
+    if (A->ia[i + 1] - A->ia[i] > maxrow) maxrow = A->ia[i + 1] - A->ia[i];
+  unsigned *tag_all = (unsigned *)bn_xmalloc((size_t)nt * (size_t)n * sizeof(unsigned));
   INT *slot_all = (INT *)bn_xmalloc((size_t)nt * (size_t)n * sizeof(INT));
+  INT *tslot_all = (INT *)bn_xmalloc((size_t)nt * (size_t)maxrow * sizeof(INT));
+  REAL *tval_all = (REAL *)bn_xmalloc((size_t)nt * (size_t)maxrow * sizeof(REAL));
+  /* about 64 chunks per thread: small dense levels balance their heavy
+   * rows, large levels keep the scheduling traffic low */
+  INT chunk = n / (64 * (INT)nt);
+  if (chunk < 64) chunk = 64;
+  if (chunk > 1024) chunk = 1024;
 #pragma omp parallel
   {
     const size_t off = (size_t)thread_id() * (size_t)n;
-    INT *owner = owner_all + off; /* owner[j] == i  <=>  j in S_i */
+    /* tag[j] = 2i + 1: j in S_i and a C point; 2i: j in S_i, not C
+     * (unsigned: below the unset value 0xffffffff for every int row i) */
+    unsigned *tag = tag_all + off;
     INT *slot = slot_all + off;
-    for (INT j = 0; j < n; j++) owner[j] = -1;
-#pragma omp for schedule(dynamic, 1024)
+    INT *tslot = tslot_all + (size_t)thread_id() * (size_t)maxrow;
+    REAL *tval = tval_all + (size_t)thread_id() * (size_t)maxrow;
+    for (INT j = 0; j < n; j++) tag[j] = 0xffffffffu;
+#pragma omp for schedule(dynamic, chunk)
     for (INT i = 0; i < n; i++) {
       if (cf[i] == CPT) {
         P->val[P->ia[i]] = 1.0;
         continue;
       }
-      for (INT k = S->ia[i]; k < S->ia[i + 1]; k++) owner[S->ja[k]] = i;
+      const unsigned own = 2u * (unsigned)i, cown = own + 1u;
       {
         INT p = P->ia[i];
-        for (INT k = S->ia[i]; k < S->ia[i + 1]; k++)
-          if (cf[S->ja[k]] == CPT) {
+        for (INT k = S->ia[i]; k < S->ia[i + 1]; k++) {
+          const INT j = S->ja[k];
+          if (cf[j] == CPT) {
             P->val[p] = 0.0;
-            slot[S->ja[k]] = p++;
+            tag[j] = cown;
+            slot[j] = p++;
+          } else {
+            tag[j] = own;
           }
+        }
       }
       REAL diag = 0.0, weak = 0.0;
       for (INT k = A->ia[i]; k < A->ia[i + 1]; k++) {
         const INT j = A->ja[k];
         const REAL a = A->val[k];
         if (j == i) diag = a;
-        else if (owner[j] == i && cf[j] == CPT) P->val[slot[j]] += a;
-        else if (owner[j] == i && cf[j] == FPT) continue;
+        else if (tag[j] == cown) P->val[slot[j]] += a;
+        else if (tag[j] == own && cf[j] == FPT) continue;
         else weak += a;
       }
       REAL dhat = diag + weak;
       for (INT k = A->ia[i]; k < A->ia[i + 1]; k++) {
         const INT f = A->ja[k];
-        if (f == i || owner[f] != i || cf[f] != FPT) continue;
+        if (f == i || tag[f] != own || cf[f] != FPT) continue;
         const REAL aik = A->val[k];
+        /* row f's entries in C_i^s, in row order: summed into the
+         * denominator, then (the same entries, same order) distributed */
         REAL den = 0.0;
+        INT nb = 0;
         for (INT q = A->ia[f]; q < A->ia[f + 1]; q++) {
           const INT m = A->ja[q];
-          if (cf[m] == CPT && owner[m] == i) den += A->val[q];
+          if (tag[m] == cown) {
+            den += A->val[q];
+            tslot[nb] = slot[m];
+            tval[nb++] = A->val[q];
+          }
         }
         if (fabs(den) < 1e-15) {
           dhat += aik;
         } else {
-          for (INT q = A->ia[f]; q < A->ia[f + 1]; q++) {
-            const INT m = A->ja[q];
-            if (cf[m] == CPT && owner[m] == i) P->val[slot[m]] += aik * A->val[q] / den;
-          }
+          for (INT t = 0; t < nb; t++) P->val[tslot[t]] += aik * tval[t] / den;
         }
       }
       if (fabs(dhat) > 1e-15)
         for (INT q = P->ia[i]; q < P->ia[i + 1]; q++) P->val[q] /= -dhat;
     }
   }
-  free(owner_all);
+  free(tag_all);
   free(slot_all);
+  free(tslot_all);
+  free(tval_all);
 }
 
 void rs_standard_interpolation(const dcsr_mat *A, const dcsr_mat *S, const INT *cf, dcsr_mat *P) {

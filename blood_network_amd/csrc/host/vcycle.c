@@ -298,14 +298,9 @@ static int coarse_icpcg(struct bn_solver *s, const double *b, double *x) {
 /* ---------------------------------------------------------------------
  * preconditioners (input g and output z are FP64, level-0 numbering)
  * ------------------------------------------------------------------- */
-void bn_vc_ichol_precond(struct bn_solver *s, const double *g, double *z) {
+/* gv, zv in the V-cycle's precision (FP32 copies when vt = 's') */
+static void ichol_core(struct bn_solver *s, const void *gv, void *zv) {
   const char vt = s->vt;
-  const int n = s->lev[0].n;
-  const void *gv = g;
-  if (vt == 's') {
-    bn_convert('d', 's', n, g, s->g_in, s->stream);
-    gv = s->g_in;
-  }
   const bn_dcsr *A = &s->Asolve;
   /* u1 = backslash(g) from zero */
   bn_vc_backslash(s, 0, gv, s->u1, 1);
@@ -318,12 +313,22 @@ void bn_vc_ichol_precond(struct bn_solver *s, const double *g, double *z) {
   /* s->r2 now holds u2; r1 <- g - A u2 */
   apply(s, A, BN_OP_RESID, s->r2, s->r1, gv);
   /* z = u2 + fwdslash(g - A u2) from zero, final update fused */
-  void *zv = (vt == 's') ? s->zout : (void *)z;
   fwdslash_rec(s, 0, s->r1, s->corr, 1, s->r2, zv);
-  if (vt == 's') bn_convert('s', 'd', n, s->zout, z, s->stream);
 }
 
-void bn_vc_vcycle_precond(struct bn_solver *s, const double *g, double *z) {
+static void vcycle_core(struct bn_solver *s, const void *gv, void *zv) {
+  bn_vc_backslash(s, 0, gv, s->u1, 1);
+  apply(s, &s->lev[0].A, BN_OP_RESID, s->u1, s->r1, gv);
+  fwdslash_rec(s, 0, s->r1, s->corr, 1, s->u1, zv);
+}
+
+static void precond_core(struct bn_solver *s, const void *gv, void *zv) {
+  if (s->opt.fine_ic == BN_IC_NONE) vcycle_core(s, gv, zv);
+  else ichol_core(s, gv, zv);
+}
+
+/* FP64 g, z; an FP32 V-cycle works on FP32 copies */
+static void precond64(struct bn_solver *s, const double *g, double *z, int ic) {
   const char vt = s->vt;
   const int n = s->lev[0].n;
   const void *gv = g;
@@ -331,12 +336,19 @@ void bn_vc_vcycle_precond(struct bn_solver *s, const double *g, double *z) {
     bn_convert('d', 's', n, g, s->g_in, s->stream);
     gv = s->g_in;
   }
-  bn_vc_backslash(s, 0, gv, s->u1, 1);
-  apply(s, &s->lev[0].A, BN_OP_RESID, s->u1, s->r1, gv);
   void *zv = (vt == 's') ? s->zout : (void *)z;
-  fwdslash_rec(s, 0, s->r1, s->corr, 1, s->u1, zv);
+  if (ic) ichol_core(s, gv, zv);
+  else vcycle_core(s, gv, zv);
   if (vt == 's') bn_convert('s', 'd', n, s->zout, z, s->stream);
 }
+
+void bn_vc_ichol_precond(struct bn_solver *s, const double *g, double *z) { precond64(s, g, z, 1); }
+
+void bn_vc_vcycle_precond(struct bn_solver *s, const double *g, double *z) { precond64(s, g, z, 0); }
+
+/* FP32 V-cycle only: input already in s->g_in (FP32), result left in
+ * s->zout (FP32) for consumers that widen it on the fly */
+void bn_vc_precond_s(struct bn_solver *s) { precond_core(s, s->g_in, s->zout); }
 
 void bn_vc_precond(struct bn_solver *s, const double *g, double *z) {
   if (s->opt.fine_ic == BN_IC_NONE) bn_vc_vcycle_precond(s, g, z);

@@ -180,6 +180,16 @@ void bn_pcg_update_xr(int n, double *x, const double *p, double *r,
 /* p = z + beta p, beta = s[BN_S_BETA] */
 void bn_pcg_update_p(int n, double *p, const double *z, const double *s,
                      hipStream_t stream);
+/* FP32-preconditioner forms: the x/r update also stores r rounded to
+ * FP32 in rs; the r.z (and z.q if q != NULL) partials and the p update
+ * read z in FP32 (exactly widened, so results equal the FP64-z forms') */
+void bn_pcg_update_xr_s(int n, double *x, const double *p, double *r,
+                        const double *q, const double *s, double *partial,
+                        float *rs, hipStream_t stream);
+void bn_dot_rz_s(int n, const double *r, const float *zs, const double *q,
+                 double *partial, double *partial2, hipStream_t stream);
+void bn_pcg_update_p_s(int n, double *p, const float *zs, const double *s,
+                       hipStream_t stream);
 /* y = y + x (vt) */
 void bn_axpy1(char vt, int n, const void *x, void *y, hipStream_t stream);
 

@@ -215,8 +215,12 @@ void launch_csr(int op, int n, const int *ia, const int *ja, const T *val, const
   const long long nrb = ((long long)n + RPB - 1) / RPB;
 #define BN_CSR_LAUNCH(OPV, DOTV, G, CH) \
   hipLaunchKernelGGL((k_csr<T, LANES, OPV, DOTV, R>), dim3(G), dim3(BLOCK), 0, st, n, CH, ia, ja, val, x, y, b, w, partial, long_len)
-  if (partial) {
-    // fused dot: capped grid so the partials fit BN_RED_BLOCKS slots
+  // fused dot: a capped grid so the partials fit BN_RED_BLOCKS slots.
+  // Past that many row blocks each block would walk several of them in
+  // turn, which measured slower than the one-row-block-per-block launch
+  // followed by a separate w.y pass: FP64 takes that route
+  const bool split = partial && nrb > BN_RED_BLOCKS && sizeof(T) == sizeof(double);
+  if (partial && !split) {
     const int g = grid_for(nrb);
     const int chunk = (int)((nrb + g - 1) / g);
     if (op == BN_OP_SET) BN_CSR_LAUNCH(BN_OP_SET, true, g, chunk);
@@ -228,6 +232,7 @@ void launch_csr(int op, int n, const int *ia, const int *ja, const T *val, const
     if (op == BN_OP_SET) BN_CSR_LAUNCH(BN_OP_SET, false, g, 1);
     else if (op == BN_OP_RESID) BN_CSR_LAUNCH(BN_OP_RESID, false, g, 1);
     else BN_CSR_LAUNCH(BN_OP_ADD, false, g, 1);
+    if (split) bn_dot_partial(n, (const double *)w, (const double *)y, NULL, NULL, partial, NULL, st);
   }
 #undef BN_CSR_LAUNCH
 }
@@ -745,10 +750,39 @@ __global__ __launch_bounds__(BLOCK) void k_dot2(int n, int chunk, const double *
   }
 }
 
+// partials of r.z (and of z.q when q != NULL) with z held in FP32: the
+// products and their order are k_dot2's over the widened z
+__global__ __launch_bounds__(BLOCK) void k_dot_rz_s(int n, int chunk, const double *__restrict__ r,
+                                                    const float *__restrict__ zs, const double *__restrict__ q,
+                                                    double *__restrict__ p1, double *__restrict__ p2) {
+  const int lb = logical_block();
+  const long long i0 = (long long)lb * chunk * BLOCK;
+  long long i1 = i0 + (long long)chunk * BLOCK;
+  if (i1 > n) i1 = n;
+  double a = 0.0, c = 0.0;
+  for (long long i = i0 + threadIdx.x; i < i1; i += BLOCK) {
+    const double z = (double)zs[i];
+    a += r[i] * z;
+    if (q) c += z * q[i];
+  }
+  a = block_sum(a);
+  if (q) c = block_sum(c);
+  if (threadIdx.x == 0) {
+    p1[lb] = a;
+    if (q) p2[lb] = c;
+  }
+  if (lb == 0) {
+    clear_tail(p1);
+    if (q) clear_tail(p2);
+  }
+}
+
+// WRITE_S: also store r's FP32 rounding into rs (the V-cycle's input)
+template <bool WRITE_S>
 __global__ __launch_bounds__(BLOCK) void k_update_xr(int n, int chunk, double *__restrict__ x,
                                                      const double *__restrict__ p, double *__restrict__ r,
                                                      const double *__restrict__ q, const double *__restrict__ s,
-                                                     double *__restrict__ part) {
+                                                     double *__restrict__ part, float *__restrict__ rs) {
   const int lb = logical_block();
   const double alpha = s[BN_S_ALPHA];
   const long long i0 = (long long)lb * chunk * BLOCK;
@@ -759,6 +793,7 @@ __global__ __launch_bounds__(BLOCK) void k_update_xr(int n, int chunk, double *_
     x[i] = x[i] + p[i] * alpha;
     const double rn = r[i] + q[i] * (-alpha);
     r[i] = rn;
+    if (WRITE_S) rs[i] = (float)rn;
     a += rn * rn;
   }
   a = block_sum(a);
@@ -773,18 +808,36 @@ __global__ void k_update_p(int n, double *__restrict__ p, const double *__restri
   if (i < n) p[i] = beta * p[i] + z[i];
 }
 
-// fixed-order finish of BN_RED_BLOCKS partials by one block
-__device__ double finish_sum(const double *p) {
-  double v = 0.0;
-  for (int k = threadIdx.x; k < BN_RED_BLOCKS; k += BLOCK) v += p[k];
-  return block_sum(v);
+// p = beta p + z with z held in FP32 (widening is exact: the same sums as
+// k_update_p over the widened copy)
+__global__ void k_update_p_s(int n, double *__restrict__ p, const float *__restrict__ zs,
+                             const double *__restrict__ s) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const double beta = s[BN_S_BETA];
+  if (i < n) p[i] = beta * p[i] + (double)zs[i];
 }
 
 __global__ __launch_bounds__(BLOCK) void k_scalar_finish(int which, const double *p1, const double *p2,
                                                          const double *p3, double *s, int slot) {
-  const double a = p1 ? finish_sum(p1) : 0.0;
-  const double b = p2 ? finish_sum(p2) : 0.0;
-  const double c = p3 ? finish_sum(p3) : 0.0;
+  // fixed-order finish of BN_RED_BLOCKS partials per array by one block:
+  // each thread sums its strided slots in ascending order, then the block
+  // sum; the three arrays' loads are issued in one pass
+  // (an absent array reads p1's slots instead, so no load sits behind a
+  // branch; its sum is not used)
+  const double *q1 = p1 ? p1 : (p2 ? p2 : p3);
+  const double *q2 = p2 ? p2 : q1, *q3 = p3 ? p3 : q1;
+  double va = 0.0, vb = 0.0, vc = 0.0;
+  if (q1) {
+#pragma unroll
+    for (int k = threadIdx.x; k < BN_RED_BLOCKS; k += BLOCK) {
+      va += q1[k];
+      vb += q2[k];
+      vc += q3[k];
+    }
+  }
+  const double a = p1 ? block_sum(va) : 0.0;
+  const double b = p2 ? block_sum(vb) : 0.0;
+  const double c = p3 ? block_sum(vc) : 0.0;
   if (threadIdx.x != 0) return;
   switch (which) {
     case 0:
@@ -1099,7 +1152,38 @@ void bn_pcg_update_xr(int n, double *x, const double *p, double *r, const double
   const long long nb = ((long long)n + BLOCK - 1) / BLOCK;
   const int g = grid_for(nb);
   const int chunk = (int)((nb + g - 1) / g);
-  hipLaunchKernelGGL(k_update_xr, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, p, r, q, s, partial);
+  hipLaunchKernelGGL(k_update_xr<false>, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, p, r, q, s, partial,
+                     (float *)NULL);
+}
+
+void bn_pcg_update_xr_s(int n, double *x, const double *p, double *r, const double *q, const double *s,
+                        double *partial, float *rs, hipStream_t stream) {
+  if (n <= 0) {
+    bn_fill('d', BN_RED_BLOCKS, partial, 0.0, stream);
+    return;
+  }
+  const long long nb = ((long long)n + BLOCK - 1) / BLOCK;
+  const int g = grid_for(nb);
+  const int chunk = (int)((nb + g - 1) / g);
+  hipLaunchKernelGGL(k_update_xr<true>, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, p, r, q, s, partial, rs);
+}
+
+void bn_dot_rz_s(int n, const double *r, const float *zs, const double *q, double *partial, double *partial2,
+                 hipStream_t stream) {
+  if (n <= 0) {
+    bn_fill('d', BN_RED_BLOCKS, partial, 0.0, stream);
+    if (q) bn_fill('d', BN_RED_BLOCKS, partial2, 0.0, stream);
+    return;
+  }
+  const long long nb = ((long long)n + BLOCK - 1) / BLOCK;
+  const int g = grid_for(nb);
+  const int chunk = (int)((nb + g - 1) / g);
+  hipLaunchKernelGGL(k_dot_rz_s, dim3(g), dim3(BLOCK), 0, stream, n, chunk, r, zs, q, partial, partial2);
+}
+
+void bn_pcg_update_p_s(int n, double *p, const float *zs, const double *s, hipStream_t stream) {
+  if (n <= 0) return;
+  hipLaunchKernelGGL(k_update_p_s, g1(n), dim3(BLOCK), 0, stream, n, p, zs, s);
 }
 
 void bn_pcg_update_p(int n, double *p, const double *z, const double *s, hipStream_t stream) {
