@@ -65,6 +65,7 @@ typedef struct {
   bn_dcsr Ac;           /* GS levels: A's rows copied in colour order,
                            columns in the level numbering */
   int *cmap_d;          /* device: row r of Ac is level row cmap_d[r] */
+  unsigned char *colour_d; /* device: colour of each level row (NULL above 255 colours) */
   double *dinvc_d;      /* 1/a_ii in Ac row order */
   float *dinvc_s;
   int gs_lanes, gs_lanes_s; /* lanes per row of the colour kernels, timed on

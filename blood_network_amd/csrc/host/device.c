@@ -643,6 +643,21 @@ static int upload_colour_rows(struct bn_solver *s, bn_dlevel *d, const dcsr_mat 
   d->cmap_d = (int *)dmalloc(s, (size_t)(n > 0 ? n : 1) * sizeof(int));
   if (!d->cmap_d) return BN_ERR_HIP;
   CK(hipMemcpy(d->cmap_d, cperm, (size_t)n * sizeof(int), hipMemcpyHostToDevice));
+  /* each level row's colour, for the coalesced first colour from zero */
+  d->colour_d = NULL;
+  if (d->ncol <= 255) {
+    unsigned char *col = (unsigned char *)bn_xmalloc((size_t)(n > 0 ? n : 1));
+    for (int c = 0; c < d->ncol; c++)
+      for (int r = d->col_ptr[c]; r < d->col_ptr[c + 1]; r++) col[cperm[r]] = (unsigned char)c;
+    d->colour_d = (unsigned char *)dmalloc(s, (size_t)(n > 0 ? n : 1));
+    if (!d->colour_d) {
+      free(col);
+      return BN_ERR_HIP;
+    }
+    const hipError_t e = hipMemcpy(d->colour_d, col, (size_t)n, hipMemcpyHostToDevice);
+    free(col);
+    if (e != hipSuccess) return BN_ERR_HIP;
+  }
   return 0;
 }
 

@@ -88,7 +88,10 @@ static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int bac
         const int lo = d->col_ptr[c], hi = d->col_ptr[c + 1];
         const void *dinvc = vecv(d->dinvc_d, d->dinvc_s, vt);
         if (sw == 0 && t == 0 && fused_zero) {
-          bn_gs_color_zero_map(vt, lo, hi, n, d->cmap_d, dinvc, b, x, s->stream);
+          if (d->colour_d)
+            bn_gs_color_zero_colour(vt, n, c, d->colour_d, vecv(d->dinv_d, d->dinv_s, vt), b, x, s->stream);
+          else
+            bn_gs_color_zero_map(vt, lo, hi, n, d->cmap_d, dinvc, b, x, s->stream);
           continue;
         }
         const int gl = vt == 's' ? d->gs_lanes_s : d->gs_lanes;

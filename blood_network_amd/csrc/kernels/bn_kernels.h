@@ -130,6 +130,10 @@ void bn_gs_color_map(char vt, int lo, int hi, const int *map, const int *ia, con
  * the later colours), in one pass */
 void bn_gs_color_zero_map(char vt, int lo, int hi, int all_n, const int *map, const void *dinv,
                           const void *b, void *x, hipStream_t stream);
+/* the same from level-row order: x_i = b_i dinv_i on rows whose colour[i]
+ * is c, 0 elsewhere (dinv in level-row order) */
+void bn_gs_color_zero_colour(char vt, int n, int c, const unsigned char *colour, const void *dinv, const void *b,
+                             void *x, hipStream_t stream);
 
 /* first colour of a sweep that starts from x = 0: x_i = b_i * dinv_i */
 void bn_gs_color_zero(char vt, int lo, int hi, const void *dinv,

@@ -666,6 +666,18 @@ __global__ void k_diag_scale_map(int lo, int hi, int all_n, const int *__restric
   x[i] = r < hi ? b[i] * dinv[r] : T(0);
 }
 
+// The same first colour from zero in level-row order: row i of colour c
+// gets b_i / a_ii, every other row 0, all accesses coalesced (dinv holds
+// the same 1/a_ii values as the colour-ordered copy)
+template <typename T>
+__global__ void k_diag_scale_colour(int n, int c, const unsigned char *__restrict__ colour,
+                                    const T *__restrict__ dinv, const T *__restrict__ b, T *__restrict__ x) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const T v = b[i] * dinv[i];
+  x[i] = colour[i] == c ? v : T(0);
+}
+
 template <typename T>
 __global__ void k_diag_scale(int lo, int cnt, const T *__restrict__ dinv, const T *__restrict__ b,
                              T *__restrict__ x) {
@@ -1016,6 +1028,17 @@ void bn_gs_color_zero_map(char vt, int lo, int hi, int all_n, const int *map, co
                        (const float *)dinv, (const float *)b, (float *)x);
   else
     hipLaunchKernelGGL(k_diag_scale_map<double>, g1(cnt), dim3(BLOCK), 0, stream, lo, hi, all_n, map,
+                       (const double *)dinv, (const double *)b, (double *)x);
+}
+
+void bn_gs_color_zero_colour(char vt, int n, int c, const unsigned char *colour, const void *dinv, const void *b,
+                             void *x, hipStream_t stream) {
+  if (n <= 0) return;
+  if (vt == 's')
+    hipLaunchKernelGGL(k_diag_scale_colour<float>, g1(n), dim3(BLOCK), 0, stream, n, c, colour,
+                       (const float *)dinv, (const float *)b, (float *)x);
+  else
+    hipLaunchKernelGGL(k_diag_scale_colour<double>, g1(n), dim3(BLOCK), 0, stream, n, c, colour,
                        (const double *)dinv, (const double *)b, (double *)x);
 }
 
