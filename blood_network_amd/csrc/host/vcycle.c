@@ -226,17 +226,24 @@ static void tri_jacobi(struct bn_solver *s, bn_dtri *t, char vt, const void *b, 
   int cur = 0;
   bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, NULL, bufs[cur], bn_lanes(&t->lo, vt), NULL,
                  NULL, s->stream);
+  /* the last lower sweep also writes the upper triangle's first step
+   * (yy_i dinv_i) into x, which the upper sweeps start from */
   for (int k = 1; k < sweeps; k++) {
-    bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, bufs[cur], bufs[1 - cur],
-                   bn_lanes(&t->lo, vt), NULL, NULL, s->stream);
+    if (k == sweeps - 1)
+      bn_jacobi_rows_scaled(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, bufs[cur], bufs[1 - cur], x,
+                            bn_lanes(&t->lo, vt), s->stream);
+    else
+      bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, bufs[cur], bufs[1 - cur],
+                     bn_lanes(&t->lo, vt), NULL, NULL, s->stream);
     cur = 1 - cur;
   }
   const void *yy = bufs[cur];
   void *xb[2] = {x, bufs[1 - cur]};
   int cx = 0;
   const int fuse = zsum && sweeps > 1;
-  bn_jacobi_rows(vt, n, t->up.ia, t->up.ja, opv(&t->up, vt), dinv, yy, NULL, xb[cx], bn_lanes(&t->up, vt), NULL,
-                 NULL, s->stream);
+  if (sweeps < 2)
+    bn_jacobi_rows(vt, n, t->up.ia, t->up.ja, opv(&t->up, vt), dinv, yy, NULL, xb[cx], bn_lanes(&t->up, vt), NULL,
+                   NULL, s->stream);
   for (int k = 1; k < sweeps; k++) {
     const int last = (k == sweeps - 1);
     bn_jacobi_rows(vt, n, t->up.ia, t->up.ja, opv(&t->up, vt), dinv, yy, xb[cx], xb[1 - cx],

@@ -162,6 +162,12 @@ void bn_trsv_level(char vt, int cnt, const int *rows, const int *ia,
 void bn_jacobi_rows(char vt, int n, const int *ia, const int *ja, const void *val,
                     const void *dinv, const void *b, const void *xin, void *xout,
                     int lanes, const void *zbase, void *zsum, hipStream_t stream);
+/* bn_jacobi_rows (xin != NULL, no zsum) that also writes xs_i = xout_i
+ * dinv_i: the first step from zero of a triangle whose right-hand side
+ * is xout and whose diagonal inverse is the same dinv */
+void bn_jacobi_rows_scaled(char vt, int n, const int *ia, const int *ja, const void *val,
+                           const void *dinv, const void *b, const void *xin, void *xout,
+                           void *xs, int lanes, hipStream_t stream);
 
 /* dense y = M x (row-major n x n), M in vt, x,y in vt */
 void bn_dense_gemv(char vt, int n, const void *M, const void *x, void *y,

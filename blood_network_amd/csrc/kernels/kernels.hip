@@ -616,12 +616,16 @@ void dispatch_block_gs(int nchunk, int max_rows, const int *chunk, const int *cc
   }
 }
 
-template <typename T, int LANES>
+// SCALED: also write xs_i = x_i dinv_i, the next triangle's first Jacobi
+// step from zero when its right-hand side is this sweep's x and both
+// triangles share the diagonal (the product k_diag_scale would form)
+template <typename T, int LANES, bool SCALED = false>
 __global__ __launch_bounds__(BLOCK) void k_jacobi_rows(int n, const int *__restrict__ ia,
                                                        const int *__restrict__ ja, const T *__restrict__ val,
                                                        const T *__restrict__ dinv, const T *__restrict__ b,
                                                        const T *__restrict__ xin, T *__restrict__ xout,
-                                                       const T *__restrict__ zbase, T *__restrict__ zsum) {
+                                                       const T *__restrict__ zbase, T *__restrict__ zsum,
+                                                       T *__restrict__ xs = nullptr) {
   constexpr int RPB = BLOCK / LANES;
   const int i = logical_block() * RPB + threadIdx.x / LANES;
   const int sub = threadIdx.x % LANES;
@@ -633,6 +637,7 @@ __global__ __launch_bounds__(BLOCK) void k_jacobi_rows(int n, const int *__restr
     const T v = (bi - s) * di;
     if (zsum) zsum[i] = zbase[i] + v;
     else xout[i] = v;
+    if (SCALED) xs[i] = v * di;
   }
 }
 
@@ -652,6 +657,24 @@ void dispatch_jacobi_rows(int n, const int *ia, const int *ja, const T *val, con
     default: BN_JR(64); break;
   }
 #undef BN_JR
+}
+
+template <typename T>
+void dispatch_jacobi_rows_scaled(int n, const int *ia, const int *ja, const T *val, const T *dinv, const T *b,
+                                 const T *xin, T *xout, T *xs, int lanes, hipStream_t st) {
+#define BN_JRS(L)                                                                                        \
+  hipLaunchKernelGGL((k_jacobi_rows<T, L, true>), dim3(grid_rows((n + BLOCK / L - 1) / (BLOCK / L))), dim3(BLOCK), \
+                     0, st, n, ia, ja, val, dinv, b, xin, xout, (const T *)nullptr, (T *)nullptr, xs)
+  switch (lanes) {
+    case 1: BN_JRS(1); break;
+    case 2: BN_JRS(2); break;
+    case 4: BN_JRS(4); break;
+    case 8: BN_JRS(8); break;
+    case 16: BN_JRS(16); break;
+    case 32: BN_JRS(32); break;
+    default: BN_JRS(64); break;
+  }
+#undef BN_JRS
 }
 
 // first colour from zero in RS_MAP layout: x_i = b_i dinv_r, i = map[r],
@@ -1070,6 +1093,17 @@ void bn_jacobi_rows(char vt, int n, const int *ia, const int *ja, const void *va
     dispatch_jacobi_rows<double>(n, ia, ja, (const double *)val, (const double *)dinv, (const double *)b,
                                  (const double *)xin, (double *)xout, lanes, (const double *)zbase,
                                  (double *)zsum, stream);
+}
+
+void bn_jacobi_rows_scaled(char vt, int n, const int *ia, const int *ja, const void *val, const void *dinv,
+                           const void *b, const void *xin, void *xout, void *xs, int lanes, hipStream_t stream) {
+  if (n <= 0) return;
+  if (vt == 's')
+    dispatch_jacobi_rows_scaled<float>(n, ia, ja, (const float *)val, (const float *)dinv, (const float *)b,
+                                       (const float *)xin, (float *)xout, (float *)xs, lanes, stream);
+  else
+    dispatch_jacobi_rows_scaled<double>(n, ia, ja, (const double *)val, (const double *)dinv, (const double *)b,
+                                        (const double *)xin, (double *)xout, (double *)xs, lanes, stream);
 }
 
 void bn_jacobi_update(char vt, int n, const void *r, const void *dscale, double omega, void *x,
