@@ -190,6 +190,7 @@ def _load():
                                         C.c_int, VP, VP]),
         "bn_gs_color_map": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
         "bn_gs_color_zero_map": (None, [C.c_char, C.c_int, C.c_int, C.c_int, VP, VP, VP, VP, VP]),
+        "bn_gs_color_zero_colour": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP]),
         "bn_block_gs": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP,
                                 C.c_int, C.c_int, VP, VP, VP]),
         "bn_chunk_order": (C.c_int, [C.POINTER(CSR), IP, C.c_int, C.c_int, IP, C.POINTER(IP), C.POINTER(IP),
@@ -203,6 +204,7 @@ def _load():
         "bn_trsv_level": (None, [C.c_char, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
         "bn_gs_color": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
         "bn_jacobi_rows": (None, [C.c_char, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
+        "bn_jacobi_rows_scaled": (None, [C.c_char, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP]),
         "bn_dense_gemv": (None, [C.c_char, C.c_int, VP, VP, VP, VP]),
         "bn_dot_partial": (None, [C.c_int, VP, VP, VP, VP, VP, VP, VP]),
         "bn_scalar_finish": (None, [C.c_int, VP, VP, VP, VP, C.c_int, VP]),

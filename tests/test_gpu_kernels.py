@@ -279,6 +279,23 @@ def test_mapped_multicolour_sweep_is_exact_gauss_seidel(vt, lanes):
     got = dx0.numpy().astype(np.float64)
     assert relerr(got[perm[lo:hi]], (b / D)[perm[lo:hi]]) < (1e-15 if vt == "d" else 1e-6)
     assert np.all(got[perm[hi:]] == 0.0)
+    # the same step in level-row order, from a one-byte colour index and
+    # 1/a_ii in level-row order: bit for bit the colour-order form's
+    # values, +0.0 on every other row, for the first and the last colour
+    colour = np.empty(n, np.uint8)
+    for c in range(ncol):
+        colour[perm[col_ptr[c]:col_ptr[c + 1]]] = c
+    dcol, ddn = dev(colour), dev((1.0 / D).astype(dt))
+    bits = np.uint64 if vt == "d" else np.uint32
+    for c in (0, ncol - 1):
+        lo, hi = int(col_ptr[c]), int(col_ptr[c + 1])
+        dm, dcl = dev(np.full(n, np.nan, dt)), dev(np.full(n, np.nan, dt))
+        lib.bn_gs_color_zero_map(vt.encode(), lo, hi, n, dmap.ptr, ddinv.ptr, db.ptr, dm.ptr, None)
+        lib.bn_gs_color_zero_colour(vt.encode(), n, c, dcol.ptr, ddn.ptr, db.ptr, dcl.ptr, None)
+        gm, gc = dm.numpy(), dcl.numpy()
+        rows = perm[lo:]  # the rows the colour-order form writes
+        assert np.array_equal(gc[rows].view(bits), gm[rows].view(bits))
+        assert np.array_equal(gc[colour != c].view(bits), np.zeros(int(np.sum(colour != c)), dt).view(bits))
 
 
 @pytest.mark.parametrize("vt", ["d", "s"])
@@ -328,6 +345,39 @@ def test_block_gs_sweep_solves_its_splitting(vt, backward, zero, lanes):
     rhs = b - Ap @ x_old
     assert relerr(lhs, rhs) < (1e-12 if vt == "d" else 2e-5)
     assert relerr(dzs.numpy().astype(np.float64), zbase + x_new) < (1e-15 if vt == "d" else 1e-6)
+
+
+@pytest.mark.parametrize("vt", ["d", "s"])
+@pytest.mark.parametrize("lanes", [1, 4])
+def test_scaled_jacobi_sweep_writes_next_first_step(vt, lanes):
+    # The scaled sweep writes the plain sweep's x bit for bit and also
+    # x_i dinv_i, which is what the diagonal scaling of the next triangle's
+    # first step forms from that x
+    net = make_network(14, 8, seed=5)
+    A = sp.csr_matrix((net.val, net.ja, net.ia), shape=(net.n, net.n))
+    Lo = sp.tril(A).tocsr()
+    Lo.sort_indices()
+    n = net.n
+    case(f"scaled Jacobi sweep n={n} vt={vt} lanes={lanes}")
+    dt = np.float64 if vt == "d" else np.float32
+    rng = np.random.default_rng(7)
+    b, xin = rng.random(n), rng.random(n)
+    dinv = (1.0 / Lo.diagonal()).astype(dt)
+    dia, dja, dv = dev(Lo.indptr.astype(np.int32)), dev(Lo.indices.astype(np.int32)), dev(Lo.data.astype(dt))
+    dd, db, dxi = dev(dinv), dev(b.astype(dt)), dev(xin.astype(dt))
+    x1, x2, xs = dev(np.zeros(n, dt)), dev(np.zeros(n, dt)), dev(np.zeros(n, dt))
+    lib.bn_jacobi_rows(vt.encode(), n, dia.ptr, dja.ptr, dv.ptr, dd.ptr, db.ptr, dxi.ptr, x1.ptr, lanes, None, None,
+                       None)
+    lib.bn_jacobi_rows_scaled(vt.encode(), n, dia.ptr, dja.ptr, dv.ptr, dd.ptr, db.ptr, dxi.ptr, x2.ptr, xs.ptr,
+                              lanes, None)
+    bits = np.uint64 if vt == "d" else np.uint32
+    g1, g2, gs = x1.numpy(), x2.numpy(), xs.numpy()
+    assert np.array_equal(g1.view(bits), g2.view(bits))
+    assert np.array_equal(gs.view(bits), (g2 * dinv).astype(dt).view(bits))
+    # and the sweep itself: x = D^-1 (b - L_off xin)
+    off = (Lo - sp.diags(Lo.diagonal())).tocsr()
+    ref = (b - off @ xin) / Lo.diagonal()
+    assert relerr(g1.astype(np.float64), ref) < (1e-14 if vt == "d" else 1e-6)
 
 
 def test_level_scheduled_ic_against_oracle(oracle):
