@@ -245,6 +245,23 @@ static void ic_jacobi_k(struct bn_solver *s, const void *B, const void *zbase, v
   }
 }
 
+/* z = B g in the V-cycle's precision (FP32 blocks b->g_in / b->zout when
+ * vt = 's') */
+static void precond_core_k(struct bn_solver *s, const void *gv, void *zv) {
+  const char vt = s->vt;
+  bn_block *b = &s->blk;
+  const bn_dcsr *A = &s->Asolve;
+  backslash_k(s, 0, gv, b->u1, 1);
+  apply_k(s, A, vt, BN_OP_RESID, b->u1, b->r1, gv);
+  if (s->opt.fine_ic == BN_IC_NONE) {
+    fwdslash_k(s, 0, b->r1, b->corr, b->u1, zv);
+  } else {
+    ic_jacobi_k(s, b->r1, b->u1, b->r2);
+    apply_k(s, A, vt, BN_OP_RESID, b->r2, b->r1, gv);
+    fwdslash_k(s, 0, b->r1, b->corr, b->r2, zv);
+  }
+}
+
 /* z = B g for the K columns of g (FP64 block vectors) */
 static void precond_k(struct bn_solver *s, const double *g, double *z) {
   const char vt = s->vt;
@@ -255,17 +272,7 @@ static void precond_k(struct bn_solver *s, const double *g, double *z) {
     bn_convert('d', 's', n * K, g, b->g_in, s->stream);
     gv = b->g_in;
   }
-  const bn_dcsr *A = &s->Asolve;
-  void *zv = (vt == 's') ? b->zout : (void *)z;
-  backslash_k(s, 0, gv, b->u1, 1);
-  apply_k(s, A, vt, BN_OP_RESID, b->u1, b->r1, gv);
-  if (s->opt.fine_ic == BN_IC_NONE) {
-    fwdslash_k(s, 0, b->r1, b->corr, b->u1, zv);
-  } else {
-    ic_jacobi_k(s, b->r1, b->u1, b->r2);
-    apply_k(s, A, vt, BN_OP_RESID, b->r2, b->r1, gv);
-    fwdslash_k(s, 0, b->r1, b->corr, b->r2, zv);
-  }
+  precond_core_k(s, gv, (vt == 's') ? b->zout : (void *)z);
   if (vt == 's') bn_convert('s', 'd', n * K, b->zout, z, s->stream);
 }
 
@@ -276,13 +283,24 @@ static void enqueue_step_k(struct bn_solver *s) {
   bn_block *b = &s->blk;
   const int n = s->lev[0].n, K = b->K;
   hipStream_t st = s->stream;
-  bn_update_p_k(K, n, b->p0, b->z0, b->scal, st);
+  /* FP32 V-cycle: r's FP32 copy comes out of the x/r update and z stays
+   * FP32 (b->zout, widened where it is read); the first step's z is the
+   * one the start-up preconditioning left there */
+  const int f32 = (s->vt == 's');
+  if (f32) bn_update_p_k_s(K, n, b->p0, (const float *)b->zout, b->scal, st);
+  else bn_update_p_k(K, n, b->p0, b->z0, b->scal, st);
   apply_k(s, &s->Asolve, 'd', BN_OP_SET, b->p0, b->q0, NULL);
   bn_dot_k(K, n, b->p0, b->q0, NULL, NULL, b->part1, NULL, st);
   bn_finish_k(K, 1, n, b->part1, NULL, NULL, b->scal, st);
-  bn_update_xr_k(K, n, b->x0, b->p0, b->r0, b->q0, b->scal, b->part2, st);
-  precond_k(s, b->r0, b->z0);
-  bn_dot_k(K, n, b->r0, b->z0, b->z0, b->q0, b->part1, b->part3, st);
+  if (f32) {
+    bn_update_xr_k_s(K, n, b->x0, b->p0, b->r0, b->q0, b->scal, b->part2, (float *)b->g_in, st);
+    precond_core_k(s, b->g_in, b->zout);
+    bn_dot_rz_k_s(K, n, b->r0, (const float *)b->zout, b->q0, b->part1, b->part3, st);
+  } else {
+    bn_update_xr_k(K, n, b->x0, b->p0, b->r0, b->q0, b->scal, b->part2, st);
+    precond_k(s, b->r0, b->z0);
+    bn_dot_k(K, n, b->r0, b->z0, b->z0, b->q0, b->part1, b->part3, st);
+  }
   bn_finish_k(K, 4, n, b->part1, b->part2, b->part3, b->scal, st);
 }
 

@@ -362,11 +362,12 @@ __global__ __launch_bounds__(BLOCK) void k_dot_k(int n, int chunk, const double 
 // per-column scalar slots (BN_S_COUNT doubles per column)
 #define SC(c, slot) s[(size_t)(c) * BN_S_COUNT + (slot)]
 
-template <int K>
+// WRITE_S: also store r rounded to FP32 in rs (the FP32 V-cycle's input)
+template <int K, bool WRITE_S = false>
 __global__ __launch_bounds__(BLOCK) void k_update_xr_k(int n, int chunk, double *__restrict__ x,
                                                        const double *__restrict__ p, double *__restrict__ r,
                                                        const double *__restrict__ q, const double *__restrict__ s,
-                                                       double *__restrict__ part) {
+                                                       double *__restrict__ part, float *__restrict__ rs = nullptr) {
   constexpr int SL = BLOCK / K;
   __shared__ double red[BLOCK];
   const int b = blockIdx.x;
@@ -379,9 +380,32 @@ __global__ __launch_bounds__(BLOCK) void k_update_xr_k(int n, int chunk, double 
     x[e] = x[e] + p[e] * al;
     const double rn = r[e] + q[e] * (-al);
     r[e] = rn;
+    if (WRITE_S) rs[e] = (float)rn;
     a += rn * rn;
   }
   col_partial<K>(a, red, part, b);
+}
+
+// k_dot_k(r, z, z, q) with z held in FP32 (widened exactly: the same
+// products in the same order)
+template <int K>
+__global__ __launch_bounds__(BLOCK) void k_dot_rz_k_s(int n, int chunk, const double *__restrict__ r,
+                                                      const float *__restrict__ zs, const double *__restrict__ q,
+                                                      double *__restrict__ p1, double *__restrict__ p2) {
+  constexpr int SL = BLOCK / K;
+  __shared__ double red[BLOCK];
+  const int b = blockIdx.x;
+  const int sl = threadIdx.x / K, c = threadIdx.x % K;
+  const int i0 = b * chunk, i1 = min(n, i0 + chunk);
+  double a = 0.0, w = 0.0;
+  for (int i = i0 + sl; i < i1; i += SL) {
+    const size_t e = (size_t)i * K + c;
+    const double z = (double)zs[e];
+    a += r[e] * z;
+    if (q) w += z * q[e];
+  }
+  col_partial<K>(a, red, p1, b);
+  if (q) col_partial<K>(w, red, p2, b);
 }
 
 template <int K>
@@ -391,6 +415,16 @@ __global__ void k_update_p_k(long long ne, double *__restrict__ p, const double 
   if (e >= ne) return;
   const int c = (int)(e % K);
   p[e] = SC(c, BN_S_BETA) * p[e] + z[e];
+}
+
+// the same with z held in FP32
+template <int K>
+__global__ void k_update_p_k_s(long long ne, double *__restrict__ p, const float *__restrict__ zs,
+                               const double *__restrict__ s) {
+  const long long e = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= ne) return;
+  const int c = (int)(e % K);
+  p[e] = SC(c, BN_S_BETA) * p[e] + (double)zs[e];
 }
 
 // Per-column scalar logic, one block per column (block c sums column c's
@@ -760,6 +794,25 @@ void bn_update_xr_k(int K, int n, double *x, const double *p, double *r, const d
   const int nb = bn_block_nb(n), chunk = (n + nb - 1) / nb;
   BN_K_DISPATCH(K, hipLaunchKernelGGL((k_update_xr_k<KK>), dim3(nb), dim3(BLOCK), 0, st, n, chunk, x, p, r, q, s,
                                       part));
+}
+
+void bn_update_xr_k_s(int K, int n, double *x, const double *p, double *r, const double *q, const double *s,
+                      double *part, float *rs, hipStream_t st) {
+  const int nb = bn_block_nb(n), chunk = (n + nb - 1) / nb;
+  BN_K_DISPATCH(K, hipLaunchKernelGGL((k_update_xr_k<KK, true>), dim3(nb), dim3(BLOCK), 0, st, n, chunk, x, p, r, q,
+                                      s, part, rs));
+}
+
+void bn_dot_rz_k_s(int K, int n, const double *r, const float *zs, const double *q, double *p1, double *p2,
+                   hipStream_t st) {
+  const int nb = bn_block_nb(n), chunk = (n + nb - 1) / nb;
+  BN_K_DISPATCH(K, hipLaunchKernelGGL((k_dot_rz_k_s<KK>), dim3(nb), dim3(BLOCK), 0, st, n, chunk, r, zs, q, p1, p2));
+}
+
+void bn_update_p_k_s(int K, int n, double *p, const float *zs, const double *s, hipStream_t st) {
+  const long long ne = (long long)n * K;
+  if (ne <= 0) return;
+  BN_K_DISPATCH(K, hipLaunchKernelGGL((k_update_p_k_s<KK>), g1(ne), dim3(BLOCK), 0, st, ne, p, zs, s));
 }
 
 void bn_update_p_k(int K, int n, double *p, const double *z, const double *s, hipStream_t st) {

@@ -271,6 +271,12 @@ void bn_update_xr_k(int K, int n, double *x, const double *p, double *r, const d
                     const double *s, double *part, hipStream_t stream);
 /* p = z + beta_c p */
 void bn_update_p_k(int K, int n, double *p, const double *z, const double *s, hipStream_t stream);
+/* FP32-V-cycle forms (as bn_pcg_update_xr_s, bn_dot_rz_s, bn_pcg_update_p_s) */
+void bn_update_xr_k_s(int K, int n, double *x, const double *p, double *r, const double *q,
+                      const double *s, double *part, float *rs, hipStream_t stream);
+void bn_dot_rz_k_s(int K, int n, const double *r, const float *zs, const double *q, double *p1,
+                   double *p2, hipStream_t stream);
+void bn_update_p_k_s(int K, int n, double *p, const float *zs, const double *s, hipStream_t stream);
 /* per-column scalar logic (modes as bn_scalar_finish) with the stop test
  * and freezing of converged columns in modes 2 and 4 */
 void bn_finish_k(int K, int which, int n, const double *p1, const double *p2, const double *p3,
