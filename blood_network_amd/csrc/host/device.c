@@ -324,8 +324,12 @@ static void tune_gs(struct bn_solver *s, const bn_tune_env *env, bn_dlevel *d, v
     const void *val = p ? (const void *)d->Ac.vs : (const void *)d->Ac.vd;
     const void *dinvc = p ? (const void *)d->dinvc_s : (const void *)d->dinvc_d;
     int best_l = 0;
-    float best = 1e30f;
-    for (int o = 0; o < 5; o++) {
+    float best = 1e30f, cand_ms[5];
+    /* two passes over the candidates, each keeping its better pass, so
+     * that a drift in clocks or cache state across the candidate order
+     * does not decide between two close choices */
+    for (int o10 = 0; o10 < 10; o10++) {
+      const int o = o10 % 5, pass = o10 / 5;
       const int lanes = opts[o];
       for (int r = -1; r < BN_TUNE_REPS; r++) {
         if (env->flush) bn_dot_partial(env->nflush, env->flush, env->flush, NULL, NULL, s->part1, NULL, s->stream);
@@ -344,10 +348,12 @@ static void tune_gs(struct bn_solver *s, const bn_tune_env *env, bn_dlevel *d, v
       }
       const char *lg = getenv("BN_TUNE_LOG");
       if (lg && lg[0] == '2')
-        fprintf(stderr, "  gs %c rows=%d colours=%d lanes=%d: %.1f us per cold sweep\n", vt, d->n, d->ncol, lanes,
-                1e3 * total / BN_TUNE_REPS);
-      if (total < best) best = total, best_l = lanes;
+        fprintf(stderr, "  gs %c rows=%d colours=%d lanes=%d pass %d: %.1f us per cold sweep\n", vt, d->n, d->ncol,
+                lanes, pass, 1e3 * total / BN_TUNE_REPS);
+      if (pass == 0 || total < cand_ms[o]) cand_ms[o] = total;
     }
+    for (int o = 0; o < 5; o++)
+      if (cand_ms[o] < best) best = cand_ms[o], best_l = opts[o];
     const char *lg = getenv("BN_TUNE_LOG");
     if (lg && (lg[0] == '1' || lg[0] == '2'))
       fprintf(stderr, "tune gs %c rows=%d colours=%d: lanes=%d (%.1f us per cold sweep)\n", vt, d->n, d->ncol, best_l,
