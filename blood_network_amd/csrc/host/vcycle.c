@@ -218,14 +218,17 @@ static void tri_solve(struct bn_solver *s, bn_dtri *t, char vt, const void *b, v
  * the flexible outer iteration).  The result goes to x, or, when zsum is
  * given, only zsum = zbase + result is written (fused into the last
  * sweep). */
+/* first_done: y already holds the lower triangle's first step b_i dinv_i
+ * (written by the residual kernel that formed b) */
 static void tri_jacobi(struct bn_solver *s, bn_dtri *t, char vt, const void *b, void *y, void *x,
-                       void *tmp, int sweeps, const void *zbase, void *zsum) {
+                       void *tmp, int sweeps, const void *zbase, void *zsum, int first_done) {
   const void *dinv = vt == 's' ? (const void *)t->dinv_s : (const void *)t->dinv_d;
   const int n = t->n;
   void *bufs[2] = {y, tmp};
   int cur = 0;
-  bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, NULL, bufs[cur], bn_lanes(&t->lo, vt), NULL,
-                 NULL, s->stream);
+  if (!first_done)
+    bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, NULL, bufs[cur], bn_lanes(&t->lo, vt), NULL,
+                   NULL, s->stream);
   /* the last lower sweep also writes the upper triangle's first step
    * (yy_i dinv_i) into x, which the upper sweeps start from */
   for (int k = 1; k < sweeps; k++) {
@@ -264,7 +267,7 @@ void bn_vc_tri_apply(struct bn_solver *s, bn_dtri *t, char vt, const void *b, vo
 
 void bn_vc_fine_ic(struct bn_solver *s, const void *b, void *x) {
   if (s->opt.fine_ic == BN_IC_JACOBI)
-    tri_jacobi(s, &s->fine_ic, s->vt, b, s->icy, x, s->lev[0].r, s->opt.ic_sweeps, NULL, NULL);
+    tri_jacobi(s, &s->fine_ic, s->vt, b, s->icy, x, s->lev[0].r, s->opt.ic_sweeps, NULL, NULL, 0);
   else
     tri_solve(s, &s->fine_ic, s->vt, b, s->icy, x, NULL, NULL);
 }
@@ -314,10 +317,19 @@ static void ichol_core(struct bn_solver *s, const void *gv, void *zv) {
   const bn_dcsr *A = &s->Asolve;
   /* u1 = backslash(g) from zero */
   bn_vc_backslash(s, 0, gv, s->u1, 1);
-  /* r1 = g - A u1; u2 = u1 + (L L^T)^{-1} r1 (u2 kept in icx's partner) */
-  apply(s, A, BN_OP_RESID, s->u1, s->r1, gv);
-  if (s->opt.fine_ic == BN_IC_JACOBI)
-    tri_jacobi(s, &s->fine_ic, vt, s->r1, s->icy, s->icx, s->lev[0].r, s->opt.ic_sweeps, s->u1, s->r2);
+  /* r1 = g - A u1; u2 = u1 + (L L^T)^{-1} r1 (u2 kept in icx's partner).
+   * With Jacobi-swept IC and the row-block kernel for A, the residual
+   * kernel also writes the lower triangle's first step r1_i / l_ii */
+  const int jac = (s->opt.fine_ic == BN_IC_JACOBI);
+  const int first = jac && (vt == 's' ? A->adapt_s : A->adapt) && A->blk && s->fine_ic.n == s->lev[0].n;
+  if (first)
+    bn_csr_resid_blocks_scaled(vt, A->nblk, A->blk, A->ia, A->ja, opv(A, vt), s->u1, s->r1, gv,
+                               vt == 's' ? (const void *)s->fine_ic.dinv_s : (const void *)s->fine_ic.dinv_d, s->icy,
+                               s->stream);
+  else
+    apply(s, A, BN_OP_RESID, s->u1, s->r1, gv);
+  if (jac)
+    tri_jacobi(s, &s->fine_ic, vt, s->r1, s->icy, s->icx, s->lev[0].r, s->opt.ic_sweeps, s->u1, s->r2, first);
   else
     tri_solve(s, &s->fine_ic, vt, s->r1, s->icy, s->icx, s->u1, s->r2);
   /* s->r2 now holds u2; r1 <- g - A u2 */

@@ -97,6 +97,10 @@ void bn_csr_apply_blocks(char vt, int op, int nblk, const int *blocks,
                          const int *ia, const int *ja, const void *val,
                          const void *x, void *y, const void *b,
                          hipStream_t stream);
+/* y = b - A x by the row-block kernel, also writing ys_r = y_r dsc_r */
+void bn_csr_resid_blocks_scaled(char vt, int nblk, const int *blocks, const int *ia, const int *ja,
+                                const void *val, const void *x, void *y, const void *b, const void *dsc,
+                                void *ys, hipStream_t stream);
 
 /* One multicolour Gauss-Seidel colour: rows lo..hi-1 of a level whose
  * rows are stored in colour order:

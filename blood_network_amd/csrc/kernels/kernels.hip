@@ -375,11 +375,16 @@ void launch_long(int op, int nlong, const int *rows, const int *ia, const int *j
 // fully coalesced segment, stages val * x[ja] in LDS, then sums each row
 // with tpr = pow2 <= 256 / rows threads.  Fixed summation order per row.
 // ---------------------------------------------------------------------
-template <typename T, int OP>
+// SCALED (RESID only): also write ys_r = y_r dsc_r, the first Jacobi step
+// from zero of a triangle whose right-hand side is this residual (the
+// product k_diag_scale would form)
+template <typename T, int OP, bool SCALED = false>
 __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__restrict__ blocks,
                                                       const int *__restrict__ ia, const int *__restrict__ ja,
                                                       const T *__restrict__ val, const T *__restrict__ x,
-                                                      T *__restrict__ y, const T *__restrict__ b) {
+                                                      T *__restrict__ y, const T *__restrict__ b,
+                                                      const T *__restrict__ dsc = nullptr,
+                                                      T *__restrict__ ys = nullptr) {
   constexpr int PER = BN_ROWBLOCK_NNZ / BLOCK;
   __shared__ T prod[BN_ROWBLOCK_NNZ + 1];  // + a spare word padding slots write
   const int bid = logical_block();
@@ -393,8 +398,11 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__res
     const double t = block_sum((double)s);
     if (threadIdx.x == 0) {
       if (OP == BN_OP_SET) y[r0] = (T)t;
-      else if (OP == BN_OP_RESID) y[r0] = b[r0] - (T)t;
-      else y[r0] = y[r0] + (T)t;
+      else if (OP == BN_OP_RESID) {
+        const T out = b[r0] - (T)t;
+        y[r0] = out;
+        if (SCALED) ys[r0] = out * dsc[r0];
+      } else y[r0] = y[r0] + (T)t;
     }
     return;
   }
@@ -436,8 +444,11 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__res
     for (int o = tpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (sub == 0 && r < r1) {
       if (OP == BN_OP_SET) y[r] = s;
-      else if (OP == BN_OP_RESID) y[r] = b[r] - s;
-      else y[r] = y[r] + s;
+      else if (OP == BN_OP_RESID) {
+        const T out = b[r] - s;
+        y[r] = out;
+        if (SCALED) ys[r] = out * dsc[r];
+      } else y[r] = y[r] + s;
     }
   }
 }
@@ -969,6 +980,20 @@ void bn_csr_apply_binned(char vt, int op, int n, const int *ia, const int *ja, c
     launch_long<double>(op, nlong, long_rows, ia, ja, (const double *)val, (const double *)x, (double *)y,
                         (const double *)b, stream);
   }
+}
+
+void bn_csr_resid_blocks_scaled(char vt, int nblk, const int *blocks, const int *ia, const int *ja, const void *val,
+                                const void *x, void *y, const void *b, const void *dsc, void *ys, hipStream_t stream) {
+  if (nblk <= 0) return;
+  const int g = grid_rows(nblk);
+  if (vt == 's')
+    hipLaunchKernelGGL((k_csr_blocks<float, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, stream, nblk, blocks, ia, ja,
+                       (const float *)val, (const float *)x, (float *)y, (const float *)b, (const float *)dsc,
+                       (float *)ys);
+  else
+    hipLaunchKernelGGL((k_csr_blocks<double, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, stream, nblk, blocks, ia,
+                       ja, (const double *)val, (const double *)x, (double *)y, (const double *)b,
+                       (const double *)dsc, (double *)ys);
 }
 
 void bn_csr_apply_blocks(char vt, int op, int nblk, const int *blocks, const int *ia, const int *ja,

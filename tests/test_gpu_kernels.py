@@ -116,6 +116,17 @@ def test_row_block_kernel_against_oracle(oracle, vt):
                                     db.ptr, None)
             got = dy.numpy().astype(np.float64)
             assert relerr(got, ref) < tol, (name, op)
+        # the scaled residual writes the plain residual bit for bit and its
+        # product with dsc (hub rows take the one-long-row path)
+        bits = np.uint64 if vt == "d" else np.uint32
+        dsc = (rng.random(n) + 0.5).astype(dt)
+        dy1, dy2, dys, dd = dev(np.zeros(n, dt)), dev(np.zeros(n, dt)), dev(np.zeros(n, dt)), dev(dsc)
+        lib.bn_csr_apply_blocks(vt.encode(), 1, nb, dblk.ptr, dia.ptr, dja.ptr, dv.ptr, dx.ptr, dy1.ptr, db.ptr, None)
+        lib.bn_csr_resid_blocks_scaled(vt.encode(), nb, dblk.ptr, dia.ptr, dja.ptr, dv.ptr, dx.ptr, dy2.ptr, db.ptr,
+                                       dd.ptr, dys.ptr, None)
+        g1, g2 = dy1.numpy(), dy2.numpy()
+        assert np.array_equal(g1.view(bits), g2.view(bits)), name
+        assert np.array_equal(dys.numpy().view(bits), (g2 * dsc).astype(dt).view(bits)), name
 
 
 def seg_bin(ia, long_len, seg=1024):
