@@ -182,6 +182,7 @@ static int upload_vec2(struct bn_solver *s, const double *h, int n, double **vd,
 /* diagonal inverse as the GS row update uses it: 1/a_ii, 1.0 if absent
  * (C_amg1/src/amg_cycle.c:57); l1 inverse 1/sum|a_ij| */
 static void diag_inverses(const dcsr_mat *A, double *dinv, double *l1inv) {
+#pragma omp parallel for schedule(static) if (A->nnz > 65536)
   for (int i = 0; i < A->row; i++) {
     double d = 1.0, l1 = 0.0;
     for (int k = A->ia[i]; k < A->ia[i + 1]; k++) {
@@ -636,6 +637,7 @@ static int upload_colour_rows(struct bn_solver *s, bn_dlevel *d, const dcsr_mat 
   double *dc = (double *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(double));
   /* row r of Ac holds the diagonal of level row cperm[r]; 1.0 when
    * absent, as diag_inverses */
+#pragma omp parallel for schedule(static) if (Ac.nnz > 65536)
   for (int r = 0; r < n; r++) {
     double dg = 1.0;
     for (int q = Ac.ia[r]; q < Ac.ia[r + 1]; q++)
