@@ -57,7 +57,8 @@ class Options(C.Structure):
                 ("max_colours", C.c_int), ("flexible", C.c_int),
                 ("stop", C.c_int), ("symmetrize", C.c_int), ("device", C.c_int),
                 ("verbose", C.c_int), ("graph", C.c_int),
-                ("block_smoother", C.c_int), ("chunk_rows", C.c_int), ("strength", C.c_int)]
+                ("block_smoother", C.c_int), ("chunk_rows", C.c_int), ("strength", C.c_int),
+                ("gs_exact", C.c_int)]
 
 
 class Nwk(C.Structure):
@@ -202,6 +203,10 @@ def _load():
         "bn_row_blocks": (C.c_int, [IP, C.c_int, C.c_int, C.c_int, IP]),
         "bn_colour_order": (C.c_int, [C.POINTER(CSR), IP, C.c_int, IP, C.POINTER(IP)]),
         "bn_level_sets": (C.c_int, [IP, IP, C.c_int, C.c_int, IP, C.POINTER(IP)]),
+        "bn_sweep_levels": (C.c_int, [C.POINTER(CSR), IP, C.c_int, IP, C.POINTER(IP)]),
+        "cf_gauss_seidel": (C.c_int, [C.c_int, IP, IP, DP, DP, DP, DP, C.c_int, C.c_int, DP]),
+        "bn_cf_gauss_seidel_device": (C.c_int, [C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, C.c_int]),
+        "bn_set_library_options": (None, [C.POINTER(Options)]),
         "bn_trsv_level": (None, [C.c_char, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
         "bn_gs_color": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
         "bn_jacobi_rows": (None, [C.c_char, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),

@@ -52,7 +52,9 @@ class AMGSolver:
     ("bnorm" | "relres"), ``smoother``, ``block_smoother`` (chunked l1 block
     Gauss-Seidel on every level, one launch per sweep) with ``chunk_rows``,
     ``strength`` ("c": C_amg1's rule | "mex": the MATLAB MEX rule, which
-    clamps -a_ij at 0).
+    clamps -a_ij at 0), ``gs_exact`` (SMOOTHER_CF_GS sweeps in the reference's
+    exact cf_order, level-scheduled, instead of the C-then-F multicolour
+    sweep; SMOOTHER_GS is always the exact natural-order sweep).
     """
 
     _COARSEN = {"rs": L.COARSEN_RS, "pmis": L.COARSEN_PMIS, "hmis": L.COARSEN_HMIS}
@@ -65,13 +67,13 @@ class AMGSolver:
                  coarsening=None, precision="fp64", coarse="auto", fine_ic="exact",
                  ic_sweeps=3, flexible=None, stop="bnorm", smoother="cf_gs", max_colours=32,
                  symmetrize=True, verbose=False, graph=True, block_smoother=False, chunk_rows=0,
-                 strength="c"):
+                 strength="c", gs_exact=False):
         self.ia, self.ja, self.val = _csr(ia, ja, val)
         self.n = len(self.ia) - 1
         plain = (coarsening is None and precision == "fp64" and coarse == "auto"
                  and fine_ic == "exact" and stop == "bnorm" and smoother == "cf_gs"
                  and flexible is None and symmetrize and not verbose and max_colours == 32
-                 and graph and not block_smoother and strength == "c")
+                 and graph and not block_smoother and strength == "c" and not gs_exact)
         if plain:
             h = L.lib.amg_solver_create(self.n, len(self.ja), L.ip(self.ia), L.ip(self.ja),
                                         L.dp(self.val), threshold, min_size, max_levels, nu)
@@ -100,6 +102,7 @@ class AMGSolver:
             if chunk_rows:
                 o.chunk_rows = int(chunk_rows)
             o.strength = {"c": L.STRENGTH_C, "mex": L.STRENGTH_MEX}[strength]
+            o.gs_exact = int(bool(gs_exact))
             h = L.lib.bn_solver_create(self.n, len(self.ja), L.ip(self.ia), L.ip(self.ja),
                                        L.dp(self.val), C.byref(o))
         if not h:

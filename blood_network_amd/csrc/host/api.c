@@ -98,6 +98,8 @@ void bn_device_release(struct bn_solver *s) {
   for (int l = 0; l < AMG_MAX_LEVELS; l++) {
     free(s->lev[l].col_ptr);
     free(s->lev[l].perm_h);
+    bn_sweep_free_host(&s->lev[l].xs[0]);
+    bn_sweep_free_host(&s->lev[l].xs[1]);
   }
   memset(s->lev, 0, sizeof(s->lev));
   free_tri_host(&s->fine_ic);
@@ -241,12 +243,24 @@ void *amg_solver_create(int n, int nnz, const int *ia, const int *ja, const doub
   return bn_solver_create(n, nnz, ia, ja, val, &o);
 }
 
+/* Upload after update / set_matrix.  Those reference entry points always
+ * return 0 (C_amg1/src/amg_wrapper.c:148,198), so a failed upload is
+ * reported on stderr and through bn_last_error(); the next solve tries the
+ * upload again and returns -maxiter if it still fails. */
+static void reupload(struct bn_solver *s, const char *who) {
+  if (ensure_device(s)) fprintf(stderr, "%s: device upload failed: %s\n", who, g_err);
+}
+
 /* C_amg1/src/amg_wrapper.c:114-149: same pattern, values rebuilt on the
  * stored split and P pattern, new fine IC(0); the device mirror is
  * uploaded again. */
 int amg_solver_update(void *h, int n, int nnz, const int *ia, const int *ja, const double *val) {
   struct bn_solver *s = (struct bn_solver *)h;
-  if (!s || s->borrowed) return BN_ERR_ARG;
+  if (!s || s->borrowed) {
+    bn_set_error("amg_solver_update: not a solver handle");
+    fprintf(stderr, "amg_solver_update: %s\n", g_err);
+    return 0;
+  }
   dcsr_mat nw;
   copy_input(n, nnz, ia, ja, val, s->opt.symmetrize, &nw);
   const double t0 = now_ms();
@@ -258,7 +272,7 @@ int amg_solver_update(void *h, int n, int nnz, const int *ia, const int *ja, con
   s->last.setup_ms = now_ms() - t0;
   int was = s->dev_ready;
   bn_device_release(s);
-  if (was || bn_device_ok()) return ensure_device(s) ? BN_ERR_HIP : 0;
+  if (was || bn_device_ok()) reupload(s, "amg_solver_update");
   return 0;
 }
 
@@ -266,7 +280,11 @@ int amg_solver_update(void *h, int n, int nnz, const int *ia, const int *ja, con
  * IC(0), keep the hierarchy (two-matrix preconditioning) */
 int amg_solver_set_matrix(void *h, int n, int nnz, const int *ia, const int *ja, const double *val) {
   struct bn_solver *s = (struct bn_solver *)h;
-  if (!s || s->borrowed) return BN_ERR_ARG;
+  if (!s || s->borrowed) {
+    bn_set_error("amg_solver_set_matrix: not a solver handle");
+    fprintf(stderr, "amg_solver_set_matrix: %s\n", g_err);
+    return 0;
+  }
   dcsr_mat nw;
   copy_input(n, nnz, ia, ja, val, s->opt.symmetrize, &nw);
   ichol_free(&s->L_own);
@@ -275,7 +293,7 @@ int amg_solver_set_matrix(void *h, int n, int nnz, const int *ia, const int *ja,
   ichol_compute(s->A, &s->L_own);
   int was = s->dev_ready;
   bn_device_release(s);
-  if (was || bn_device_ok()) return ensure_device(s) ? BN_ERR_HIP : 0;
+  if (was || bn_device_ok()) reupload(s, "amg_solver_set_matrix");
   return 0;
 }
 
@@ -308,11 +326,18 @@ int bn_solver_solve_info(void *h, const double *b, double *x, double tol, int ma
   return it;
 }
 
-/* C_amg1/src/amg_wrapper.c:214-225 */
+/* C_amg1/src/amg_wrapper.c:214-225.  Returns the iteration count or
+ * -maxiter, as the reference does: a solve that cannot run (no device, a
+ * HIP failure, a NULL argument) counts as not converged, leaves x as it
+ * was, and says why on stderr and through bn_last_error().  The bn_*
+ * entry points return the BN_ERR_* code instead. */
 int amg_solver_solve(void *h, const double *b, double *x, double tol, int maxiter, int print_level) {
   int rc = bn_solver_solve_info(h, b, x, tol, maxiter, print_level, NULL);
-  if (rc == BN_ERR_NO_DEVICE || rc == BN_ERR_HIP)
+  if (rc == BN_ERR_NO_DEVICE || rc == BN_ERR_HIP || rc == BN_ERR_ARG) {
+    if (rc == BN_ERR_ARG && !g_err[0]) bn_set_error("NULL handle or vector");
     fprintf(stderr, "amg_solver_solve: %s\n", bn_last_error());
+    return -maxiter;
+  }
   return rc;
 }
 
@@ -331,6 +356,7 @@ int bn_solver_precond(void *h, const double *g, double *z, int which) {
   int rc = ensure_device(s);
   if (rc) return rc;
   if (which == 0 && !s->L) return BN_ERR_ARG;
+  if ((rc = bn_vc_prepare(s))) return rc;
   const int n = s->lev[0].n;
   double *db, *dx;
   if (stage_in(s, g, NULL, &db, &dx)) return BN_ERR_HIP;
@@ -366,6 +392,9 @@ int bn_solver_set_mode(void *h, int precision, int fine_ic, int ic_sweeps, int f
 void amg_solver_set_smoother(void *h, int smoother, double omega) {
   struct bn_solver *s = (struct bn_solver *)h;
   if (!s) return;
+  /* captured step graphs hold the smoother's launches and omega as kernel
+   * arguments: a change drops them, the next solve captures again */
+  if (smoother != s->opt.smoother || (smoother == SMOOTHER_JACOBI && omega != s->opt.omega)) bn_drop_graphs(s);
   s->opt.smoother = smoother;
   s->amg->smoother = smoother;
   if (smoother == SMOOTHER_JACOBI) {

@@ -129,7 +129,7 @@ static void smooth_k(struct bn_solver *s, int lev, const void *B, void *X, int b
   const int nu = s->opt.nu > 0 ? s->opt.nu : 1;
   const int n = d->n;
   const long long ne = (long long)n * K;
-  const int gs = d->use_gs && (s->opt.smoother == SMOOTHER_CF_GS || s->opt.smoother == SMOOTHER_GS);
+  const int gs = d->use_gs && s->opt.smoother == SMOOTHER_CF_GS;
   if (gs) {
     const int fused_zero = x_zero && !(nu == 1 && zsum) && d->ncol > 0;
     if (x_zero && !fused_zero) bn_fill(vt, (int)ne, X, 0.0, s->stream);
@@ -438,6 +438,11 @@ static int block_check(struct bn_solver *s, int k) {
     }
   if (s->opt.fine_ic == BN_IC_EXACT) {
     bn_set_error("block solve: the fine IC(0) must be BN_IC_JACOBI or BN_IC_NONE");
+    return BN_ERR_ARG;
+  }
+  if (bn_vc_exact_kind(s) >= 0) {
+    bn_set_error("block solve: exact-order Gauss-Seidel sweeps (SMOOTHER_GS, gs_exact) are not supported; "
+                 "use the multicolour SMOOTHER_CF_GS, SMOOTHER_L1 or SMOOTHER_JACOBI");
     return BN_ERR_ARG;
   }
   return 0;

@@ -50,6 +50,15 @@ typedef struct {
   int *ptr_f, *ptr_b;     /* host: level boundaries */
 } bn_dtri;
 
+/* exact-order Gauss-Seidel sweep of a level: the rows of the sequential
+ * sweep grouped by dependency level, forward and backward (built on first
+ * use by bn_vc_prepare) */
+typedef struct {
+  int nf, nb;           /* levels of the forward / backward sweep */
+  int *rows_f, *rows_b; /* device: rows grouped by level */
+  int *ptr_f, *ptr_b;   /* host: level boundaries (NULL: not built) */
+} bn_sweep;
+
 typedef struct {
   int n, nc;
   bn_dcsr A;            /* level operator (relabelled) */
@@ -82,6 +91,8 @@ typedef struct {
   double *bdinv_d;      /* 1 / (a_ii + sum_{j outside the chunk} |a_ij|) */
   float *bdinv_s;
   void *x2;             /* second buffer for sweeps from a nonzero start */
+  bn_sweep xs[2];       /* exact sweeps: [0] natural order (SMOOTHER_GS),
+                           [1] cf_order (SMOOTHER_CF_GS with gs_exact) */
 } bn_dlevel;
 
 /* device work space of the block (multi-right-hand-side) solve for K
@@ -236,10 +247,21 @@ int bn_coarse_inverse_device(struct bn_solver *s, const dcsr_mat *A, double *cin
 
 /* device.c */
 int bn_device_upload(struct bn_solver *s);
+/* host level sets of the sequential sweep of A in `order` (NULL: natural)
+ * and their device row lists; 0 or a BN_ERR_* code */
+int bn_sweep_upload(struct bn_solver *s, const dcsr_mat *A, const int *order, bn_sweep *w);
+void bn_sweep_free_host(bn_sweep *w);
+/* default lanes per row for an operator of that size */
+int bn_lanes_for(long long nnz, long long rows);
 void bn_device_release(struct bn_solver *s);
 int bn_device_ok(void);
 
 /* vcycle.c */
+/* Builds what the current smoother needs before any V-cycle is enqueued
+ * or captured (the exact sweeps' schedules); 0 or a BN_ERR_* code. */
+int bn_vc_prepare(struct bn_solver *s);
+/* -1: the smoother runs no exact-order sweep, else the bn_sweep slot */
+int bn_vc_exact_kind(const struct bn_solver *s);
 void bn_vc_backslash(struct bn_solver *s, int lev, const void *b, void *x, int x_zero);
 void bn_vc_fwdslash(struct bn_solver *s, int lev, const void *b, void *x, int x_zero);
 void bn_vc_ichol_precond(struct bn_solver *s, const double *g, double *z);

@@ -65,6 +65,8 @@ static int lanes_for(long long nnz, long long rows) {
   return l;
 }
 
+int bn_lanes_for(long long nnz, long long rows) { return lanes_for(nnz, rows); }
+
 static int cmp_desc_ll(const void *a, const void *b) {
   const long long x = *(const long long *)a, y = *(const long long *)b;
   return (x < y) - (x > y);
@@ -567,6 +569,54 @@ int bn_tri_upload(struct bn_solver *s, const ichol_factor *F, bn_dtri *t) {
 }
 
 int bn_csr_upload(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) { return upload_csr(s, m, d); }
+
+void bn_sweep_free_host(bn_sweep *w) {
+  free(w->ptr_f);
+  free(w->ptr_b);
+  memset(w, 0, sizeof(*w));
+}
+
+int bn_sweep_upload(struct bn_solver *s, const dcsr_mat *A, const int *order, bn_sweep *w) {
+  const int n = A->row;
+  int *rf = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  int *rb = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  int *pf = NULL, *pb = NULL;
+  const int nf = bn_sweep_levels(A, order, 0, rf, &pf);
+  const int nb = nf < 0 ? -1 : bn_sweep_levels(A, order, 1, rb, &pb);
+  int rc = 0;
+  if (nf < 0 || nb < 0) {
+    bn_set_error("Gauss-Seidel order is not a permutation of the %d rows", n);
+    rc = BN_ERR_ARG;
+  }
+  int *df = NULL, *db = NULL;
+  if (!rc) {
+    df = (int *)dmalloc(s, (size_t)(n > 0 ? n : 1) * sizeof(int));
+    db = (int *)dmalloc(s, (size_t)(n > 0 ? n : 1) * sizeof(int));
+    if (!df || !db) {
+      bn_set_error("device out of memory (sweep schedule of %d rows)", n);
+      rc = BN_ERR_HIP;
+    }
+  }
+  if (!rc && (hipMemcpy(df, rf, (size_t)n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+              hipMemcpy(db, rb, (size_t)n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)) {
+    bn_set_error("copy of a sweep schedule to the device failed");
+    rc = BN_ERR_HIP;
+  }
+  free(rf);
+  free(rb);
+  if (rc) {
+    free(pf);
+    free(pb);
+    return rc;
+  }
+  w->nf = nf;
+  w->nb = nb;
+  w->rows_f = df;
+  w->rows_b = db;
+  w->ptr_f = pf;
+  w->ptr_b = pb;
+  return 0;
+}
 
 /* Chunks of the block smoother: at most max_rows rows each; at least 256
  * chunks (one per CU) while chunks keep >= 64 rows; above 256 a multiple

@@ -143,15 +143,21 @@ int bn_solver_set_values_device(void *h, int nnz_in, const double *d_val) {
   const size_t nnz = (size_t)s->A->nnz;
   if (s->asolve_shared) {
     /* the level-0 V-cycle operator keeps its values: the solve matrix
-       gets FP64 values of its own (pattern arrays stay shared) */
-    void *v = NULL;
-    if (hipMalloc(&v, (nnz ? nnz : 1) * sizeof(double)) != hipSuccess) {
+       gets FP64 and FP32 values of its own (pattern arrays stay shared;
+       the FP32 copy serves the residuals inside an FP32 preconditioner,
+       which use the solve matrix as amg_ichol_precond.c:37,44 do) */
+    void *v = NULL, *v32 = NULL;
+    if (hipMalloc(&v, (nnz ? nnz : 1) * sizeof(double)) != hipSuccess ||
+        hipMalloc(&v32, (nnz ? nnz : 1) * sizeof(float)) != hipSuccess) {
+      if (v) hipFree(v);
       bn_set_error("device out of memory for the solve-matrix values");
       return BN_ERR_HIP;
     }
     bn_track_device(s, v);
+    bn_track_device(s, v32);
     bn_drop_graphs(s);
     s->Asolve.vd = (double *)v;
+    s->Asolve.vs = (float *)v32;
     s->asolve_shared = 0;
   }
   hipStream_t st = s->stream;
@@ -161,6 +167,8 @@ int bn_solver_set_values_device(void *h, int nnz_in, const double *d_val) {
   } else {
     bn_permute((int)nnz, s->vperm_d, d_val, s->Asolve.vd, 0, st);
   }
+  /* the FP32 copy follows (same rounding as at upload) */
+  bn_rb_to_float((int)nnz, s->Asolve.vd, s->Asolve.vs, st);
   return hipStreamSynchronize(st) == hipSuccess ? 0 : BN_ERR_HIP;
 }
 

@@ -2,11 +2,20 @@
  * libentry.c - the library-level solve entry points of C_amg1
  * (include/amg.h:81-96,111-115) over the caller's host structures.
  *
- * A device mirror of the caller's amg_data / dcsr_mat / ichol_factor is
- * built on first use and cached by address; amg_free, amg_rebuild_values
- * and ichol_free drop the entries that reference the structure they
- * change.  Vectors cross the host/device boundary on every call (these
- * entry points take host pointers); the handle API and
+ * A device mirror of the caller's amg_data / ichol_factor is built on
+ * first use and cached by address.  The library's own routines that change
+ * such a structure (amg_build_hierarchy, amg_rebuild_values, amg_free,
+ * ichol_compute, ichol_free) drop the mirrors that reference it.  A caller's
+ * dcsr_mat has no library routine that frees or changes it (dcsr_mat_free
+ * is header-only), so every call compares it with the host copy its mirror
+ * was made from: a new pattern builds the mirror again, new values are
+ * copied to the device, as the reference reads A afresh on every call.
+ *
+ * Each cached mirror has a lock held for the whole of a call that uses it,
+ * so two threads may call these entry points on the same structures (they
+ * run one after the other) or on different ones (they run concurrently, one
+ * HIP stream per mirror).  Vectors cross the host/device boundary on every
+ * call (these entry points take host pointers); the handle API and
  * bn_solver_solve_device avoid that.
  */
 #include <math.h>
@@ -31,15 +40,22 @@ typedef struct entry {
   int kind;
   const void *k1, *k2, *k3;
   struct bn_solver *s;
-  bn_dtri tri;       /* KIND_TRI */
-  double *v[6];      /* KIND_TRI/KIND_OP device vectors */
-  double *hv;        /* pinned host staging */
+  pthread_mutex_t mu;  /* held for the whole of a call on this mirror */
+  dcsr_mat held;       /* host copy of the caller's dcsr_mat the mirror holds */
+  unsigned gen;        /* library-option generation it was built under */
+  bn_dtri tri;         /* KIND_TRI */
+  double *v[6];        /* KIND_TRI/KIND_OP device vectors */
+  double *hv;          /* pinned host staging */
   int n;
   struct entry *next;
 } entry;
 
 static pthread_mutex_t g_mu = PTHREAD_MUTEX_INITIALIZER;
 static entry *g_head = NULL;
+/* options of the hidden solvers behind library-level mirrors */
+static bn_options g_lib_opt;
+static int g_lib_opt_set = 0;
+static unsigned g_lib_gen = 0;
 
 static void fatal(const char *what) {
   fprintf(stderr, "libbnamg: %s: %s\n", what, bn_last_error());
@@ -59,7 +75,17 @@ static void destroy(entry *e) {
       free(e->s);
     }
   }
+  bn_csr_free(&e->held);
+  pthread_mutex_destroy(&e->mu);
   free(e);
+}
+
+static void unlink_entry(entry *e) {
+  for (entry **pp = &g_head; *pp; pp = &(*pp)->next)
+    if (*pp == e) {
+      *pp = e->next;
+      return;
+    }
 }
 
 /* drop every cached mirror that references p */
@@ -70,6 +96,8 @@ static void drop(const void *p) {
     entry *e = *pp;
     if (e->k1 == p || e->k2 == p || e->k3 == p) {
       *pp = e->next;
+      pthread_mutex_lock(&e->mu); /* wait for a call in flight on it */
+      pthread_mutex_unlock(&e->mu);
       destroy(e);
     } else {
       pp = &e->next;
@@ -80,10 +108,60 @@ static void drop(const void *p) {
 
 void bn_cache_drop(const void *p) { drop(p); }
 
+/* Options for the solvers behind the library-level entry points (half
+ * cycles, V-cycle, AMG+IC preconditioner on a caller's amg_data): NULL
+ * restores bn_options_default.  The smoother, nu, omega and coarsening
+ * always come from the amg_data.  Mirrors built under other options are
+ * dropped. */
+void bn_set_library_options(const bn_options *o) {
+  pthread_mutex_lock(&g_mu);
+  if (o) {
+    g_lib_opt = *o;
+    g_lib_opt_set = 1;
+  } else {
+    g_lib_opt_set = 0;
+  }
+  g_lib_gen++;
+  entry **pp = &g_head;
+  while (*pp) {
+    entry *e = *pp;
+    if (e->kind == KIND_HIER) {
+      *pp = e->next;
+      pthread_mutex_lock(&e->mu);
+      pthread_mutex_unlock(&e->mu);
+      destroy(e);
+    } else {
+      pp = &e->next;
+    }
+  }
+  pthread_mutex_unlock(&g_mu);
+}
+
 static entry *find(int kind, const void *k1, const void *k2, const void *k3) {
   for (entry *e = g_head; e; e = e->next)
     if (e->kind == kind && e->k1 == k1 && e->k2 == k2 && e->k3 == k3) return e;
   return NULL;
+}
+
+static entry *new_entry(int kind, const void *k1, const void *k2, const void *k3) {
+  entry *e = (entry *)calloc(1, sizeof(entry));
+  if (!e) fatal("out of host memory");
+  e->kind = kind;
+  e->k1 = k1;
+  e->k2 = k2;
+  e->k3 = k3;
+  pthread_mutex_init(&e->mu, NULL);
+  return e;
+}
+
+/* 0: A is the matrix `held` copies; 1: same pattern, other values;
+ * 2: another size or pattern */
+static int csr_change(const dcsr_mat *held, const dcsr_mat *A) {
+  if (held->row != A->row || held->col != A->col || held->nnz != A->nnz) return 2;
+  if (memcmp(held->ia, A->ia, ((size_t)A->row + 1) * sizeof(int)) ||
+      memcmp(held->ja, A->ja, (size_t)A->nnz * sizeof(int)))
+    return 2;
+  return memcmp(held->val, A->val, (size_t)A->nnz * sizeof(double)) ? 1 : 0;
 }
 
 static int new_stream_solver(struct bn_solver **out) {
@@ -110,20 +188,32 @@ static double *dvec(struct bn_solver *s, int n) {
   return (double *)p;
 }
 
-/* hierarchy mirror (optionally with solve matrix A and fine factor L) */
+/* Hierarchy mirror (optionally with solve matrix A and fine factor L),
+ * returned locked: the caller unlocks e->mu when its call is done. */
 static entry *hier_entry(const amg_data *amg, const dcsr_mat *A, const ichol_factor *L) {
   pthread_mutex_lock(&g_mu);
   entry *e = find(KIND_HIER, amg, A, L);
+  if (e && (e->gen != g_lib_gen || (A && csr_change(&e->held, A)))) {
+    /* built under other options, or the caller's solve matrix changed */
+    unlink_entry(e);
+    pthread_mutex_lock(&e->mu);
+    pthread_mutex_unlock(&e->mu);
+    destroy(e);
+    e = NULL;
+  }
   if (e) {
+    pthread_mutex_lock(&e->mu);
     pthread_mutex_unlock(&g_mu);
     return e;
   }
   struct bn_solver *s = (struct bn_solver *)calloc(1, sizeof(struct bn_solver));
-  bn_options_default(&s->opt);
+  if (g_lib_opt_set) s->opt = g_lib_opt;
+  else bn_options_default(&s->opt);
   s->opt.smoother = amg->smoother;
   s->opt.nu = amg->nu > 0 ? amg->nu : 1;
   s->opt.omega = amg->omega_jacobi;
   s->opt.coarsening = amg->coarsening;
+  s->opt.precision = BN_PREC_FP64;
   s->amg = (amg_data *)amg;
   s->A = A ? A : &amg->levels[0].A;
   s->L = L;
@@ -133,21 +223,32 @@ static entry *hier_entry(const amg_data *amg, const dcsr_mat *A, const ichol_fac
     pthread_mutex_unlock(&g_mu);
     fatal("device upload of the AMG hierarchy failed");
   }
-  e = (entry *)calloc(1, sizeof(entry));
-  e->kind = KIND_HIER;
-  e->k1 = amg;
-  e->k2 = A;
-  e->k3 = L;
+  e = new_entry(KIND_HIER, amg, A, L);
   e->s = s;
+  e->gen = g_lib_gen;
   e->n = amg->levels[0].n;
+  if (A) bn_csr_copy(A, &e->held);
   if (hipHostMalloc((void **)&e->hv, 2 * (size_t)e->n * sizeof(double), 0) != hipSuccess) {
     pthread_mutex_unlock(&g_mu);
     fatal("pinned allocation failed");
   }
   e->next = g_head;
   g_head = e;
+  pthread_mutex_lock(&e->mu);
   pthread_mutex_unlock(&g_mu);
   return e;
+}
+
+/* the smoother settings of the amg_data, and what they need on the device */
+static void use_amg_settings(struct bn_solver *s, const amg_data *amg, int nu) {
+  const int sm = amg->smoother;
+  const double om = amg->omega_jacobi;
+  if (sm != s->opt.smoother || (sm == SMOOTHER_JACOBI && om != s->opt.omega)) bn_drop_graphs(s);
+  s->opt.nu = nu > 0 ? nu : 1;
+  s->opt.smoother = sm;
+  s->opt.omega = om;
+  s->vt = 'd';
+  if (bn_vc_prepare(s)) fatal("device set-up of the smoother failed");
 }
 
 /* copy host vector into level-0 numbering on the device */
@@ -168,12 +269,13 @@ static void from_dev(struct bn_solver *s, double *stage_h, const double *d_in, i
  * amg.h:81-86 half cycles and V-cycle
  * ------------------------------------------------------------------- */
 static void half_cycle(const amg_data *amg, INT lev, const REAL *b, REAL *x, INT nu, int fwd) {
+  if (lev < 0 || lev >= amg->num_levels) {
+    bn_set_error("level %d outside the hierarchy's %d levels", lev, amg->num_levels);
+    fatal(fwd ? "amg_fwdslash" : "amg_backslash");
+  }
   entry *e = hier_entry(amg, NULL, NULL);
   struct bn_solver *s = e->s;
-  s->opt.nu = nu > 0 ? nu : 1;
-  s->opt.smoother = amg->smoother;
-  s->opt.omega = amg->omega_jacobi;
-  s->vt = 'd';
+  use_amg_settings(s, amg, nu);
   const int nl = s->lev[lev].n;
   double *hb = e->hv, *hx = e->hv + e->n;
   /* level `lev` vectors in that level's numbering */
@@ -190,6 +292,7 @@ static void half_cycle(const amg_data *amg, INT lev, const REAL *b, REAL *x, INT
   hipMemcpyAsync(hx, tx, (size_t)nl * sizeof(double), hipMemcpyDeviceToHost, s->stream);
   if (hipStreamSynchronize(s->stream) != hipSuccess) fatal("device failure in a V-cycle half");
   memcpy(x, hx, (size_t)nl * sizeof(double));
+  pthread_mutex_unlock(&e->mu);
 }
 
 void amg_backslash(const amg_data *amg, INT lev, const REAL *b, REAL *x, INT nu) {
@@ -203,26 +306,22 @@ void amg_fwdslash(const amg_data *amg, INT lev, const REAL *b, REAL *x, INT nu) 
 void amg_vcycle_precond(const amg_data *amg, const REAL *g, REAL *x) {
   entry *e = hier_entry(amg, NULL, NULL);
   struct bn_solver *s = e->s;
-  s->opt.nu = amg->nu > 0 ? amg->nu : 1;
-  s->opt.smoother = amg->smoother;
-  s->opt.omega = amg->omega_jacobi;
-  s->vt = 'd';
+  use_amg_settings(s, amg, amg->nu);
   to_dev(s, e->hv, g, e->n, s->q, s->r);
   bn_vc_vcycle_precond(s, s->r, s->z);
   from_dev(s, e->hv, s->z, e->n, s->q, x);
+  pthread_mutex_unlock(&e->mu);
 }
 
 void amg_ichol_precond(const amg_data *amg, const dcsr_mat *A, const ichol_factor *L, const REAL *g,
                        REAL *x) {
   entry *e = hier_entry(amg, A, L);
   struct bn_solver *s = e->s;
-  s->opt.nu = amg->nu > 0 ? amg->nu : 1;
-  s->opt.smoother = amg->smoother;
-  s->opt.omega = amg->omega_jacobi;
-  s->vt = 'd';
+  use_amg_settings(s, amg, amg->nu);
   to_dev(s, e->hv, g, e->n, s->q, s->r);
   bn_vc_ichol_precond(s, s->r, s->z);
   from_dev(s, e->hv, s->z, e->n, s->q, x);
+  pthread_mutex_unlock(&e->mu);
 }
 
 /* ---------------------------------------------------------------------
@@ -232,9 +331,7 @@ void ichol_solve(const ichol_factor *L, const REAL *b, REAL *x) {
   pthread_mutex_lock(&g_mu);
   entry *e = find(KIND_TRI, L, NULL, NULL);
   if (!e) {
-    e = (entry *)calloc(1, sizeof(entry));
-    e->kind = KIND_TRI;
-    e->k1 = L;
+    e = new_entry(KIND_TRI, L, NULL, NULL);
     e->n = L->n;
     if (new_stream_solver(&e->s) || bn_tri_upload(e->s, L, &e->tri)) {
       pthread_mutex_unlock(&g_mu);
@@ -245,6 +342,7 @@ void ichol_solve(const ichol_factor *L, const REAL *b, REAL *x) {
     e->next = g_head;
     g_head = e;
   }
+  pthread_mutex_lock(&e->mu);
   pthread_mutex_unlock(&g_mu);
   struct bn_solver *s = e->s;
   const int n = e->n;
@@ -254,21 +352,29 @@ void ichol_solve(const ichol_factor *L, const REAL *b, REAL *x) {
   hipMemcpyAsync(e->hv, e->v[2], (size_t)n * sizeof(double), hipMemcpyDeviceToHost, s->stream);
   if (hipStreamSynchronize(s->stream) != hipSuccess) fatal("device failure in ichol_solve");
   memcpy(x, e->hv, (size_t)n * sizeof(double));
+  pthread_mutex_unlock(&e->mu);
 }
 
 /* ---------------------------------------------------------------------
  * amg_pcg (amg.h:111): PCG with a host preconditioner callback; SpMV and
  * BLAS-1 on the device, r and z cross to the host once per step.
  * ------------------------------------------------------------------- */
-INT amg_pcg(const dcsr_mat *A, const REAL *b, REAL *x,
-            void (*precond)(const void *data, const REAL *g, REAL *Bg), const void *precond_data,
-            REAL tol, INT maxiter, INT print_level) {
+/* the device copy of A for amg_pcg, locked; built again when A's size or
+ * pattern changed since the last call, values refreshed when only they
+ * changed */
+static entry *op_entry(const dcsr_mat *A) {
   pthread_mutex_lock(&g_mu);
   entry *e = find(KIND_OP, A, NULL, NULL);
+  int change = e ? csr_change(&e->held, A) : 2;
+  if (e && change == 2) {
+    unlink_entry(e);
+    pthread_mutex_lock(&e->mu);
+    pthread_mutex_unlock(&e->mu);
+    destroy(e);
+    e = NULL;
+  }
   if (!e) {
-    e = (entry *)calloc(1, sizeof(entry));
-    e->kind = KIND_OP;
-    e->k1 = A;
+    e = new_entry(KIND_OP, A, NULL, NULL);
     e->n = A->row;
     if (new_stream_solver(&e->s) || bn_csr_upload(e->s, A, &e->s->Asolve)) {
       pthread_mutex_unlock(&g_mu);
@@ -279,10 +385,27 @@ INT amg_pcg(const dcsr_mat *A, const REAL *b, REAL *x,
     e->s->part1 = dvec(e->s, BN_RED_BLOCKS);
     e->s->part2 = dvec(e->s, BN_RED_BLOCKS);
     hipHostMalloc((void **)&e->hv, 2 * (size_t)(A->row > 0 ? A->row : 1) * sizeof(double), 0);
+    bn_csr_copy(A, &e->held);
     e->next = g_head;
     g_head = e;
+    change = 0;
   }
+  pthread_mutex_lock(&e->mu);
   pthread_mutex_unlock(&g_mu);
+  if (change == 1) {
+    /* same pattern: the new values replace the device copy's FP64 values
+     * (the only ones amg_pcg multiplies by) */
+    memcpy(e->held.val, A->val, (size_t)A->nnz * sizeof(double));
+    if (hipMemcpy(e->s->Asolve.vd, A->val, (size_t)A->nnz * sizeof(double), hipMemcpyHostToDevice) != hipSuccess)
+      fatal("amg_pcg: copy of new matrix values failed");
+  }
+  return e;
+}
+
+INT amg_pcg(const dcsr_mat *A, const REAL *b, REAL *x,
+            void (*precond)(const void *data, const REAL *g, REAL *Bg), const void *precond_data,
+            REAL tol, INT maxiter, INT print_level) {
+  entry *e = op_entry(A);
   struct bn_solver *s = e->s;
   hipStream_t st = s->stream;
   const int n = e->n;
@@ -342,5 +465,6 @@ INT amg_pcg(const dcsr_mat *A, const REAL *b, REAL *x,
   hipMemcpyAsync(hr, dx, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st);
   if (hipStreamSynchronize(st) != hipSuccess) fatal("device failure in amg_pcg");
   memcpy(x, hr, (size_t)n * sizeof(double));
+  pthread_mutex_unlock(&e->mu);
   return it;
 }

@@ -60,7 +60,7 @@ static void enqueue_step(struct bn_solver *s) {
 
 static int mode_key(const struct bn_solver *s) {
   return (s->vt == 's') | (s->opt.fine_ic << 1) | (s->opt.flexible << 3) | (s->opt.smoother << 4) |
-         (s->opt.nu << 8) | (s->opt.ic_sweeps << 16);
+         (s->opt.nu << 8) | (s->opt.ic_sweeps << 16) | ((s->opt.gs_exact != 0) << 28);
 }
 
 /* The captured step for the current mode, capturing it on first use.
@@ -101,6 +101,9 @@ int bn_pcg_run(struct bn_solver *s, const double *d_b, double *d_x, double tol, 
   double *sc = s->scal;
   double *hs = s->host_scal;
   const int relres_stop = (s->opt.stop == BN_STOP_RELRES);
+  /* schedules the smoother needs are built before anything is captured */
+  const int prc = bn_vc_prepare(s);
+  if (prc) return prc;
   hipGraphExec_t gexec = step_graph(s);
 
   hipEventRecord(s->ev0, st);

@@ -708,10 +708,12 @@ int bn_solver_rebuild_values_device(void *h, int nnz_in, const double *d_val) {
       bn_rb_dinv_map(d->n, d->Ac.ia, d->Ac.ja, d->Ac.vd, d->cmap_d, d->dinvc_d, d->dinvc_s, st);
     }
   }
-  if (!s->asolve_shared &&
-      hipMemcpyAsync(s->Asolve.vd, s->lev[0].A.vd, (size_t)nz0 * sizeof(double), hipMemcpyDeviceToDevice, st) !=
-          hipSuccess)
-    return BN_ERR_HIP;
+  if (!s->asolve_shared) {
+    if (hipMemcpyAsync(s->Asolve.vd, s->lev[0].A.vd, (size_t)nz0 * sizeof(double), hipMemcpyDeviceToDevice, st) !=
+        hipSuccess)
+      return BN_ERR_HIP;
+    if (s->Asolve.vs != s->lev[0].A.vs) bn_rb_to_float(nz0, s->Asolve.vd, s->Asolve.vs, st);
+  }
   hipEvent_t ev;
   if (hipEventCreateWithFlags(&ev, hipEventDisableTiming) != hipSuccess) return BN_ERR_HIP;
   hipError_t e = hipEventRecord(ev, st);

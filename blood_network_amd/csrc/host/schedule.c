@@ -4,7 +4,9 @@
  *   - greedy multicolouring of the C set and of the F set of a level
  *     (C-then-F multicolour Gauss-Seidel), or of all points (plain GS);
  *   - level sets of a triangular factor (dependency depth per row) for
- *     level-scheduled triangular solves.
+ *     level-scheduled triangular solves;
+ *   - level sets of a sequential Gauss-Seidel sweep in a given visiting
+ *     order (natural or C/F order), for sweeps that reproduce it exactly.
  */
 #include <stdlib.h>
 #include <string.h>
@@ -184,6 +186,55 @@ int bn_level_sets(const int *ia, const int *ja, int n, int lower, int *rows, int
       if (d + 1 > nlev) nlev = d + 1;
     }
   }
+  int *cnt = (int *)bn_xcalloc((size_t)nlev + 1, sizeof(int));
+  for (int i = 0; i < n; i++) cnt[depth[i] + 1]++;
+  for (int l = 0; l < nlev; l++) cnt[l + 1] += cnt[l];
+  int *ptr = (int *)bn_xmalloc(((size_t)nlev + 1) * sizeof(int));
+  memcpy(ptr, cnt, ((size_t)nlev + 1) * sizeof(int));
+  for (int i = 0; i < n; i++) rows[cnt[depth[i]]++] = i;
+  free(cnt);
+  free(depth);
+  *lev_ptr = ptr;
+  return nlev;
+}
+
+int bn_sweep_levels(const dcsr_mat *A, const int *order, int backward, int *rows, int **lev_ptr) {
+  const int n = A->row;
+  *lev_ptr = NULL;
+  /* pos[i] = when the sweep visits row i */
+  int *pos = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  for (int i = 0; i < n; i++) pos[i] = -1;
+  for (int k = 0; k < n; k++) {
+    const int i = order ? order[k] : k;
+    if (i < 0 || i >= n || pos[i] >= 0) {
+      free(pos);
+      return -1;
+    }
+    pos[i] = backward ? n - 1 - k : k;
+  }
+  /* couplings from either triangle: row i of A and of A^T */
+  dcsr_mat At;
+  bn_csr_transpose(A, &At);
+  int *depth = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  int nlev = 0;
+  for (int t = 0; t < n; t++) {
+    const int k = backward ? n - 1 - t : t;
+    const int i = order ? order[k] : k;
+    const int pi = pos[i];
+    int d = 0;
+    for (int q = A->ia[i]; q < A->ia[i + 1]; q++) {
+      const int j = A->ja[q];
+      if (j != i && pos[j] < pi && depth[j] + 1 > d) d = depth[j] + 1;
+    }
+    for (int q = At.ia[i]; q < At.ia[i + 1]; q++) {
+      const int j = At.ja[q];
+      if (j != i && pos[j] < pi && depth[j] + 1 > d) d = depth[j] + 1;
+    }
+    depth[i] = d;
+    if (d + 1 > nlev) nlev = d + 1;
+  }
+  bn_csr_free(&At);
+  free(pos);
   int *cnt = (int *)bn_xcalloc((size_t)nlev + 1, sizeof(int));
   for (int i = 0; i < n; i++) cnt[depth[i] + 1]++;
   for (int l = 0; l < nlev; l++) cnt[l + 1] += cnt[l];

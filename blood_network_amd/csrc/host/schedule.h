@@ -21,6 +21,18 @@ int bn_colour_order(const dcsr_mat *A, const int *mis, int split, int *perm, int
  * *lev_ptr (malloc'ed) delimits levels.  Returns the level count. */
 int bn_level_sets(const int *ia, const int *ja, int n, int lower, int *rows, int **lev_ptr);
 
+/* Level sets of a sequential Gauss-Seidel sweep over the rows of A in the
+ * visiting order order[0], order[1], ... (order NULL: 0, 1, ...;
+ * backward != 0 visits the order from its end).  Row i depends on every
+ * row j coupled to it (A_ij or A_ji stored, j != i) that the sweep visits
+ * before i, so a level's rows are mutually uncoupled: every row they read
+ * was either visited earlier (an earlier level, already updated) or is
+ * visited later (a later level, still old) - the values the sequential
+ * sweep reads.  rows[] receives the rows grouped by level (ascending within
+ * a level); *lev_ptr (malloc'ed) delimits the levels.  Returns the level
+ * count, or -1 when order is not a permutation of 0..n-1. */
+int bn_sweep_levels(const dcsr_mat *A, const int *order, int backward, int *rows, int **lev_ptr);
+
 /* Chunked colour order for the block Gauss-Seidel smoother.  The rows are
  * cut into nchunks contiguous ranges of the current numbering (chunk k =
  * rows [k*S, min(n, (k+1)*S)), S = ceil(n / nchunks)); inside each chunk

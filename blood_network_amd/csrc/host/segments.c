@@ -142,7 +142,7 @@ static hipGraphExec_t step_graph_seg(struct bn_solver *s, bn_segs *g) {
   const char *env = getenv("BN_NO_GRAPH");
   if (env && env[0] && env[0] != '0') return NULL;
   const int key = (s->vt == 's') | (s->opt.fine_ic << 1) | (s->opt.smoother << 3) | (s->opt.nu << 8) |
-                  (s->opt.ic_sweeps << 16);
+                  (s->opt.ic_sweeps << 16) | ((s->opt.gs_exact != 0) << 28);
   if (g->gexec && g->graph_key == key) return g->gexec;
   if (g->gexec) return NULL; /* one captured mode per segmentation; other modes run eagerly */
   hipGraph_t gr = NULL;
@@ -182,7 +182,7 @@ static int seg_check(struct bn_solver *s, int nsys, const int *seg_ptr) {
       bn_set_error("segmented solve: the block Gauss-Seidel smoother option is not supported here");
       return BN_ERR_ARG;
     }
-  return 0;
+  return bn_vc_prepare(s);
 }
 
 /* d_b, d_x: n-vectors in caller numbering (= level-0 numbering) */

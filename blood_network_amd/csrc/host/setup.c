@@ -9,6 +9,7 @@
  * RS/HMIS pass, the second pass and IC(0) are inherently sequential.
  */
 #include <math.h>
+#include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -303,6 +304,28 @@ static void independent_set_rounds(INT n, const INT *gia, const INT *gja, const 
   free(pick);
 }
 
+/* The PMIS/HMIS tie-break draws: the sequence rand() gives after
+ * srand(12345) (C_amg1/src/rs_classical.c:295-298,537-541), taken from a
+ * private glibc random_r state of the same type (TYPE_3, 128-byte state,
+ * which is what srand/rand use), so the draws match the reference's on the
+ * same C library while concurrent set-ups on other threads neither disturb
+ * them nor the caller's own rand() sequence. */
+typedef struct {
+  struct random_data rd;
+  char state[128];
+} weight_rng;
+
+static void weight_rng_init(weight_rng *g) {
+  memset(g, 0, sizeof(*g));
+  initstate_r(12345u, g->state, sizeof(g->state), &g->rd);
+}
+
+static REAL weight_draw(weight_rng *g) {
+  int32_t v = 0;
+  random_r(&g->rd, &v);
+  return (REAL)v / (REAL)RAND_MAX;
+}
+
 /* PMIS, C_amg1/src/rs_classical.c:246-371.  Weights lambda_i +
  * rand()/RAND_MAX after srand(12345), drawn in index order, so the split
  * matches the reference on the same C library. */
@@ -313,9 +336,10 @@ void pmis_coarsening(const dcsr_mat *S, INT *cf, INT *n_coarse) {
   INT *gia, *gja;
   strength_graph(S, &ST, &gia, &gja);
   REAL *w = (REAL *)bn_xmalloc((size_t)n * sizeof(REAL));
-  srand(12345);
+  weight_rng g;
+  weight_rng_init(&g);
   for (INT i = 0; i < n; i++) {
-    w[i] = (REAL)(ST.ia[i + 1] - ST.ia[i]) + (REAL)rand() / (REAL)RAND_MAX;
+    w[i] = (REAL)(ST.ia[i + 1] - ST.ia[i]) + weight_draw(&g);
     cf[i] = 0;
   }
   bn_csr_free(&ST);
@@ -352,10 +376,10 @@ void hmis_coarsening(const dcsr_mat *S, INT *cf, INT *n_coarse) {
     INT *gia, *gja;
     strength_graph(S, &ST, &gia, &gja);
     REAL *w = (REAL *)bn_xmalloc((size_t)n * sizeof(REAL));
-    srand(12345);
+    weight_rng g;
+    weight_rng_init(&g);
     for (INT i = 0; i < n; i++)
-      w[i] = (cf[i] == 0) ? (REAL)(ST.ia[i + 1] - ST.ia[i]) + (REAL)rand() / (REAL)RAND_MAX
-                          : -1.0;
+      w[i] = (cf[i] == 0) ? (REAL)(ST.ia[i + 1] - ST.ia[i]) + weight_draw(&g) : -1.0;
     independent_set_rounds(n, gia, gja, w, cf, undecided);
     free(w);
     free(gia);
@@ -592,6 +616,7 @@ void dcsr_rap(dcsr_mat *R, dcsr_mat *A, dcsr_mat *P, dcsr_mat *RAP) {
  * ------------------------------------------------------------------- */
 void ichol_compute(const dcsr_mat *A, ichol_factor *L) {
   const INT n = A->row;
+  bn_cache_drop(L); /* a device mirror of a factor held in *L is stale now */
   INT *cnt = (INT *)bn_xmalloc((size_t)n * sizeof(INT));
 #pragma omp parallel for schedule(static)
   for (INT i = 0; i < n; i++) {
@@ -699,6 +724,7 @@ static double setup_ms(void) { return 1e3 * omp_get_wtime(); }
 
 /* amg_build_hierarchy with a choice of strength rule (BN_STRENGTH_*) */
 void bn_build_hierarchy(amg_data *amg, const dcsr_mat *A, int strength) {
+  bn_cache_drop(amg); /* a device mirror of a hierarchy held in *amg is stale now */
   const char *lg = getenv("BN_SETUP_LOG");
   const int log = lg && *lg && *lg != '0';
   INT max_levels = amg->max_levels;

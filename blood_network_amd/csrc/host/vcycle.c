@@ -11,7 +11,11 @@
  *   V-cycle    C_amg1/src/amg_cycle.c:263-284
  * Differences by design: the CF-ordered Gauss-Seidel sweep runs as a
  * C-then-F multicolour sweep (levels needing more than max_colours
- * colours fall back to l1-Jacobi), restriction gathers over a stored
+ * colours fall back to l1-Jacobi) unless the exact order is asked for
+ * (gs_exact; SMOOTHER_GS always sweeps in exact natural order): then the
+ * sequential sweep runs level by level over its dependency DAG and reads
+ * exactly the values the reference's sweep reads (amg_cycle.c:21-80),
+ * restriction gathers over a stored
  * R = P^T instead of scattering, the coarsest solve is a dense inverse
  * (or the reference IC-PCG), and zero initial guesses skip the work they
  * make redundant (a residual equal to b, the first colour's row sums).
@@ -21,6 +25,8 @@
 #include <stdlib.h>
 
 #include "bn_solver.h"
+
+void bn_set_error(const char *fmt, ...);
 
 static inline const void *opv(const bn_dcsr *m, char vt) {
   return vt == 's' ? (const void *)m->vs : (const void *)m->vd;
@@ -56,6 +62,58 @@ static void smooth_block(struct bn_solver *s, int lev, const void *b, const void
   }
 }
 
+int bn_vc_exact_kind(const struct bn_solver *s) {
+  if (s->opt.smoother == SMOOTHER_GS) return 0;
+  if (s->opt.smoother == SMOOTHER_CF_GS && s->opt.gs_exact) return 1;
+  return -1;
+}
+
+int bn_vc_prepare(struct bn_solver *s) {
+  const int kind = bn_vc_exact_kind(s);
+  if (kind < 0) return 0;
+  for (int l = 0; l < s->nlev - 1; l++) {
+    bn_dlevel *d = &s->lev[l];
+    if (d->block || d->xs[kind].ptr_f) continue;
+    const amg_level *L = &s->amg->levels[l];
+    if (kind == 1 && !L->cf_order) {
+      bn_set_error("level %d has no cf_order for the exact C/F Gauss-Seidel sweep", l);
+      return BN_ERR_ARG;
+    }
+    /* level rows keep the host numbering outside the block layout */
+    const int rc = bn_sweep_upload(s, &L->A, kind == 1 ? L->cf_order : NULL, &d->xs[kind]);
+    if (rc) return rc;
+  }
+  return 0;
+}
+
+/* nu exact-order sweeps (the sequential sweep, level by level over its
+ * dependency DAG): forward visits the order, backward its reverse
+ * (amg_cycle.c:21-80) */
+static void smooth_exact(struct bn_solver *s, int lev, int kind, const void *b, void *x, int backward, int x_zero,
+                         const void *zbase, void *zsum) {
+  bn_dlevel *d = &s->lev[lev];
+  const char vt = s->vt;
+  const int nu = s->opt.nu > 0 ? s->opt.nu : 1;
+  bn_sweep *w = &d->xs[kind];
+  if (!w->ptr_f && bn_vc_prepare(s)) {
+    fprintf(stderr, "libbnamg: exact Gauss-Seidel schedule unavailable: %s\n", bn_last_error());
+    abort();
+  }
+  if (x_zero) bn_fill(vt, d->n, x, 0.0, s->stream);
+  const int nl = backward ? w->nb : w->nf;
+  const int *rows = backward ? w->rows_b : w->rows_f;
+  const int *ptr = backward ? w->ptr_b : w->ptr_f;
+  const void *dinv = vecv(d->dinv_d, d->dinv_s, vt);
+  for (int sw = 0; sw < nu; sw++)
+    for (int l = 0; l < nl; l++)
+      bn_trsv_level(vt, ptr[l + 1] - ptr[l], rows + ptr[l], d->A.ia, d->A.ja, opv(&d->A, vt), dinv, b, x,
+                    bn_lanes(&d->A, vt), NULL, NULL, s->stream);
+  if (zsum) {
+    bn_copy(vt, d->n, zbase, zsum, s->stream);
+    bn_axpy1(vt, d->n, x, zsum, s->stream);
+  }
+}
+
 /* nu smoothing sweeps on level `lev`; backward runs colours in reverse.
  * x_zero: x is zero on entry (first colour then needs no row sums).
  * zsum != NULL: the final sweep also writes zsum = zbase + x. */
@@ -75,7 +133,12 @@ static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int bac
     }
     return;
   }
-  const int gs = d->use_gs && (s->opt.smoother == SMOOTHER_CF_GS || s->opt.smoother == SMOOTHER_GS);
+  const int xk = bn_vc_exact_kind(s);
+  if (xk >= 0) {
+    smooth_exact(s, lev, xk, b, x, backward, x_zero, zbase, zsum);
+    return;
+  }
+  const int gs = d->use_gs && s->opt.smoother == SMOOTHER_CF_GS;
   if (gs) {
     /* from zero, the first colour's kernel also zeroes the other rows
      * (unless the first sweep is also the last and must write zsum) */

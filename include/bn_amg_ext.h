@@ -81,6 +81,11 @@ typedef struct {
   int chunk_rows;     /* largest chunk of the block smoother (<= 8192;    */
                       /*    0: 8192)                                      */
   int strength;       /* BN_STRENGTH_*                                    */
+  int gs_exact;       /* SMOOTHER_CF_GS: 0 runs the C-then-F multicolour  */
+                      /*    sweep, 1 the reference's exact cf_order sweep */
+                      /*    (level-scheduled over its dependency DAG).    */
+                      /*    SMOOTHER_GS is always the exact natural-order */
+                      /*    sweep (C_amg1/src/amg_cycle.c:21-80)          */
 } bn_options;
 
 typedef struct {
@@ -234,6 +239,39 @@ int bn_solver_solve_segments_device(void *h, int nsys, const int *seg_ptr, const
 /* Same with device pointers. */
 int bn_solver_solve_block_device(void *h, int k, const double *d_b, double *d_x, double tol,
                                  int maxiter, int *its, bn_solve_info *info);
+
+/* ---- C-typed twin of the MEX smoother
+ *   x = cf_gauss_seidel_mex(A, b, x0, cf_order, nu, direction)
+ * (Matlab_amg/cf_gauss_seidel_mex.c:7,27-105), same argument list with C
+ * types: A is the n x n CSR matrix (ia, ja, val; A symmetric, so MATLAB's
+ * CSC arrays jc/ir/pr describe the same matrix), cf_order holds 1-based
+ * row numbers as doubles (MATLAB's convention), direction 0 sweeps
+ * cf_order(1) .. cf_order(n), 1 the reverse.  nu Gauss-Seidel sweeps start
+ * from x0 and end in x (x may be x0): x_i = (b_i - sum_{j != i} a_ij x_j)
+ * / a_ii with the row's first diagonal entry (1.0 if none).  The sweeps
+ * run on the GPU level by level over the order's dependency DAG and read
+ * exactly the values the sequential sweep reads.  Returns 0, BN_ERR_ARG
+ * (bad sizes, direction, or a cf_order that is not a permutation of
+ * 1..n; bn_last_error() then starts with the MEX error id it corresponds
+ * to, "cf_gs:args" or "cf_gs:order"), BN_ERR_NO_DEVICE or BN_ERR_HIP. */
+int cf_gauss_seidel(int n, const int *ia, const int *ja, const double *val, const double *b, const double *x0,
+                    const double *cf_order, int nu, int direction, double *x);
+
+/* The same sweeps on device arrays: the matrix (nnz entries), b and x
+ * (the iterate, updated in place) and order (0-based row numbers, int)
+ * are all device pointers.  The schedule is host analysis, so the pattern,
+ * the values and the order are read back once per call. */
+int bn_cf_gauss_seidel_device(int n, int nnz, const int *d_ia, const int *d_ja, const double *d_val,
+                              const double *d_b, double *d_x, const int *d_order, int nu, int direction);
+
+/* ---- options of the solvers behind the library-level entry points on a
+ * caller's amg_data (amg_backslash, amg_fwdslash, amg_vcycle_precond,
+ * amg_ichol_precond): coarsest solver, exact-order sweeps (gs_exact), ...
+ * NULL restores bn_options_default.  The smoother, nu, omega and
+ * coarsening always come from the amg_data itself.  Device mirrors built
+ * under the previous options are dropped.  The reference has no options
+ * beyond the amg_data fields. */
+void bn_set_library_options(const bn_options *o);
 
 #ifdef __cplusplus
 }

@@ -101,6 +101,9 @@ void amg_fwdslash(const amg_data *amg, INT lev, const REAL *b, REAL *x, INT nu);
 void amg_vcycle_precond(const amg_data *amg, const REAL *g, REAL *x);
 void amg_ichol_precond(const amg_data *amg, const dcsr_mat *A,
                        const ichol_factor *L, const REAL *g, REAL *x);
+/* MEX smoother (Matlab_amg/cf_gauss_seidel_mex.c:27-105) */
+void oracle_cf_gauss_seidel(INT n, const INT *ia, const INT *ja, const REAL *val, const REAL *b, const REAL *x0,
+                            const REAL *cf_order, INT nu, INT direction, REAL *x);
 INT amg_pcg(const dcsr_mat *A, const REAL *b, REAL *x,
             void (*precond)(const void *, const REAL *, REAL *),
             const void *data, REAL tol, INT maxiter, INT print_level);

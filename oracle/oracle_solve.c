@@ -266,3 +266,33 @@ INT amg_pcg(const dcsr_mat *A, const REAL *b, REAL *x,
   free(r); free(z); free(p); free(Ap);
   return it;
 }
+
+/* The MEX smoother x = cf_gauss_seidel_mex(A, b, x0, cf_order, nu,
+ * direction) (Matlab_amg/cf_gauss_seidel_mex.c:27-105): x starts as a copy
+ * of x0; each row's diagonal is its first diagonal entry, 1.0 if none,
+ * taken once before the sweeps (:66-75); cf_order holds 1-based row numbers
+ * as doubles (:83); direction 0 visits cf_order(1) .. cf_order(n), 1 the
+ * reverse (:79-103); x_i = (b_i - sum_{j != i} a_ij x_j) / diag_i. */
+void oracle_cf_gauss_seidel(INT n, const INT *ia, const INT *ja, const REAL *val, const REAL *b, const REAL *x0,
+                            const REAL *cf_order, INT nu, INT direction, REAL *x) {
+  REAL *dg = (REAL *)malloc((size_t)(n > 0 ? n : 1) * sizeof(REAL));
+  memcpy(x, x0, (size_t)n * sizeof(REAL));
+  for (INT i = 0; i < n; i++) {
+    dg[i] = 1.0;
+    for (INT k = ia[i]; k < ia[i + 1]; k++)
+      if (ja[k] == i) {
+        dg[i] = val[k];
+        break;
+      }
+  }
+  for (INT s = 0; s < nu; s++)
+    for (INT t = 0; t < n; t++) {
+      const INT ii = direction ? n - 1 - t : t;
+      const INT i = (INT)(cf_order[ii] - 1);
+      REAL acc = b[i];
+      for (INT k = ia[i]; k < ia[i + 1]; k++)
+        if (ja[k] != i) acc -= val[k] * x[ja[k]];
+      x[i] = acc / dg[i];
+    }
+  free(dg);
+}

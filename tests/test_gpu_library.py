@@ -153,3 +153,122 @@ def test_amg_pcg_with_host_preconditioner(hier, net30):
     print(f"amg_pcg iterations {it}, handle {ih}, rel(x, direct) {rel(x, xd):.2e}")
     assert it > 0 and abs(it - ih) <= 1
     assert rel(x, xd) < 1e-8
+
+
+def test_amg_pcg_reads_the_matrix_afresh():
+    # one dcsr_mat struct, solved three times: as built, with its values
+    # changed in place, then pointing at another (larger) matrix; the
+    # reference reads A on every call, so must the device copy
+    case("amg_pcg on one dcsr_mat: new values in place, then a new size")
+    jac = {}
+
+    @PRECOND
+    def precond(data, gp, zp):
+        d = jac["d"]
+        g = np.ctypeslib.as_array(gp, (len(d),))
+        z = np.ctypeslib.as_array(zp, (len(d),))
+        z[:] = g / d
+
+    def solve_and_check(A, m, b):
+        jac["d"] = m.diagonal()
+        x = np.zeros(m.shape[0])
+        it = lib.amg_pcg(C.byref(A), dp(b), dp(x), precond, None, 1e-11, 20000, 0)
+        xd = spla.spsolve(m.tocsc(), b)
+        print(f"n={m.shape[0]} iterations {it} rel(x, direct) {rel(x, xd):.2e}")
+        assert it > 0 and rel(x, xd) < 1e-7
+
+    n1 = make_network(10, 6, seed=3)
+    ia, ja, val = n1.ia.copy(), n1.ja.copy(), n1.val.copy()
+    A = L.CSR(n1.n, n1.n, n1.nnz, ip(ia), ip(ja), dp(val))
+    solve_and_check(A, sp.csr_matrix((val, ja, ia), shape=(n1.n, n1.n)), n1.b)
+    val *= 2.0                                   # in place: same pointers
+    solve_and_check(A, sp.csr_matrix((val, ja, ia), shape=(n1.n, n1.n)), n1.b)
+    n2 = make_network(12, 6, seed=4)
+    ia2, ja2, val2 = n2.ia.copy(), n2.ja.copy(), n2.val.copy()
+    A.row = A.col = n2.n
+    A.nnz = n2.nnz
+    A.ia, A.ja, A.val = ip(ia2), ip(ja2), dp(val2)
+    solve_and_check(A, sp.csr_matrix((val2, ja2, ia2), shape=(n2.n, n2.n)), n2.b)
+
+
+def test_two_handles_on_two_threads():
+    # handles are independent (one HIP stream each, no shared state); a
+    # handle's results do not depend on another thread working beside it
+    import threading
+    case("two solver handles solved concurrently on two threads")
+    na, nb = make_network(16, 9, seed=1), make_network(14, 8, seed=2)
+    sa = AMGSolver(na.ia, na.ja, na.val, coarsening="pmis", min_size=500)
+    sb = AMGSolver(nb.ia, nb.ja, nb.val, coarsening="hmis", min_size=500)
+    ref_a, _ = sa.solve(na.b, tol=1e-10)
+    ref_b, _ = sb.solve(nb.b, tol=1e-10)
+    out, errs = {}, []
+
+    def work(key, s, b):
+        try:
+            for rep in range(4):
+                out[(key, rep)] = s.solve(b, tol=1e-10)[0]
+        except Exception as e:  # reported below
+            errs.append(e)
+
+    ts = [threading.Thread(target=work, args=("a", sa, na.b)), threading.Thread(target=work, args=("b", sb, nb.b))]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errs, errs
+    for rep in range(4):
+        assert np.array_equal(out[("a", rep)], ref_a)
+        assert np.array_equal(out[("b", rep)], ref_b)
+    sa.free()
+    sb.free()
+
+
+def test_library_preconditioner_from_two_threads(hier):
+    # two threads on one amg_data: the mirror's lock runs them one after the
+    # other, each gets the single-thread result
+    import threading
+    case("amg_vcycle_precond on one hierarchy from two threads")
+    rng = np.random.default_rng(9)
+    gs = [rng.random(hier.n) for _ in range(2)]
+    ref = []
+    for g in gs:
+        z = np.zeros(hier.n)
+        lib.amg_vcycle_precond(hier.ptr, dp(g), dp(z))
+        ref.append(z)
+    got = [[None] * 3 for _ in range(2)]
+
+    def work(k):
+        for rep in range(3):
+            z = np.zeros(hier.n)
+            lib.amg_vcycle_precond(hier.ptr, dp(gs[k]), dp(z))
+            got[k][rep] = z
+
+    ts = [threading.Thread(target=work, args=(k,)) for k in range(2)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    for k in range(2):
+        for rep in range(3):
+            assert np.array_equal(got[k][rep], ref[k])
+
+
+def test_set_values_device_reaches_the_fp32_preconditioner(net16):
+    # the residuals inside the FP32 AMG+IC preconditioner use the solve
+    # matrix (amg_ichol_precond.c:37,44): after new values on the device they
+    # must use the new ones, as the FP64 preconditioner does
+    case("set_values_device: FP32 and FP64 preconditioners see the same new matrix")
+    net = net16
+    s = AMGSolver(net.ia, net.ja, net.val, coarsening="pmis", min_size=500, precision="fp32", fine_ic="jacobi")
+    g = np.random.default_rng(10).random(net.n)
+    before = s.precondition(g)
+    dv = L.DeviceArray(net.val * 1.7)
+    s.set_values_device(net.nnz, dv.ptr)
+    z32 = s.precondition(g)
+    s.set_mode(precision="fp64", fine_ic="jacobi")
+    z64 = s.precondition(g)
+    e, moved = rel(z32, z64), rel(z32, before)
+    print(f"FP32 vs FP64 after new values {e:.2e}; change from before {moved:.2e}")
+    assert e < 1e-4 and moved > 1e-2
+    dv.free()
+    s.free()

@@ -169,3 +169,114 @@ def test_chunk_order_colours_each_chunk_properly(nchunks):
     sel = off & same_chunk
     assert not np.any(colour[A.row[sel]] == colour[A.col[sel]])
 
+
+
+def sweep_levels(A, order, backward):
+    """bn_sweep_levels on a scipy CSR matrix: level of each row."""
+    n = A.shape[0]
+    ia, ja = A.indptr.astype(np.int32), A.indices.astype(np.int32)
+    va = A.data.astype(np.float64)
+    m = L.CSR(n, n, len(ja), L.ip(ia), L.ip(ja), L.dp(va))
+    rows = np.zeros(n, np.int32)
+    pp = C.POINTER(C.c_int)()
+    o = None if order is None else np.ascontiguousarray(order, np.int32)
+    nl = lib.bn_sweep_levels(C.byref(m), L.ip(o) if o is not None else None, int(backward), L.ip(rows),
+                             C.byref(pp))
+    if nl < 0:
+        return None
+    ptr = np.ctypeslib.as_array(pp, (nl + 1,)).copy()
+    level = np.empty(n, np.int64)
+    for k in range(nl):
+        level[rows[ptr[k]:ptr[k + 1]]] = k
+        assert np.all(np.diff(rows[ptr[k]:ptr[k + 1]]) > 0)
+    assert sorted(rows) == list(range(n))
+    return level
+
+
+def sequential_sweep(A, b, x, order):
+    """The reference's sweep (C_amg1/src/amg_cycle.c:21-80) row by row."""
+    x = x.copy()
+    for i in order:
+        s, d = b[i], 1.0
+        for k in range(A.indptr[i], A.indptr[i + 1]):
+            j = A.indices[k]
+            if j == i:
+                d = A.data[k]
+            else:
+                s -= A.data[k] * x[j]
+        x[i] = s / d
+    return x
+
+
+def levelled_sweep(A, b, x, level):
+    """The same row updates, all rows of a level reading the iterate as it
+    stood before the level (what one device launch per level does)."""
+    x = x.copy()
+    for lv in range(level.max() + 1):
+        rows = np.flatnonzero(level == lv)
+        new = {}
+        for i in rows:
+            s, d = b[i], 1.0
+            for k in range(A.indptr[i], A.indptr[i + 1]):
+                j = A.indices[k]
+                if j == i:
+                    d = A.data[k]
+                else:
+                    s -= A.data[k] * x[j]
+            new[i] = s / d
+        for i, v in new.items():
+            x[i] = v
+    return x
+
+
+@pytest.mark.parametrize("kind", ["natural", "cf", "random"])
+@pytest.mark.parametrize("backward", [0, 1])
+def test_sweep_levels_reproduce_the_sequential_sweep(kind, backward):
+    net = make_network(8, 5, seed=9)
+    A = sp.csr_matrix((net.val, net.ja, net.ia), shape=(net.n, net.n))
+    n = net.n
+    rng = np.random.default_rng(3)
+    if kind == "natural":
+        order = None
+        seq = np.arange(n)
+    elif kind == "cf":
+        c = rng.random(n) < 0.3                      # C points first, then F, ascending
+        seq = np.concatenate([np.flatnonzero(c), np.flatnonzero(~c)])
+        order = seq
+    else:
+        seq = rng.permutation(n)
+        order = seq
+    level = sweep_levels(A, order, backward)
+    visit = seq[::-1] if backward else seq
+    pos = np.empty(n, np.int64)
+    pos[visit] = np.arange(n)
+    # a coupled pair sits in levels ordered as the sweep visits them, and
+    # levels are tight: one more than the latest earlier-visited neighbour
+    coo = (A + A.T).tocoo()
+    off = coo.row != coo.col
+    r, c = coo.row[off], coo.col[off]
+    before = pos[c] < pos[r]
+    assert np.all(level[c[before]] < level[r[before]])
+    need = np.zeros(n, np.int64)
+    np.maximum.at(need, r[before], level[c[before]] + 1)
+    assert np.array_equal(level, need)
+    # and the level-by-level update gives the sequential sweep bit for bit
+    b = rng.random(n)
+    x0 = rng.random(n)
+    assert np.array_equal(levelled_sweep(A, b, x0, level), sequential_sweep(A, b, x0, visit))
+
+
+def test_sweep_levels_couple_through_either_triangle():
+    # a pattern stored in one triangle only: row 2 reads x_0, row 0 does not
+    # store a_02; visiting 2 before 0 must still separate them
+    A = sp.csr_matrix(np.array([[4.0, 0, 0], [0, 4.0, 0], [1.0, 0, 4.0]]))
+    level = sweep_levels(A, np.array([2, 0, 1]), 0)
+    assert level[2] < level[0]
+    level = sweep_levels(A, None, 0)
+    assert level[0] < level[2]
+
+
+def test_sweep_levels_reject_a_non_permutation():
+    A = sp.identity(4, format="csr")
+    assert sweep_levels(A, np.array([0, 1, 1, 3]), 0) is None
+    assert sweep_levels(A, np.array([0, 1, 2, 4]), 0) is None
