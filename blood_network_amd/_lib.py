@@ -153,6 +153,8 @@ def _load():
         "bn_solver_adjoint_grad": (C.c_int, [VP, DP, DP, C.c_int, IP, IP, DP, DP, C.c_double, C.c_int, DP, DP, DP]),
         "bn_solver_adjoint_grad_device": (C.c_int, [VP, VP, VP, C.c_int, VP, VP, VP, VP, C.c_int, C.c_double,
                                                     C.c_int, VP, VP, VP]),
+        "bn_solver_adjoint_grad_noalloc": (C.c_int, [VP, DP, DP, C.c_int, IP, IP, DP, DP, C.c_double, C.c_int,
+                                                     DP, DP, DP]),
         "bn_solver_set_values_device": (C.c_int, [VP, C.c_int, VP]),
         "bn_solver_rebuild_values_device": (C.c_int, [VP, C.c_int, VP]),
         "bn_solver_coarse_inverse": (C.c_int, [VP, IP, DP]),

@@ -192,6 +192,24 @@ class AMGSolver:
         self._check(it)
         return lam, dg, dlr, it
 
+    def adjoint_grad_noalloc(self, p, dLdp, eu, ev, g, pdir, tol=1e-10, maxiter=1000):
+        """adjoint_grad without device allocation (static scratch; host
+        arrays, same results)."""
+        p = np.ascontiguousarray(p, dtype=np.float64)
+        dLdp = np.ascontiguousarray(dLdp, dtype=np.float64)
+        eu = np.ascontiguousarray(eu, dtype=np.int32)
+        ev = np.ascontiguousarray(ev, dtype=np.int32)
+        g = np.ascontiguousarray(g, dtype=np.float64)
+        pdir = np.ascontiguousarray(pdir, dtype=np.float64)
+        lam = np.zeros(self.n)
+        dg = np.zeros(len(eu))
+        dlr = np.zeros(len(eu))
+        it = L.lib.bn_solver_adjoint_grad_noalloc(self.h, L.dp(p), L.dp(dLdp), len(eu), L.ip(eu), L.ip(ev),
+                                                  L.dp(g), L.dp(pdir), tol, maxiter, L.dp(lam), L.dp(dg),
+                                                  L.dp(dlr))
+        self._check(it)
+        return lam, dg, dlr, it
+
     def adjoint_grad_device(self, p_ptr, dLdp_ptr, m, eu_ptr, ev_ptr, g_ptr, pdir_ptr, nd, lam_ptr,
                             dLdg_ptr=None, dLdlogr_ptr=None, tol=1e-10, maxiter=1000):
         """The adjoint gradient on device pointers; lam_ptr is the initial

@@ -128,6 +128,9 @@ struct bn_solver {
   bn_options opt;
   int device;           /* HIP device ordinal */
   int dev_ready;        /* device mirror uploaded */
+  int noalloc;          /* inside an allocation-free call: step graphs are
+                           replayed if captured but none is captured, and
+                           no smoother schedule is built */
   /* device side */
   hipStream_t stream;
   char vt;              /* V-cycle precision: 'd' or 's' */

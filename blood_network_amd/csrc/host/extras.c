@@ -120,6 +120,93 @@ int bn_value_map(struct bn_solver *s) {
   return rc;
 }
 
+/* bn_solver_adjoint_grad without any device allocation: the solve runs on
+ * the solver's own vectors (step graphs already captured are replayed,
+ * none is captured), p sits in one of the solver's work vectors, and the
+ * segment arrays pass through the static device scratch (noalloc.hip) in
+ * chunks of BN_SCRATCH_EDGES, so m is unbounded; at most
+ * BN_SCRATCH_DIRICHLET Dirichlet pressures.  Every output element is
+ * written by the kernel, so nothing in the scratch needs clearing.  Calls
+ * that overlap on one GPU share its scratch and race. */
+int bn_solver_adjoint_grad_noalloc(void *h, const double *p, const double *dLdp, int m, const int *eu,
+                                   const int *ev, const double *g, const double *pdir, double tol, int maxiter,
+                                   double *lam, double *dLdg, double *dLdlogr) {
+  struct bn_solver *s = (struct bn_solver *)h;
+  if (!s || !p || !dLdp || !lam || m < 0 || (m > 0 && (!eu || !ev || !g))) {
+    bn_set_error("adjoint_grad_noalloc: NULL argument or m < 0");
+    return BN_ERR_ARG;
+  }
+  if (!s->dev_ready) {
+    bn_set_error("adjoint_grad_noalloc: the solver has no device mirror (it is uploaded at creation when a GPU "
+                 "is present; this entry point does not upload)");
+    return BN_ERR_ARG;
+  }
+  int nd = 0;
+  for (int e = 0; e < m; e++) {
+    if (eu[e] < -nd) nd = -eu[e];
+    if (ev[e] < -nd) nd = -ev[e];
+  }
+  if (nd > 0 && !pdir) {
+    bn_set_error("adjoint_grad_noalloc: Dirichlet ends given but no pdir");
+    return BN_ERR_ARG;
+  }
+  if (nd > BN_SCRATCH_DIRICHLET) {
+    bn_set_error("adjoint_grad_noalloc: %d Dirichlet pressures, more than the %d the static scratch holds", nd,
+                 BN_SCRATCH_DIRICHLET);
+    return BN_ERR_ARG;
+  }
+  if (hipSetDevice(s->device) != hipSuccess) return BN_ERR_HIP;
+  unsigned char *scr = NULL;
+  size_t bytes = 0;
+  int rc = bn_scratch_ptr((void **)&scr, &bytes);
+  if (rc) {
+    bn_set_error("adjoint_grad_noalloc: device scratch unavailable");
+    return rc;
+  }
+  const int n = s->lev[0].n;
+  hipStream_t st = s->stream;
+  /* lambda = A^{-1} dL/dp from zero, on the solver's level-0 vectors */
+  double *db = (double *)s->lev[0].b, *dx = (double *)s->lev[0].x;
+  if (hipMemcpyAsync(db, dLdp, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+    return BN_ERR_HIP;
+  bn_fill('d', n, dx, 0.0, st);
+  s->noalloc = 1;
+  const int it = bn_pcg_run(s, db, dx, tol, maxiter, 0, NULL);
+  s->noalloc = 0;
+  if (it == BN_ERR_HIP || it == BN_ERR_NO_DEVICE || it == BN_ERR_ARG) return it;
+  if (hipMemcpyAsync(lam, dx, (size_t)n * sizeof(double), hipMemcpyDeviceToHost, st) != hipSuccess)
+    return BN_ERR_HIP;
+  if (m > 0 && (dLdg || dLdlogr)) {
+    /* scratch: pdir, then per pass eu | ev | g | dg | dlogr */
+    double *pdir_d = (double *)scr;
+    unsigned char *chunk = scr + (size_t)BN_SCRATCH_DIRICHLET * sizeof(double);
+    int *eu_d = (int *)chunk;
+    int *ev_d = eu_d + BN_SCRATCH_EDGES;
+    double *g_d = (double *)(ev_d + BN_SCRATCH_EDGES);
+    double *dg_d = g_d + BN_SCRATCH_EDGES;
+    double *dr_d = dg_d + BN_SCRATCH_EDGES;
+    double *p_d = s->q; /* a free FP64 n-vector of the solver after the solve */
+    if (hipMemcpyAsync(p_d, p, (size_t)n * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess ||
+        (nd > 0 && hipMemcpyAsync(pdir_d, pdir, (size_t)nd * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess))
+      return BN_ERR_HIP;
+    /* stream order keeps each pass's copies behind the previous pass's reads */
+    for (int e0 = 0; e0 < m; e0 += BN_SCRATCH_EDGES) {
+      const int mc = (m - e0 < BN_SCRATCH_EDGES) ? m - e0 : BN_SCRATCH_EDGES;
+      if (hipMemcpyAsync(eu_d, eu + e0, (size_t)mc * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
+          hipMemcpyAsync(ev_d, ev + e0, (size_t)mc * sizeof(int), hipMemcpyHostToDevice, st) != hipSuccess ||
+          hipMemcpyAsync(g_d, g + e0, (size_t)mc * sizeof(double), hipMemcpyHostToDevice, st) != hipSuccess)
+        return BN_ERR_HIP;
+      bn_edge_grad(mc, eu_d, ev_d, g_d, p_d, dx, pdir_d, dg_d, dr_d, st);
+      if ((dLdg && hipMemcpyAsync(dLdg + e0, dg_d, (size_t)mc * sizeof(double), hipMemcpyDeviceToHost, st) !=
+                       hipSuccess) ||
+          (dLdlogr && hipMemcpyAsync(dLdlogr + e0, dr_d, (size_t)mc * sizeof(double), hipMemcpyDeviceToHost, st) !=
+                          hipSuccess))
+        return BN_ERR_HIP;
+    }
+  }
+  return hipStreamSynchronize(st) == hipSuccess ? it : BN_ERR_HIP;
+}
+
 /* New FP64 values for the solve matrix, in device memory, on the pattern
  * and in the entry order of the stored (symmetrised) matrix: the creation
  * order whenever the input pattern was symmetric without duplicates, as

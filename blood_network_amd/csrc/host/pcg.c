@@ -74,6 +74,7 @@ static hipGraphExec_t step_graph(struct bn_solver *s) {
   const int key = mode_key(s);
   for (int k = 0; k < s->ngraph; k++)
     if (s->graph_key[k] == key) return s->gexec[k];
+  if (s->noalloc) return NULL; /* capturing allocates: run the step eagerly */
   if (s->ngraph == (int)(sizeof(s->gexec) / sizeof(s->gexec[0]))) return NULL;
   hipGraph_t g = NULL;
   hipGraphExec_t e = NULL;

@@ -74,6 +74,11 @@ int bn_vc_prepare(struct bn_solver *s) {
   for (int l = 0; l < s->nlev - 1; l++) {
     bn_dlevel *d = &s->lev[l];
     if (d->block || d->xs[kind].ptr_f) continue;
+    if (s->noalloc) {
+      bn_set_error("the exact Gauss-Seidel schedules are not built yet (an allocation-free call does not "
+                   "build them): run one ordinary solve with this smoother first");
+      return BN_ERR_ARG;
+    }
     const amg_level *L = &s->amg->levels[l];
     if (kind == 1 && !L->cf_order) {
       bn_set_error("level %d has no cf_order for the exact C/F Gauss-Seidel sweep", l);

@@ -37,6 +37,17 @@ enum {
 /* reduction partial buffer holds at most this many blocks per dot */
 #define BN_RED_BLOCKS 2048
 
+/* Static device scratch of the allocation-free entry points (noalloc.hip):
+ * BN_SCRATCH_EDGES segments staged per pass (4 + 4 + 8 bytes in, 8 + 8
+ * out) plus BN_SCRATCH_DIRICHLET Dirichlet pressures; one copy per GPU. */
+#define BN_SCRATCH_EDGES (1 << 20)
+#define BN_SCRATCH_DIRICHLET (1 << 16)
+#define BN_SCRATCH_BYTES ((size_t)BN_SCRATCH_EDGES * 32 + (size_t)BN_SCRATCH_DIRICHLET * 8)
+#define BN_SCRATCH_MAX_DEVICES 64
+/* this device's copy of the scratch (resolved once per device, the
+ * device current); 0 or a BN_ERR_* code */
+int bn_scratch_ptr(void **ptr, size_t *bytes);
+
 /* Lanes-per-row SpMV variants: the lanes argument of bn_csr_apply and
  * bn_csr_apply_seg may carry one of these bits.  Each lane then issues 8
  * (DEEP8) or 16 (DEEP16) index and value loads before its x gathers

@@ -176,6 +176,23 @@ int bn_solver_adjoint_grad_device(void *h, const double *d_p, const double *d_dL
                                   const double *d_pdir, int nd, double tol, int maxiter,
                                   double *d_lam, double *d_dLdg, double *d_dLdlogr);
 
+/* bn_solver_adjoint_grad (same arguments, host arrays, same results bit
+ * for bit) without any device allocation.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (tight
+ * descent or serving loops, code that forbids hipMalloc).  The solve runs
+ * on the solver's own vectors (a step graph already captured for the
+ * mode is replayed, none is captured) and the segment arrays pass through
+ * a static device scratch of BN_SCRATCH_BYTES per GPU (about 32.5 MiB,
+ * held until the process ends) in chunks, so m is unbounded; at most
+ * BN_SCRATCH_DIRICHLET (65,536) Dirichlet pressures, else BN_ERR_ARG.
+ * The solver must already be on the device (creation uploads it), and an
+ * exact-order Gauss-Seidel smoother needs one ordinary solve first to
+ * build its schedules.  Calls that overlap on one GPU share the scratch
+ * and race: serialise them. */
+int bn_solver_adjoint_grad_noalloc(void *h, const double *p, const double *dLdp, int m, const int *eu,
+                                   const int *ev, const double *g, const double *pdir, double tol,
+                                   int maxiter, double *lam, double *dLdg, double *dLdlogr);
+
 /* ---- new solve-matrix values from device memory: nnz values on the
  * stored pattern, in its entry order (the creation order when the input
  * pattern was symmetric without duplicates; nnz must match).  The
