@@ -7,9 +7,10 @@ package loads that library and fails if it is missing.
 from ._lib import device_available, last_error, lib
 from .amg import AMGSolver, SolveReport
 from .inverse import PressureFit
-from .netgen import (WORKLOADS, NetworkCase, VesselNetwork, make_network, read_network_case, uneven_batch,
-                     union_networks, write_labelled, write_pressures)
+from .netgen import (WORKLOAD_OPTIONS, WORKLOADS, NetworkCase, VesselNetwork, make_network, read_network_case,
+                     uneven_batch, union_networks, workload_network, write_labelled, write_pressures)
 
-__all__ = ["AMGSolver", "SolveReport", "PressureFit", "VesselNetwork", "make_network", "WORKLOADS", "union_networks", "uneven_batch",
+__all__ = ["AMGSolver", "SolveReport", "PressureFit", "VesselNetwork", "make_network", "workload_network", "WORKLOADS",
+           "WORKLOAD_OPTIONS", "union_networks", "uneven_batch",
            "NetworkCase", "read_network_case", "write_pressures", "write_labelled",
            "device_available", "last_error", "lib"]

@@ -87,7 +87,7 @@ class NetParams(C.Structure):
     _fields_ = [("L", C.c_int), ("depth", C.c_int), ("n_trees", C.c_int),
                 ("seed", C.c_uint64), ("p_art", C.c_double), ("p_ven", C.c_double),
                 ("mu", C.c_double), ("spacing", C.c_double), ("cap_r", C.c_double),
-                ("cap_sigma", C.c_double), ("root_r", C.c_double)]
+                ("cap_sigma", C.c_double), ("root_r", C.c_double), ("boundary_roots", C.c_int)]
 
 
 class Network(C.Structure):

@@ -7,6 +7,7 @@
  *   1          capillary radii, one normal per lattice segment
  *   16 + t     tree t radius factors U(0.9,1.1), heap order
  *   8          leaf-attachment targets (partial Fisher-Yates over L^3)
+ *   9          boundary_roots: each tree's boundary-face root node
  * Segment order: lattice segments (node ascending; +x, +y, +z), then for
  * each tree (arterial trees, then venous) its parent->child segments in
  * heap order of the child, then its leaf attachments in heap order.
@@ -71,6 +72,68 @@ void bn_netgen_default_params(bn_netgen_params *p) {
   p->cap_r = 4e-6;
   p->cap_sigma = 0.2;
   p->root_r = 500e-6;
+  p->boundary_roots = 0;
+}
+
+/* boundary_roots: tree t is rooted at a distinct random node of the
+ * lattice's boundary faces; its `leaves` targets are the first lattice nodes
+ * not taken by an earlier tree in breadth-first order from that node
+ * (neighbours visited -x, +x, -y, +y, -z, +z).  target[] receives them tree
+ * by tree.  Returns 0, or -1 on allocation failure. */
+static int boundary_rooted_targets(long long L, int ntrees, long long leaves, uint64_t seed, int *target) {
+  const long long NL = L * L * L;
+  unsigned char *taken = (unsigned char *)calloc((size_t)NL, 1);
+  int *stamp = (int *)malloc((size_t)NL * sizeof(int));
+  int *queue = (int *)malloc((size_t)NL * sizeof(int));
+  long long nb = 0;
+  for (long long id = 0; id < NL; id++) {
+    const long long x = id % L, y = (id / L) % L, z = id / (L * L);
+    nb += (x == 0 || x == L - 1 || y == 0 || y == L - 1 || z == 0 || z == L - 1);
+  }
+  int *face = (int *)malloc((size_t)(nb > 0 ? nb : 1) * sizeof(int));
+  unsigned char *root_used = (unsigned char *)calloc((size_t)(nb > 0 ? nb : 1), 1);
+  if (!taken || !stamp || !queue || !face || !root_used || nb < ntrees) {
+    free(taken); free(stamp); free(queue); free(face); free(root_used);
+    return -1;
+  }
+  nb = 0;
+  for (long long id = 0; id < NL; id++) {
+    const long long x = id % L, y = (id / L) % L, z = id / (L * L);
+    if (x == 0 || x == L - 1 || y == 0 || y == L - 1 || z == 0 || z == L - 1) face[nb++] = (int)id;
+  }
+  for (long long i = 0; i < NL; i++) stamp[i] = -1;
+  rng_t r;
+  rng_seed(&r, seed, 9);
+  const long long stride[3] = {1, L, L * L};
+  long long k = 0;
+  for (int t = 0; t < ntrees; t++) {
+    long long f;
+    do { f = (long long)rng_below(&r, (uint64_t)nb); } while (root_used[f]);
+    root_used[f] = 1;
+    long long head = 0, tail = 0, got = 0;
+    queue[tail++] = face[f];
+    stamp[face[f]] = t;
+    while (head < tail && got < leaves) {
+      const int id = queue[head++];
+      if (!taken[id]) {
+        taken[id] = 1;
+        target[k++] = id;
+        got++;
+      }
+      const long long c[3] = {id % L, (id / L) % L, id / (L * L)};
+      for (int dir = 0; dir < 3; dir++)
+        for (int sgn = -1; sgn <= 1; sgn += 2) {
+          const long long cc = c[dir] + sgn;
+          if (cc < 0 || cc >= L) continue;
+          const long long nid = id + sgn * stride[dir];
+          if (stamp[nid] == t) continue;
+          stamp[nid] = t;
+          queue[tail++] = (int)nid;
+        }
+    }
+  }
+  free(taken); free(stamp); free(queue); free(face); free(root_used);
+  return 0;
 }
 
 void bn_netgen_conductance(const bn_network *net, const double *radius,
@@ -185,10 +248,17 @@ int bn_netgen_build(const bn_netgen_params *p, bn_network *net) {
     }
   }
 
-  /* leaf-attachment targets: distinct lattice nodes (partial Fisher-Yates) */
+  /* leaf-attachment targets: distinct lattice nodes (partial Fisher-Yates),
+   * or the nodes nearest to each tree's boundary-face root */
   long long n_attach = 2LL * T * leaves;
   int *target = (int *)malloc((size_t)n_attach * sizeof(int));
-  {
+  if (p->boundary_roots) {
+    if (!target || boundary_rooted_targets(L, 2 * T, leaves, p->seed, target)) {
+      free(target);
+      bn_netgen_free(net);
+      return -1;
+    }
+  } else {
     int *perm = (int *)malloc((size_t)NL * sizeof(int));
     if (!perm || !target) { free(perm); free(target); bn_netgen_free(net); return -1; }
     for (long long i = 0; i < NL; i++) perm[i] = (int)i;

@@ -12,7 +12,8 @@ import numpy as np
 
 from . import _lib as L
 
-__all__ = ["VesselNetwork", "make_network", "WORKLOADS", "union_networks", "uneven_batch", "NetworkCase",
+__all__ = ["VesselNetwork", "make_network", "workload_network", "WORKLOADS", "WORKLOAD_OPTIONS",
+           "union_networks", "uneven_batch", "NetworkCase",
            "read_network_case", "write_pressures", "write_labelled"]
 
 # name -> (lattice edge, tree depth, trees per kind, coarsening)
@@ -21,6 +22,12 @@ WORKLOADS = {
     "lattice100_trees15": (100, 15, 1, "pmis"),  # 1,065,532 unknowns
     "lattice160_trees17": (160, 17, 1, "hmis"),  # 4,358,140 unknowns
     "lattice100_16x16trees11": (100, 11, 16, "pmis"),  # 1,065,472 unknowns
+}
+# generator options beyond (lattice edge, depth, trees): the many-tree
+# network roots each tree at a random boundary-face node, its leaves
+# attached to the nearest free lattice nodes
+WORKLOAD_OPTIONS = {
+    "lattice100_16x16trees11": {"boundary_roots": 1},
 }
 
 
@@ -106,6 +113,12 @@ def make_network(L_edge, depth, n_trees=1, seed=0, **overrides):
         return VesselNetwork(net, p)
     finally:
         L.lib.bn_netgen_free(C.byref(net))
+
+
+def workload_network(name, seed=0):
+    """The named workload's network (WORKLOADS, WORKLOAD_OPTIONS)."""
+    Ledge, depth, ntrees, _ = WORKLOADS[name]
+    return make_network(Ledge, depth, ntrees, seed=seed, **WORKLOAD_OPTIONS.get(name, {}))
 
 
 def union_networks(nets, rhs=None):

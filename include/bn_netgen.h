@@ -32,6 +32,12 @@ typedef struct {
   double   cap_r;      /* capillary median radius [m]                      */
   double   cap_sigma;  /* lognormal sigma of capillary radius              */
   double   root_r;     /* tree root radius [m]                             */
+  int      boundary_roots; /* 0: every leaf attaches to a distinct uniformly
+                          random lattice node; 1: each tree is rooted at a
+                          distinct random boundary-face lattice node and its
+                          leaves attach to the free lattice nodes nearest to
+                          it (breadth-first from that node), so every tree
+                          perfuses the region it enters                    */
 } bn_netgen_params;
 
 typedef struct {

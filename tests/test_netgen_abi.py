@@ -157,3 +157,34 @@ def test_union_of_networks_is_block_diagonal():
     # sizes follow the requested ranges; every network fits its leaves
     Ls = [round((net.n_nodes - 2 * (2 ** net.params.depth - 1)) ** (1 / 3)) for net in nets]
     assert all(4 <= Le <= 9 for Le in Ls)
+
+
+def test_boundary_rooted_trees_perfuse_their_entry_region():
+    # 4 + 4 trees of depth 6 on a 20^3 lattice, each rooted at a boundary-face
+    # node: its 32 leaves attach to the nearest free lattice nodes
+    from blood_network_amd import make_network
+    Le, d, T = 20, 6, 4
+    net = make_network(Le, d, T, seed=3, boundary_roots=1)
+    tn, leaves = 2 ** d - 1, 2 ** (d - 1)
+    assert net.n == Le ** 3 + 2 * T * (tn - 1)
+    e = 3 * Le * Le * (Le - 1)
+    targets = []
+    for t in range(2 * T):
+        e += tn - 1
+        ids = net.ev[e:e + leaves]
+        e += leaves
+        c = np.stack([ids % Le, (ids // Le) % Le, ids // (Le * Le)], 1)
+        # the first target is the root's boundary-face node
+        assert np.any((c[0] == 0) | (c[0] == Le - 1))
+        # breadth-first from it: every later target is at least as far (in
+        # lattice steps, up to nodes an earlier tree took) and all lie close
+        dist = np.abs(c - c[0]).sum(1)
+        assert dist.max() <= 2 * leaves ** (1 / 3) + 4
+        targets.append(ids)
+    allt = np.concatenate(targets)
+    assert len(np.unique(allt)) == len(allt)          # distinct across trees
+    # the uniform form spreads the same leaves over the whole lattice
+    flat = make_network(Le, d, T, seed=3)
+    ids = flat.ev[3 * Le * Le * (Le - 1) + tn - 1:3 * Le * Le * (Le - 1) + tn - 1 + leaves]
+    c = np.stack([ids % Le, (ids // Le) % Le, ids // (Le * Le)], 1)
+    assert (c.max(0) - c.min(0)).min() > Le // 2
