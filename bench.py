@@ -24,7 +24,10 @@ sys.path.insert(0, ROOT)
 
 METRIC = "PCG time to 1e-8 rel. residual, 4M-node net: FP32-AMG vs FP64 GPU; HBM GB/s"
 HBM_PEAK_GBS = 8000.0
-TRAFFIC_IC_JACOBI = {"lattice160_trees17": 238058915}  # measured HBM bytes per launch
+# HBM bytes per launch of the roofline kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE
+# over its launches in one solve, each counter scaled by its reading for a
+# copy kernel of known size), by (workload, kernel symbol)
+TRAFFIC = {}
 FAST_COLOURS = 16    # GS on levels with <= 16 colours, l1-Jacobi above
 FAST_IC_SWEEPS = 4   # Jacobi sweeps per IC(0) triangle
 
@@ -414,6 +417,10 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--rtol", type=float, default=None,
                     help="PCG tolerance (default 1e-8; 1e-10 B-norm for --descent)")
+    ap.add_argument("--counters", action="store_true",
+                    help="for counter runs: build the hierarchy, run --steps solves on the benchmark's path "
+                         "with every kernel enqueued eagerly, then 5 plain FP64 copies of n values (the "
+                         "byte-counter calibration), print one JSON line")
     ap.add_argument("--kernels-only", action="store_true",
                     help="build the hierarchy, time the fine-level kernels and the copy "
                          "calibration kernel, print them as one JSON line (for counter runs)")
@@ -476,6 +483,19 @@ def main():
             raise RuntimeError(L.last_error())
         return ms.value, by.value
 
+    if args.counters:
+        os.environ["BN_NO_GRAPH"] = "1"
+        d_b = L.DeviceArray(net.b)
+        d_x = L.DeviceArray(np.zeros(net.n))
+        its = []
+        for _ in range(args.steps):
+            d_x.upload(np.zeros(net.n))
+            its.append(solver.solve_device(d_b.ptr, d_x.ptr, tol=args.rtol, maxiter=1000))
+        copy_ms, copy_b = tk(5, reps=5)
+        print(json.dumps({"workload": args.workload, "n": net.n, "solves": args.steps, "iterations": its,
+                          "copy_bytes": copy_b, "copy_us": round(copy_ms * 1e3, 2)}), flush=True)
+        return
+
     if args.kernels_only:
         names = ("fine_spmv_fp64", "fine_gs_sweep", "fine_residual", "fine_ic", "preconditioner", "copy_fp64",
                  "fine_ic_jacobi_sweep")
@@ -535,12 +555,24 @@ def main():
     log(f"[bench] fp64 plain: {it64} its {rep64.solve_ms:.2f} ms; fp64 same layout: {it64j} its "
         f"{rep64j.solve_ms:.2f} ms")
 
-    # roofline of the dominant kernel, timed live on the solver's stream
+    # kernel census of one more solve on the benchmark's path: every launch
+    # of its bandwidth kernels timed by HIP events on the solver's stream as
+    # the solve runs (and each launch shape again cold); the roofline kernel
+    # is the symbol with the most device time per solve
+    solver.set_mode(**fp32)
+    d_x.upload(zeros)
+    census = solver.kernel_census(d_b.ptr, d_x.ptr, tol=args.rtol, reps=3)
+    top = census[0]
+    ach = top["bytes"] / (top["us_solve"] * 1e-6) / 1e9
+    ach_cold = top["bytes"] / (top["us_cold"] * 1e-6) / 1e9
+    for c in census[:8]:
+        log(f"[bench] census {c['name']}: {c['launches']} launches, {c['us_solve'] / 1e3:.3f} ms per solve "
+            f"({c['bytes'] / c['us_solve'] / 1e3:.0f} GB/s as run, {c['bytes'] / c['us_cold'] / 1e3:.0f} GB/s cold)")
+    # single fine-level kernels (HIP events over repeated launches)
     spmv_ms, spmv_b = tk(0)
     gs_ms, gs_b = tk(1)
     res_ms, res_b = tk(2)
     icj_ms, icj_b = tk(6)
-    ach = icj_b / (icj_ms * 1e-3) / 1e9
     kern = {"fine_ic_jacobi_sweep_fp32": (icj_ms, icj_b), "fine_residual_fp32": (res_ms, res_b),
             "fine_spmv_fp64": (spmv_ms, spmv_b), "fine_gs_sweep_fp32": (gs_ms, gs_b)}
     log("[bench] kernels: " + ", ".join(f"{k} {v[0]*1e3:.1f} us {v[1]/v[0]/1e6:.0f} GB/s"
@@ -561,11 +593,7 @@ def main():
         log(f"[bench] CPU: {cpu['value']} ms, {itref} iterations")
 
     value = float(np.median(dev_ms))
-    # HBM bytes per launch of the roofline kernel from rocprofv3 FETCH_SIZE
-    # and WRITE_SIZE on this workload, each scaled by the same counter's
-    # reading for a plain copy kernel of known size (FETCH x2.0, WRITE
-    # x1.0), recorded in profiles/ic_jacobi_sweep_lattice160_trees17.md
-    traffic = TRAFFIC_IC_JACOBI.get(args.workload)
+    traffic = TRAFFIC.get((args.workload, top["name"]))
     out = {
         "metric": METRIC,
         "value": round(value, 3),
@@ -591,11 +619,17 @@ def main():
             "relres_fp32": relres32, "rel_err_vs_fp64_1e-12": err_vs64, "fp64_1e-12_iterations": it64_12,
             "kernels_us": {k: round(v[0] * 1e3, 2) for k, v in kern.items()},
             "kernels_GBps": {k: round(v[1] / v[0] / 1e6, 1) for k, v in kern.items()},
+            "kernel_census": [{"kernel": c["name"], "launches": c["launches"], "ms": round(c["us_solve"] / 1e3, 3),
+                               "GBps": round(c["bytes"] / c["us_solve"] / 1e3, 1),
+                               "GBps_cold": round(c["bytes"] / c["us_cold"] / 1e3, 1)} for c in census[:10]],
         },
-        "roofline": {"bound": "hbm",
-                     "kernel": "k_jacobi_rows<float,1>: one Jacobi sweep of the fine IC(0) triangle",
+        "roofline": {"bound": "hbm", "kernel": top["name"],
                      "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic},
+                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic,
+                     "launches_per_solve": top["launches"],
+                     "bytes_per_launch": round(top["bytes"] / top["launches"]),
+                     "us_per_launch": round(top["us_solve"] / top["launches"], 2),
+                     "cold": {"achieved": round(ach_cold, 1), "frac": round(ach_cold / HBM_PEAK_GBS, 4)}},
         "cpu_baseline": cpu,
     }
     if cpu:

@@ -82,6 +82,12 @@ class SolveInfo(C.Structure):
                 ("upload_ms", C.c_double), ("num_levels", C.c_int)]
 
 
+class KernelStat(C.Structure):
+    """bn_kernel_stat (include/bn_amg_ext.h): one kernel symbol's census."""
+    _fields_ = [("name", C.c_char * 80), ("launches", C.c_int), ("shapes", C.c_int), ("bytes", C.c_double),
+                ("us_solve", C.c_double), ("us_cold", C.c_double)]
+
+
 class NetParams(C.Structure):
     """bn_netgen_params (include/bn_netgen.h)."""
     _fields_ = [("L", C.c_int), ("depth", C.c_int), ("n_trees", C.c_int),
@@ -144,6 +150,8 @@ def _load():
         "bn_solver_schedule": (C.c_int, [VP, C.c_int, IP, IP, IP]),
         "bn_solver_ic_levels": (C.c_int, [VP, IP, IP]),
         "bn_solver_time_kernel": (C.c_int, [VP, C.c_int, C.c_int, DP, DP]),
+        "bn_solver_kernel_census": (C.c_int, [VP, VP, VP, C.c_double, C.c_int, C.c_int, C.POINTER(KernelStat),
+                                              C.c_int, IP]),
         "bn_dev_alloc": (C.c_int, [C.POINTER(VP), C.c_size_t]),
         "bn_dev_free": (C.c_int, [VP]),
         "bn_dev_copy": (C.c_int, [VP, VP, C.c_size_t, C.c_int]),
@@ -191,6 +199,7 @@ def _load():
         "bn_csr_apply": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
         "bn_csr_apply_binned": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, C.c_int,
                                         C.c_int, VP, VP]),
+        "bn_gs_color_blocks": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
         "bn_gs_color_map": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, VP, C.c_int, VP, VP, VP]),
         "bn_gs_color_zero_map": (None, [C.c_char, C.c_int, C.c_int, C.c_int, VP, VP, VP, VP, VP]),
         "bn_gs_color_zero_colour": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP]),
@@ -200,8 +209,8 @@ def _load():
                                      C.POINTER(IP)]),
         "bn_csr_apply_seg": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, C.c_int, C.c_int,
                                      VP, VP, VP, C.c_int, C.c_int, VP, VP]),
-        "bn_csr_apply_blocks": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP]),
-        "bn_csr_resid_blocks_scaled": (None, [C.c_char, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
+        "bn_csr_apply_blocks": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
+        "bn_csr_resid_blocks_scaled": (None, [C.c_char, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
         "bn_row_blocks": (C.c_int, [IP, C.c_int, C.c_int, C.c_int, IP]),
         "bn_colour_order": (C.c_int, [C.POINTER(CSR), IP, C.c_int, IP, C.POINTER(IP)]),
         "bn_level_sets": (C.c_int, [IP, IP, C.c_int, C.c_int, IP, C.POINTER(IP)]),

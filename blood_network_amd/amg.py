@@ -304,6 +304,19 @@ class AMGSolver:
         self.report = SolveReport(info)
         return its
 
+    def kernel_census(self, b_ptr, x_ptr, tol=1e-8, maxiter=1000, reps=3, max_symbols=64):
+        """One solve (device pointers, x the initial guess) with every
+        bandwidth-kernel launch timed by HIP events, then each launch shape
+        timed cold; returns a list of dicts per kernel symbol, slowest first:
+        name, launches, shapes, bytes, us_solve, us_cold (per solve)."""
+        out = (L.KernelStat * max_symbols)()
+        n = C.c_int(0)
+        self._check(L.lib.bn_solver_kernel_census(self.h, C.c_void_p(b_ptr), C.c_void_p(x_ptr), tol, maxiter, reps,
+                                                  out, max_symbols, C.byref(n)))
+        res = [{"name": o.name.decode(), "launches": o.launches, "shapes": o.shapes, "bytes": o.bytes,
+                "us_solve": o.us_solve, "us_cold": o.us_cold} for o in out[:n.value]]
+        return sorted(res, key=lambda r: -r["us_solve"])
+
     def level_dims(self, lev):
         d = np.zeros(8, np.int32)
         self._check(L.lib.bn_solver_level_dims(self.h, lev, L.ip(d)))

@@ -34,6 +34,7 @@ typedef struct {
   double *seg_part;/* device: partial sums of multi-segment rows       */
   int nblk;        /* row blocks for the adaptive kernel               */
   int *blk;        /* device: nblk + 1 row boundaries                  */
+  int *blkq;       /* device: their entry offsets, blkq[k] = ia[blk[k]] */
   int adapt;       /* FP64 apply uses the row-block kernel (autotuned) */
   int adapt_s;     /* FP32 likewise                                    */
   int deep;        /* FP64 SpMV load-round variant (BN_LANES_DEEP* bit) */
@@ -78,7 +79,11 @@ typedef struct {
   double *dinvc_d;      /* 1/a_ii in Ac row order */
   float *dinvc_s;
   int gs_lanes, gs_lanes_s; /* lanes per row of the colour kernels, timed on
-                               whole sweeps at upload (0: Ac's SpMV choice) */
+                               whole sweeps at upload (0: Ac's SpMV choice;
+                               BN_GS_BLOCKS: the row-block colour kernel) */
+  int gs_nblk;          /* row blocks of Ac, colour by colour */
+  int *gs_blk, *gs_blkq;/* device: their row bounds and entry offsets */
+  int *gs_cblk;         /* host: first block of each colour (ncol + 1) */
   int *perm_h;          /* host perm[new] = old */
   int *perm_d;
   void *b, *x, *r;      /* level work vectors in V-cycle precision */
@@ -175,7 +180,11 @@ struct bn_solver {
   bn_block blk;         /* block solve work space */
   void *segs;           /* cached segmentations of segmented solves (segments.c) */
   void *rb;             /* index maps of the device value-only rebuild (rebuild.c) */
+  struct bn_census *census; /* set while bn_solver_kernel_census records a solve */
 };
+
+/* gs_lanes value selecting the row-block colour kernel */
+#define BN_GS_BLOCKS 1024
 
 /* lanes per row for precision vt */
 static inline int bn_lanes(const bn_dcsr *m, char vt) { return vt == 's' ? m->lanes_s : m->lanes; }
@@ -209,7 +218,7 @@ static inline void bn_dcsr_apply_dot(const bn_dcsr *m, char vt, int op, const vo
   if (vt == 'd' && m->adapt && m->blk) {
     /* the row-block kernel was timed fastest: run it, then the dot (the
      * same deterministic block partials the fused form writes) */
-    bn_csr_apply_blocks('d', op, m->nblk, m->blk, m->ia, m->ja, m->vd, x, y, b, st);
+    bn_csr_apply_blocks('d', op, m->nblk, m->blk, m->blkq, m->ia, m->ja, m->vd, x, y, b, st);
     if (partial) bn_dot_partial(m->rows, (const double *)dotw, (const double *)y, NULL, NULL, partial, NULL, st);
     return;
   }
@@ -232,12 +241,28 @@ static inline void bn_dcsr_apply(const bn_dcsr *m, char vt, int op, int lanes, c
 static inline void bn_op_apply(const bn_dcsr *m, char vt, int op, const void *x, void *y, const void *b,
                                hipStream_t st) {
   if (vt == 's' ? m->adapt_s : m->adapt) {
-    bn_csr_apply_blocks(vt, op, m->nblk, m->blk, m->ia, m->ja, vt == 's' ? (const void *)m->vs : (const void *)m->vd,
+    bn_csr_apply_blocks(vt, op, m->nblk, m->blk, m->blkq, m->ia, m->ja, vt == 's' ? (const void *)m->vs : (const void *)m->vd,
                         x, y, b, st);
     return;
   }
   bn_dcsr_apply(m, vt, op, bn_spmv_lanes(m, vt), x, y, b, st);
 }
+
+/* census.c: the V-cycle's and the outer PCG's bandwidth kernels, launched
+ * on the solver's stream (recorded while a kernel census runs) */
+void bn_launch_apply(struct bn_solver *s, const bn_dcsr *M, char vt, int op, const void *x, void *y, const void *b);
+void bn_launch_resid_scaled(struct bn_solver *s, const bn_dcsr *M, char vt, const void *x, void *y, const void *b,
+                            const void *dsc, void *ys);
+void bn_launch_gs(struct bn_solver *s, const bn_dlevel *d, char vt, int c, int lanes, const void *b, void *x,
+                  const void *zbase, void *zsum);
+/* one Jacobi sweep of triangle T: xout = (b - T_offdiag xin) dinv, or
+ * zsum = zbase + that, or with xs also xs = xout dinv */
+void bn_launch_jacobi(struct bn_solver *s, const bn_dcsr *T, char vt, const void *dinv, const void *b, const void *xin,
+                      void *xout, const void *zbase, void *zsum, void *xs);
+/* FP64 y = op(A, x [, b]) of the outer PCG, with the dotw.y partials */
+void bn_launch_spmv(struct bn_solver *s, const bn_dcsr *A, int op, const void *x, void *y, const void *b,
+                    const void *dotw, double *partial);
+int bn_ensure_device(struct bn_solver *s);
 
 /* segments.c */
 void bn_segments_release(struct bn_solver *s);
@@ -265,6 +290,10 @@ int bn_device_ok(void);
 int bn_vc_prepare(struct bn_solver *s);
 /* -1: the smoother runs no exact-order sweep, else the bn_sweep slot */
 int bn_vc_exact_kind(const struct bn_solver *s);
+/* one colour c of level d's multicolour sweep with the given colour-kernel
+ * choice (lanes per row, or BN_GS_BLOCKS) */
+void bn_gs_colour(struct bn_solver *s, const bn_dlevel *d, char vt, int c, int lanes, const void *b, void *x,
+                  const void *zbase, void *zsum);
 void bn_vc_backslash(struct bn_solver *s, int lev, const void *b, void *x, int x_zero);
 void bn_vc_fwdslash(struct bn_solver *s, int lev, const void *b, void *x, int x_zero);
 void bn_vc_ichol_precond(struct bn_solver *s, const double *g, double *z);

@@ -149,11 +149,14 @@ static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
     int *bk = (int *)bn_xmalloc(((size_t)m->row + 1) * sizeof(int));
     d->nblk = bn_row_blocks(m->ia, m->row, BN_ROWBLOCK_NNZ, BN_ROWBLOCK_ROWS, bk);
     d->blk = (int *)dmalloc(s, ((size_t)d->nblk + 1) * sizeof(int));
-    if (!d->blk) {
+    d->blkq = (int *)dmalloc(s, ((size_t)d->nblk + 1) * sizeof(int));
+    if (!d->blk || !d->blkq) {
       free(bk);
       return BN_ERR_HIP;
     }
     CK(hipMemcpy(d->blk, bk, ((size_t)d->nblk + 1) * sizeof(int), hipMemcpyHostToDevice));
+    for (int k = 0; k <= d->nblk; k++) bk[k] = m->ia[bk[k]];
+    CK(hipMemcpy(d->blkq, bk, ((size_t)d->nblk + 1) * sizeof(int), hipMemcpyHostToDevice));
     free(bk);
     d->adapt = d->adapt_s = 0;
   }
@@ -283,7 +286,7 @@ static float time_launches(struct bn_solver *s, const bn_tune_env *env, bn_dcsr 
     if (cand > 0)
       bn_dcsr_apply(d, vt, BN_OP_SET, cand, x, y, NULL, s->stream);
     else
-      bn_csr_apply_blocks(vt, BN_OP_SET, d->nblk, d->blk, d->ia, d->ja, val, x, y, NULL, s->stream);
+      bn_csr_apply_blocks(vt, BN_OP_SET, d->nblk, d->blk, d->blkq, d->ia, d->ja, val, x, y, NULL, s->stream);
     if (r >= 0) hipEventRecord(env->ev[2 * r + 1], s->stream);
   }
   hipEventSynchronize(env->ev[2 * reps - 1]);
@@ -300,10 +303,12 @@ static float time_launches(struct bn_solver *s, const bn_tune_env *env, bn_dcsr 
  * cold forward sweeps (all colours back to back, as the V-cycle runs
  * them), FP64 and FP32 */
 static void tune_gs(struct bn_solver *s, const bn_tune_env *env, bn_dlevel *d, void *x, void *b) {
-  const int opts[5] = {1, 2, 4, 8, 16};
+  enum { NOPT = 6 };
+  const int opts[NOPT] = {1, 2, 4, 8, 16, BN_GS_BLOCKS};
   /* BN_GS_LANES pins the colour kernels' lanes per row (1, 2, 4, 8 or
-   * 16): one value for every level, or a comma-separated list by level
-   * (levels past the list, or with 0, are timed) */
+   * 16, or 1024 for the row-block kernel): one value for every level, or a
+   * comma-separated list by level (levels past the list, or with 0, are
+   * timed) */
   const char *pin = getenv("BN_GS_LANES");
   if (pin && pin[0]) {
     const int lev = (int)(d - s->lev);
@@ -316,7 +321,7 @@ static void tune_gs(struct bn_solver *s, const bn_tune_env *env, bn_dlevel *d, v
       }
       l = c ? atoi(c) : 0;
     }
-    for (int o = 0; o < 5; o++)
+    for (int o = 0; o < NOPT; o++)
       if (opts[o] == l) {
         d->gs_lanes = d->gs_lanes_s = l;
         return;
@@ -324,22 +329,18 @@ static void tune_gs(struct bn_solver *s, const bn_tune_env *env, bn_dlevel *d, v
   }
   for (int p = 0; p < 2; p++) {
     const char vt = p ? 's' : 'd';
-    const void *val = p ? (const void *)d->Ac.vs : (const void *)d->Ac.vd;
-    const void *dinvc = p ? (const void *)d->dinvc_s : (const void *)d->dinvc_d;
     int best_l = 0;
-    float best = 1e30f, cand_ms[5];
+    float best = 1e30f, cand_ms[NOPT];
     /* two passes over the candidates, each keeping its better pass, so
      * that a drift in clocks or cache state across the candidate order
      * does not decide between two close choices */
-    for (int o10 = 0; o10 < 10; o10++) {
-      const int o = o10 % 5, pass = o10 / 5;
+    for (int o10 = 0; o10 < 2 * NOPT; o10++) {
+      const int o = o10 % NOPT, pass = o10 / NOPT;
       const int lanes = opts[o];
       for (int r = -1; r < BN_TUNE_REPS; r++) {
         if (env->flush) bn_dot_partial(env->nflush, env->flush, env->flush, NULL, NULL, s->part1, NULL, s->stream);
         if (r >= 0) hipEventRecord(env->ev[2 * r], s->stream);
-        for (int c = 0; c < d->ncol; c++)
-          bn_gs_color_map(vt, d->col_ptr[c], d->col_ptr[c + 1], d->cmap_d, d->Ac.ia, d->Ac.ja, val, dinvc, b, x,
-                          lanes, NULL, NULL, s->stream);
+        for (int c = 0; c < d->ncol; c++) bn_gs_colour(s, d, vt, c, lanes, b, x, NULL, NULL);
         if (r >= 0) hipEventRecord(env->ev[2 * r + 1], s->stream);
       }
       hipEventSynchronize(env->ev[2 * BN_TUNE_REPS - 1]);
@@ -355,7 +356,7 @@ static void tune_gs(struct bn_solver *s, const bn_tune_env *env, bn_dlevel *d, v
                 lanes, pass, 1e3 * total / BN_TUNE_REPS);
       if (pass == 0 || total < cand_ms[o]) cand_ms[o] = total;
     }
-    for (int o = 0; o < 5; o++)
+    for (int o = 0; o < NOPT; o++)
       if (cand_ms[o] < best) best = cand_ms[o], best_l = opts[o];
     const char *lg = getenv("BN_TUNE_LOG");
     if (lg && (lg[0] == '1' || lg[0] == '2'))
@@ -403,7 +404,7 @@ static void autotune_csr(struct bn_solver *s, const bn_tune_env *env, bn_dcsr *d
   if (allow_blocks && d->blk) cand[nc++] = 0; /* 0: the row-block kernel */
   for (int pv = 0; pv < 2; pv++) {
     const char vt = pv ? 's' : 'd';
-    float best = 1e30f, best_lanes_ms = 1e30f;
+    float best = 1e30f, best_lanes_ms = 1e30f, blocks_ms = 1e30f;
     int bl = base, use_blocks = 0;
     /* a few cold launches each; more for small operators, whose times
      * are short and noisier */
@@ -421,11 +422,13 @@ static void autotune_csr(struct bn_solver *s, const bn_tune_env *env, bn_dcsr *d
         best_lanes_ms = ms;
         bl = cand[k];
       }
-      if (ms < best) {
-        best = ms;
-        use_blocks = cand[k] == 0;
-      }
+      if (cand[k] == 0) blocks_ms = ms;
     }
+    /* the row-block kernel unless a lanes kernel beats it by more than 3%:
+     * timing noise between close candidates then does not flip the choice
+     * from one run to the next */
+    use_blocks = blocks_ms <= 1.03f * best_lanes_ms;
+    best = use_blocks ? blocks_ms : best_lanes_ms;
     const int lanes = bl & BN_LANES_MASK, deep = bl & ~BN_LANES_MASK;
     const char *lg = getenv("BN_TUNE_LOG");
     if (lg && (lg[0] == '1' || lg[0] == '2'))
@@ -676,26 +679,60 @@ static int upload_chunks(struct bn_solver *s, bn_dlevel *d, const dcsr_mat *Ap, 
   return 0;
 }
 
-/* Colour-ordered row copy of a GS level: row r of Ac is row cperm[r] of
- * Ap with its columns unchanged, plus the row map and 1/a_ii in that
- * order. */
+/* Colour-ordered row copy of a GS level: row r of Ac is the off-diagonal
+ * part of row cperm[r] of Ap (columns unchanged; the sweep divides by the
+ * diagonal through 1/a_ii, so a row's diagonal entry is never read), plus
+ * the row map, 1/a_ii in that order and the row blocks of each colour. */
 static int upload_colour_rows(struct bn_solver *s, bn_dlevel *d, const dcsr_mat *Ap, const int *cperm) {
   const int n = Ap->row;
   dcsr_mat Ac;
-  bn_csr_permute(Ap, cperm, NULL, &Ac);
+  bn_csr_offdiag_rows(Ap, cperm, &Ac);
   int rc = upload_csr(s, &Ac, &d->Ac);
   double *dc = (double *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(double));
-  /* row r of Ac holds the diagonal of level row cperm[r]; 1.0 when
+  /* 1/a_ii of level row cperm[r]: its last diagonal entry, 1.0 when
    * absent, as diag_inverses */
-#pragma omp parallel for schedule(static) if (Ac.nnz > 65536)
+#pragma omp parallel for schedule(static) if (Ap->nnz > 65536)
   for (int r = 0; r < n; r++) {
+    const int i = cperm[r];
     double dg = 1.0;
-    for (int q = Ac.ia[r]; q < Ac.ia[r + 1]; q++)
-      if (Ac.ja[q] == cperm[r]) dg = Ac.val[q];
+    for (int q = Ap->ia[i]; q < Ap->ia[i + 1]; q++)
+      if (Ap->ja[q] == i) dg = Ap->val[q];
     dc[r] = 1.0 / dg;
   }
   if (!rc) rc = upload_vec2(s, dc, n, &d->dinvc_d, &d->dinvc_s);
   free(dc);
+  /* row blocks colour by colour: a block never holds rows of two colours */
+  if (!rc) {
+    int *bk = (int *)bn_xmalloc(((size_t)n + 2) * sizeof(int));
+    int *cb = (int *)bn_xmalloc(((size_t)d->ncol + 1) * sizeof(int));
+    int nb = 0;
+    bk[0] = 0;
+    for (int c = 0; c < d->ncol; c++) {
+      const int lo = d->col_ptr[c], hi = d->col_ptr[c + 1];
+      cb[c] = nb;
+      if (hi > lo) {
+        const int k = bn_row_blocks(Ac.ia + lo, hi - lo, BN_ROWBLOCK_NNZ, BN_ROWBLOCK_ROWS, bk + nb);
+        for (int t = 1; t <= k; t++) bk[nb + t] += lo;
+        bk[nb] = lo;
+        nb += k;
+      }
+    }
+    cb[d->ncol] = nb;
+    d->gs_nblk = nb;
+    d->gs_cblk = cb;
+    bn_track_host(s, cb);
+    d->gs_blk = (int *)dmalloc(s, ((size_t)nb + 1) * sizeof(int));
+    d->gs_blkq = (int *)dmalloc(s, ((size_t)nb + 1) * sizeof(int));
+    if (!d->gs_blk || !d->gs_blkq) {
+      free(bk);
+      bn_csr_free(&Ac);
+      return BN_ERR_HIP;
+    }
+    CK(hipMemcpy(d->gs_blk, bk, ((size_t)nb + 1) * sizeof(int), hipMemcpyHostToDevice));
+    for (int k = 0; k <= nb; k++) bk[k] = Ac.ia[bk[k]];
+    CK(hipMemcpy(d->gs_blkq, bk, ((size_t)nb + 1) * sizeof(int), hipMemcpyHostToDevice));
+    free(bk);
+  }
   bn_csr_free(&Ac);
   if (rc) return rc;
   d->cmap_d = (int *)dmalloc(s, (size_t)(n > 0 ? n : 1) * sizeof(int));

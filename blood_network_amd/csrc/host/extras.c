@@ -338,14 +338,13 @@ int bn_solver_time_kernel(void *h, int which, int reps, double *ms_avg, double *
                       bn_lanes(&d->A, vt), NULL, NULL, st);
         else
           for (int c = 0; c < d->ncol; c++)
-            bn_gs_color_map(vt, d->col_ptr[c], d->col_ptr[c + 1], d->cmap_d, d->Ac.ia, d->Ac.ja,
-                            vt == 's' ? (const void *)d->Ac.vs : (const void *)d->Ac.vd,
-                            vt == 's' ? (const void *)d->dinvc_s : (const void *)d->dinvc_d, s->r1, s->u1,
-                            (vt == 's' ? d->gs_lanes_s : d->gs_lanes) > 0
-                                ? (vt == 's' ? d->gs_lanes_s : d->gs_lanes)
-                                : bn_lanes(&d->Ac, vt),
-                            NULL, NULL, st);
-        b = (double)d->A.nnz * (4.0 + v) + (n + 1.0) * 4.0 + 3.0 * n * v + n * v; /* + dinv */
+            bn_gs_colour(s, d, vt, c, vt == 's' ? d->gs_lanes_s : d->gs_lanes, s->r1, s->u1, NULL, NULL);
+        /* the colour-ordered copy's off-diagonal entries and row pointers,
+         * the row map, b, x read and written, 1/a_ii */
+        if (d->use_gs && !d->block)
+          b = (double)d->Ac.nnz * (4.0 + v) + (n + 1.0) * 4.0 + 4.0 * n + 3.0 * n * v + n * v;
+        else
+          b = (double)d->A.nnz * (4.0 + v) + (n + 1.0) * 4.0 + 3.0 * n * v + n * v;
         break;
       }
       case 2: {

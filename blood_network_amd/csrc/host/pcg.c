@@ -44,7 +44,7 @@ static void enqueue_step(struct bn_solver *s) {
   if (f32) bn_pcg_update_p_s(n, s->p, (const float *)s->zout, sc, st);
   else bn_pcg_update_p(n, s->p, s->z, sc, st);
   /* q = A p, p'q -> alpha */
-  bn_dcsr_apply_dot(A, 'd', BN_OP_SET, s->p, s->q, NULL, s->p, s->part1, st);
+  bn_launch_spmv(s, A, BN_OP_SET, s->p, s->q, NULL, s->p, s->part1);
   bn_scalar_finish(1, s->part1, NULL, NULL, sc, 0, st);
   if (f32) {
     bn_pcg_update_xr_s(n, s->x, s->p, s->r, s->q, sc, s->part2, (float *)s->g_in, st);
@@ -111,7 +111,7 @@ int bn_pcg_run(struct bn_solver *s, const double *d_b, double *d_x, double tol, 
   bn_permute(n, s->lev[0].perm_d, d_b, s->bb, 0, st);
   bn_permute(n, s->lev[0].perm_d, d_x, s->x, 0, st);
   /* r = b - A x ; ||b||^2 */
-  bn_dcsr_apply_dot(A, 'd', BN_OP_RESID, s->x, s->r, s->bb, NULL, NULL, st);
+  bn_launch_spmv(s, A, BN_OP_RESID, s->x, s->r, s->bb, NULL, NULL);
   bn_dot_partial(n, s->bb, s->bb, NULL, NULL, s->part1, NULL, st);
   bn_scalar_finish(3, s->part1, NULL, NULL, sc, 0, st);
   bn_vc_precond(s, s->r, s->z);

@@ -504,7 +504,7 @@ static int plan_build(struct bn_solver *s) {
       dcsr_mat I = *A, Ic;
       I.val = (double *)bn_xmalloc((size_t)(A->nnz > 0 ? A->nnz : 1) * sizeof(double));
       for (int k = 0; k < A->nnz; k++) I.val[k] = (double)k;
-      bn_csr_permute(&I, cperm, NULL, &Ic);
+      bn_csr_offdiag_rows(&I, cperm, &Ic); /* the upload's off-diagonal copy */
       int *m = (int *)bn_xmalloc((size_t)(A->nnz > 0 ? A->nnz : 1) * sizeof(int));
       for (int k = 0; k < Ic.nnz; k++) m[k] = (int)Ic.val[k];
       if (!rc) r->acmap = rb_upload_int(s, m, (size_t)Ic.nnz);
@@ -705,7 +705,8 @@ int bn_solver_rebuild_values_device(void *h, int nnz_in, const double *d_val) {
                       pl->l1 + pl->l1off[l], st);
     if (d->use_gs) {
       bn_rb_gather(d->Ac.nnz, pl->lev[l].acmap, d->A.vd, d->Ac.vd, d->Ac.vs, st);
-      bn_rb_dinv_map(d->n, d->Ac.ia, d->Ac.ja, d->Ac.vd, d->cmap_d, d->dinvc_d, d->dinvc_s, st);
+      /* 1/a_ii in colour order: the level's own 1/a_ii gathered by the row map */
+      bn_rb_gather(d->n, d->cmap_d, d->dinv_d, d->dinvc_d, d->dinvc_s, st);
     }
   }
   if (!s->asolve_shared) {

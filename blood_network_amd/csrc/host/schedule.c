@@ -247,6 +247,35 @@ int bn_sweep_levels(const dcsr_mat *A, const int *order, int backward, int *rows
   return nlev;
 }
 
+void bn_csr_offdiag_rows(const dcsr_mat *A, const int *perm_r, dcsr_mat *B) {
+  const int n = A->row;
+  int *cnt = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+#pragma omp parallel for schedule(static) if (A->nnz > 65536)
+  for (int k = 0; k < n; k++) {
+    const int i = perm_r ? perm_r[k] : k;
+    int c = 0;
+    for (int q = A->ia[i]; q < A->ia[i + 1]; q++) c += (A->ja[q] != i);
+    cnt[k] = c;
+  }
+  int *ptr = (int *)bn_xmalloc(((size_t)n + 1) * sizeof(int));
+  bn_prefix(cnt, n, ptr);
+  free(cnt);
+  bn_csr_alloc(B, A->row, A->col, ptr[n]);
+  memcpy(B->ia, ptr, ((size_t)n + 1) * sizeof(int));
+  free(ptr);
+#pragma omp parallel for schedule(static) if (A->nnz > 65536)
+  for (int k = 0; k < n; k++) {
+    const int i = perm_r ? perm_r[k] : k;
+    int p = B->ia[k];
+    for (int q = A->ia[i]; q < A->ia[i + 1]; q++)
+      if (A->ja[q] != i) {
+        B->ja[p] = A->ja[q];
+        B->val[p] = A->val[q];
+        p++;
+      }
+  }
+}
+
 void bn_csr_permute(const dcsr_mat *A, const int *perm_r, const int *inv_c, dcsr_mat *B) {
   const int n = A->row;
   bn_csr_alloc(B, A->row, A->col, A->nnz);

@@ -53,6 +53,11 @@ int bn_chunk_order(const dcsr_mat *A, const int *mis, int split, int nchunks, in
  * nblk. */
 int bn_row_blocks(const int *ia, int n, int max_nnz, int max_rows, int *blocks);
 
+/* B = A's off-diagonal part with rows taken in perm_r order (NULL:
+ * identity): row k of B is row perm_r[k] of A without its entries in
+ * column perm_r[k], in A's entry order; columns keep A's numbering. */
+void bn_csr_offdiag_rows(const dcsr_mat *A, const int *perm_r, dcsr_mat *B);
+
 /* B = A with rows taken in perm_r order and columns renamed by inv_c
  * (either may be NULL for identity); entry order within a row is kept. */
 void bn_csr_permute(const dcsr_mat *A, const int *perm_r, const int *inv_c, dcsr_mat *B);

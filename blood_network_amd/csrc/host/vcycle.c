@@ -37,7 +37,7 @@ static inline const void *vecv(const double *vd, const float *vs, char vt) {
 
 /* y = op(M, x [, b]) in precision vt */
 static void apply(struct bn_solver *s, const bn_dcsr *M, int op, const void *x, void *y, const void *b) {
-  bn_op_apply(M, s->vt, op, x, y, b, s->stream);
+  bn_launch_apply(s, M, s->vt, op, x, y, b);
 }
 
 /* Block smoother: nu sweeps that start from xin (NULL: zero) and end in
@@ -119,6 +119,20 @@ static void smooth_exact(struct bn_solver *s, int lev, int kind, const void *b, 
   }
 }
 
+void bn_gs_colour(struct bn_solver *s, const bn_dlevel *d, char vt, int c, int lanes, const void *b, void *x,
+                  const void *zbase, void *zsum) {
+  const void *dinvc = vecv(d->dinvc_d, d->dinvc_s, vt);
+  if (lanes == BN_GS_BLOCKS && d->gs_cblk) {
+    const int b0 = d->gs_cblk[c];
+    bn_gs_color_blocks(vt, b0, d->gs_cblk[c + 1] - b0, d->gs_blk, d->gs_blkq, d->Ac.ia, d->Ac.ja, opv(&d->Ac, vt),
+                       d->cmap_d, dinvc, b, x, zbase, zsum, s->stream);
+    return;
+  }
+  if (lanes <= 0 || lanes == BN_GS_BLOCKS) lanes = bn_lanes(&d->Ac, vt);
+  bn_gs_color_map(vt, d->col_ptr[c], d->col_ptr[c + 1], d->cmap_d, d->Ac.ia, d->Ac.ja, opv(&d->Ac, vt), dinvc, b, x,
+                  lanes, zbase, zsum, s->stream);
+}
+
 /* nu smoothing sweeps on level `lev`; backward runs colours in reverse.
  * x_zero: x is zero on entry (first colour then needs no row sums).
  * zsum != NULL: the final sweep also writes zsum = zbase + x. */
@@ -162,9 +176,8 @@ static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int bac
             bn_gs_color_zero_map(vt, lo, hi, n, d->cmap_d, dinvc, b, x, s->stream);
           continue;
         }
-        const int gl = vt == 's' ? d->gs_lanes_s : d->gs_lanes;
-        bn_gs_color_map(vt, lo, hi, d->cmap_d, d->Ac.ia, d->Ac.ja, opv(&d->Ac, vt), dinvc, b, x,
-                        gl > 0 ? gl : bn_lanes(&d->Ac, vt), last ? zbase : NULL, last ? zsum : NULL, s->stream);
+        bn_launch_gs(s, d, vt, c, vt == 's' ? d->gs_lanes_s : d->gs_lanes, b, x, last ? zbase : NULL,
+                     last ? zsum : NULL);
       }
     }
   } else {
@@ -294,31 +307,23 @@ static void tri_jacobi(struct bn_solver *s, bn_dtri *t, char vt, const void *b, 
   const int n = t->n;
   void *bufs[2] = {y, tmp};
   int cur = 0;
-  if (!first_done)
-    bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, NULL, bufs[cur], bn_lanes(&t->lo, vt), NULL,
-                   NULL, s->stream);
+  if (!first_done) bn_launch_jacobi(s, &t->lo, vt, dinv, b, NULL, bufs[cur], NULL, NULL, NULL);
   /* the last lower sweep also writes the upper triangle's first step
    * (yy_i dinv_i) into x, which the upper sweeps start from */
   for (int k = 1; k < sweeps; k++) {
-    if (k == sweeps - 1)
-      bn_jacobi_rows_scaled(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, bufs[cur], bufs[1 - cur], x,
-                            bn_lanes(&t->lo, vt), s->stream);
-    else
-      bn_jacobi_rows(vt, n, t->lo.ia, t->lo.ja, opv(&t->lo, vt), dinv, b, bufs[cur], bufs[1 - cur],
-                     bn_lanes(&t->lo, vt), NULL, NULL, s->stream);
+    if (k == sweeps - 1) bn_launch_jacobi(s, &t->lo, vt, dinv, b, bufs[cur], bufs[1 - cur], NULL, NULL, x);
+    else bn_launch_jacobi(s, &t->lo, vt, dinv, b, bufs[cur], bufs[1 - cur], NULL, NULL, NULL);
     cur = 1 - cur;
   }
   const void *yy = bufs[cur];
   void *xb[2] = {x, bufs[1 - cur]};
   int cx = 0;
   const int fuse = zsum && sweeps > 1;
-  if (sweeps < 2)
-    bn_jacobi_rows(vt, n, t->up.ia, t->up.ja, opv(&t->up, vt), dinv, yy, NULL, xb[cx], bn_lanes(&t->up, vt), NULL,
-                   NULL, s->stream);
+  if (sweeps < 2) bn_launch_jacobi(s, &t->up, vt, dinv, yy, NULL, xb[cx], NULL, NULL, NULL);
   for (int k = 1; k < sweeps; k++) {
     const int last = (k == sweeps - 1);
-    bn_jacobi_rows(vt, n, t->up.ia, t->up.ja, opv(&t->up, vt), dinv, yy, xb[cx], xb[1 - cx],
-                   bn_lanes(&t->up, vt), fuse && last ? zbase : NULL, fuse && last ? zsum : NULL, s->stream);
+    bn_launch_jacobi(s, &t->up, vt, dinv, yy, xb[cx], xb[1 - cx], fuse && last ? zbase : NULL,
+                     fuse && last ? zsum : NULL, NULL);
     cx = 1 - cx;
   }
   if (fuse) return;
@@ -391,9 +396,8 @@ static void ichol_core(struct bn_solver *s, const void *gv, void *zv) {
   const int jac = (s->opt.fine_ic == BN_IC_JACOBI);
   const int first = jac && (vt == 's' ? A->adapt_s : A->adapt) && A->blk && s->fine_ic.n == s->lev[0].n;
   if (first)
-    bn_csr_resid_blocks_scaled(vt, A->nblk, A->blk, A->ia, A->ja, opv(A, vt), s->u1, s->r1, gv,
-                               vt == 's' ? (const void *)s->fine_ic.dinv_s : (const void *)s->fine_ic.dinv_d, s->icy,
-                               s->stream);
+    bn_launch_resid_scaled(s, A, vt, s->u1, s->r1, gv,
+                           vt == 's' ? (const void *)s->fine_ic.dinv_s : (const void *)s->fine_ic.dinv_d, s->icy);
   else
     apply(s, A, BN_OP_RESID, s->u1, s->r1, gv);
   if (jac)

@@ -48,6 +48,11 @@ enum {
  * device current); 0 or a BN_ERR_* code */
 int bn_scratch_ptr(void **ptr, size_t *bytes);
 
+/* Arms a kernel timer: the calling thread's next launch from kernels.hip
+ * records its start and end into the two events (the dispatch's own
+ * timestamps); NULL, NULL disarms it. */
+void bn_time_next_launch(hipEvent_t start, hipEvent_t stop);
+
 /* Lanes-per-row SpMV variants: the lanes argument of bn_csr_apply and
  * bn_csr_apply_seg may carry one of these bits.  Each lane then issues 8
  * (DEEP8) or 16 (DEEP16) index and value loads before its x gathers
@@ -104,14 +109,16 @@ void bn_csr_apply_seg(char vt, int op, int n, const int *ia, const int *ja,
  * their products staged in LDS and summed per row by a power-of-two
  * group of threads; a single over-long row is summed by the whole block.
  * Load balance is independent of the row-length distribution. */
-void bn_csr_apply_blocks(char vt, int op, int nblk, const int *blocks,
+/* blkq (may be NULL) holds each block boundary's entry offset,
+ * blkq[k] = ia[blocks[k]], which saves a block one dependent load. */
+void bn_csr_apply_blocks(char vt, int op, int nblk, const int *blocks, const int *blkq,
                          const int *ia, const int *ja, const void *val,
                          const void *x, void *y, const void *b,
                          hipStream_t stream);
 /* y = b - A x by the row-block kernel, also writing ys_r = y_r dsc_r */
-void bn_csr_resid_blocks_scaled(char vt, int nblk, const int *blocks, const int *ia, const int *ja,
-                                const void *val, const void *x, void *y, const void *b, const void *dsc,
-                                void *ys, hipStream_t stream);
+void bn_csr_resid_blocks_scaled(char vt, int nblk, const int *blocks, const int *blkq, const int *ia,
+                                const int *ja, const void *val, const void *x, void *y, const void *b,
+                                const void *dsc, void *ys, hipStream_t stream);
 
 /* One multicolour Gauss-Seidel colour: rows lo..hi-1 of a level whose
  * rows are stored in colour order:
@@ -137,6 +144,15 @@ void bn_block_gs(char vt, int nchunk, int max_rows, const int *chunk, const int 
  * ia/ja/val, columns in natural numbering) updating natural entry
  * i = map[r]: x_i <- (b_i - sum_{j != i} a_rj x_j) * dinv_r, dinv in
  * colour order.  Vectors stay in natural order. */
+/* One multicolour Gauss-Seidel colour by row blocks: blocks b0 .. b0+nb-1
+ * of a row-block split of the colour-ordered copy (blocks never straddle
+ * two colours; blkq their entry offsets) hold that colour's rows; the copy
+ * holds off-diagonal entries only, row r updating x[map[r]]:
+ * x_i = (b_i - sum_j a_rj x_j) dinv_r.  zsum != NULL: also zsum_i = zbase_i
+ * + x_i. */
+void bn_gs_color_blocks(char vt, int b0, int nb, const int *blocks, const int *blkq, const int *ia, const int *ja,
+                        const void *val, const int *map, const void *dinv, const void *b, void *x, const void *zbase,
+                        void *zsum, hipStream_t stream);
 void bn_gs_color_map(char vt, int lo, int hi, const int *map, const int *ia, const int *ja,
                      const void *val, const void *dinv, const void *b, void *x,
                      int lanes, const void *zbase, void *zsum, hipStream_t stream);

@@ -12,12 +12,29 @@
 //    rows then hit the same XCD's L2.
 //  * Reductions are deterministic: per-block partials in fixed order,
 //    finished by one block in fixed order (no floating-point atomics).
+#include <hip/hip_ext.h>
 #include <hip/hip_runtime.h>
 #include <math.h>
 
 #include "bn_kernels.h"
 
 #define BLOCK 256
+
+// A kernel timer armed by bn_time_next_launch: the next launch of this
+// thread records its own start and end into the two events (the dispatch
+// packet's timestamps, as a profiler reads them) and disarms it.
+static thread_local hipEvent_t g_time_start = nullptr, g_time_stop = nullptr;
+
+#define BN_LAUNCH(K, G, B, S, ST, ...)                                            \
+  do {                                                                            \
+    if (g_time_start) {                                                           \
+      hipEvent_t t0_ = g_time_start, t1_ = g_time_stop;                           \
+      g_time_start = g_time_stop = nullptr;                                       \
+      hipExtLaunchKernelGGL(K, G, B, S, ST, t0_, t1_, 0, __VA_ARGS__);            \
+    } else {                                                                      \
+      hipLaunchKernelGGL(K, G, B, S, ST, __VA_ARGS__);                            \
+    }                                                                             \
+  } while (0)
 
 namespace {
 
@@ -214,7 +231,7 @@ void launch_csr(int op, int n, const int *ia, const int *ja, const T *val, const
   constexpr int RPB = BLOCK / LANES;
   const long long nrb = ((long long)n + RPB - 1) / RPB;
 #define BN_CSR_LAUNCH(OPV, DOTV, G, CH) \
-  hipLaunchKernelGGL((k_csr<T, LANES, OPV, DOTV, R>), dim3(G), dim3(BLOCK), 0, st, n, CH, ia, ja, val, x, y, b, w, partial, long_len)
+  BN_LAUNCH((k_csr<T, LANES, OPV, DOTV, R>), dim3(G), dim3(BLOCK), 0, st, n, CH, ia, ja, val, x, y, b, w, partial, long_len)
   // fused dot: a capped grid so the partials fit BN_RED_BLOCKS slots.
   // Past that many row blocks each block would walk several of them in
   // turn, which measured slower than the one-row-block-per-block launch
@@ -342,10 +359,10 @@ void launch_seg(int op, int nlong, int nmulti, const int *rows, const int *seg_p
   if (nlong <= 0 || nseg <= 0) return;
   const dim3 gf((unsigned)((nmulti + BLOCK - 1) / BLOCK));
 #define BN_SEG(OPV)                                                                                          \
-  hipLaunchKernelGGL((k_csr_seg<T, OPV>), dim3(nseg), dim3(BLOCK), 0, st, seg_row, seg_ptr, rows, ia, ja, val, x, \
+  BN_LAUNCH((k_csr_seg<T, OPV>), dim3(nseg), dim3(BLOCK), 0, st, seg_row, seg_ptr, rows, ia, ja, val, x, \
                      y, b, part);                                                                             \
   if (nmulti > 0)                                                                                            \
-    hipLaunchKernelGGL((k_seg_finish<T, OPV>), gf, dim3(BLOCK), 0, st, nmulti, seg_ptr, rows, part, y, b)
+    BN_LAUNCH((k_seg_finish<T, OPV>), gf, dim3(BLOCK), 0, st, nmulti, seg_ptr, rows, part, y, b)
   if (op == BN_OP_SET) {
     BN_SEG(BN_OP_SET);
   } else if (op == BN_OP_RESID) {
@@ -361,11 +378,11 @@ void launch_long(int op, int nlong, const int *rows, const int *ia, const int *j
                  T *y, const T *b, hipStream_t st) {
   if (nlong <= 0) return;
   if (op == BN_OP_SET)
-    hipLaunchKernelGGL((k_csr_long<T, BN_OP_SET>), dim3(nlong), dim3(BLOCK), 0, st, rows, ia, ja, val, x, y, b);
+    BN_LAUNCH((k_csr_long<T, BN_OP_SET>), dim3(nlong), dim3(BLOCK), 0, st, rows, ia, ja, val, x, y, b);
   else if (op == BN_OP_RESID)
-    hipLaunchKernelGGL((k_csr_long<T, BN_OP_RESID>), dim3(nlong), dim3(BLOCK), 0, st, rows, ia, ja, val, x, y, b);
+    BN_LAUNCH((k_csr_long<T, BN_OP_RESID>), dim3(nlong), dim3(BLOCK), 0, st, rows, ia, ja, val, x, y, b);
   else
-    hipLaunchKernelGGL((k_csr_long<T, BN_OP_ADD>), dim3(nlong), dim3(BLOCK), 0, st, rows, ia, ja, val, x, y, b);
+    BN_LAUNCH((k_csr_long<T, BN_OP_ADD>), dim3(nlong), dim3(BLOCK), 0, st, rows, ia, ja, val, x, y, b);
 }
 
 // ---------------------------------------------------------------------
@@ -378,8 +395,17 @@ void launch_long(int op, int nlong, const int *rows, const int *ia, const int *j
 // SCALED (RESID only): also write ys_r = y_r dsc_r, the first Jacobi step
 // from zero of a triangle whose right-hand side is this residual (the
 // product k_diag_scale would form)
+//
+// Latency: a block's work is three dependent memory round trips.  (1) The
+// block's row bounds and, from blkq (the entry offsets ia[blocks[k]],
+// stored beside them), its entry bounds; (2) the entries' columns and
+// values, and - issued with them - the bounds of the row this thread
+// reduces and its b (or y) and dsc; (3) the x gathers.  Without blkq the
+// entry bounds come from ia, one round trip more.  The reduction reads
+// LDS only.
 template <typename T, int OP, bool SCALED = false>
 __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__restrict__ blocks,
+                                                      const int *__restrict__ blkq,
                                                       const int *__restrict__ ia, const int *__restrict__ ja,
                                                       const T *__restrict__ val, const T *__restrict__ x,
                                                       T *__restrict__ y, const T *__restrict__ b,
@@ -390,22 +416,38 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__res
   const int bid = logical_block();
   if (bid >= nblk) return;
   const int r0 = blocks[bid], r1 = blocks[bid + 1];
-  const int q0 = ia[r0], q1 = ia[r1];
+  const int q0 = blkq ? blkq[bid] : ia[r0];
+  const int q1 = blkq ? blkq[bid + 1] : ia[r1];
   const int cnt = q1 - q0;
   if (cnt > BN_ROWBLOCK_NNZ) {
     // vector mode: one long row (r1 == r0 + 1)
+    const T base = OP == BN_OP_RESID ? b[r0] : OP == BN_OP_ADD ? y[r0] : T(0);
     const T s = row_dot_deep<T>(q0 + threadIdx.x, q1, ja, val, x);
     const double t = block_sum((double)s);
     if (threadIdx.x == 0) {
       if (OP == BN_OP_SET) y[r0] = (T)t;
       else if (OP == BN_OP_RESID) {
-        const T out = b[r0] - (T)t;
+        const T out = base - (T)t;
         y[r0] = out;
         if (SCALED) ys[r0] = out * dsc[r0];
-      } else y[r0] = y[r0] + (T)t;
+      } else y[r0] = base + (T)t;
     }
     return;
   }
+  // tpr = largest power of two <= 256 / rows, at most one wave; with at
+  // most 256 rows a block (bn_row_blocks' limit) one pass reduces them all
+  const int nr = r1 - r0;
+  int lg = nr >= BLOCK ? 0 : 31 - __clz(BLOCK / nr);
+  if (lg > 6) lg = 6;
+  const int tpr = 1 << lg;
+  const int sub = threadIdx.x & (tpr - 1);
+  // the first pass's row, its bounds and right-hand side, loaded now so
+  // they arrive with the entries (a clamped row for threads without one)
+  const int rf = r0 + (threadIdx.x >> lg);
+  const int rc = rf < r1 ? rf : r1 - 1;
+  const int e0 = ia[rc] - q0, e1 = ia[rc + 1] - q0;
+  const T base0 = OP == BN_OP_RESID ? b[rc] : OP == BN_OP_ADD ? y[rc] : T(0);
+  const T dsc0 = SCALED ? dsc[rc] : T(0);
   if (cnt > 0) {
     // every slot loads (past the block's entries: its last entry, with a
     // zero coefficient), so all loads and gathers are in flight together
@@ -428,42 +470,136 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__res
     }
   }
   __syncthreads();
-  // tpr = largest power of two <= 256 / rows, at most one wave
-  const int nr = r1 - r0;
-  int lg = nr >= BLOCK ? 0 : 31 - __clz(BLOCK / nr);
-  if (lg > 6) lg = 6;
-  const int tpr = 1 << lg;
-  const int sub = threadIdx.x & (tpr - 1);
   for (int rb = r0; rb < r1; rb += BLOCK >> lg) {
     const int r = rb + (threadIdx.x >> lg);
-    T s = 0;
-    if (r < r1) {
-      const int e = ia[r + 1] - q0;
-      for (int k = ia[r] - q0 + sub; k < e; k += tpr) s += prod[k];
+    const bool first = (rb == r0);
+    int ks = e0, ke = e1;
+    if (!first && r < r1) {
+      ks = ia[r] - q0;
+      ke = ia[r + 1] - q0;
     }
+    T s = 0;
+    if (r < r1)
+      for (int k = ks + sub; k < ke; k += tpr) s += prod[k];
     for (int o = tpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
     if (sub == 0 && r < r1) {
+      const T base = first ? base0 : OP == BN_OP_RESID ? b[r] : OP == BN_OP_ADD ? y[r] : T(0);
       if (OP == BN_OP_SET) y[r] = s;
       else if (OP == BN_OP_RESID) {
-        const T out = b[r] - s;
+        const T out = base - s;
         y[r] = out;
-        if (SCALED) ys[r] = out * dsc[r];
-      } else y[r] = y[r] + s;
+        if (SCALED) ys[r] = out * (first ? dsc0 : dsc[r]);
+      } else y[r] = base + s;
     }
   }
 }
 
 template <typename T>
-void launch_blocks(int op, int nblk, const int *blocks, const int *ia, const int *ja, const T *val, const T *x,
-                   T *y, const T *b, hipStream_t st) {
+void launch_blocks(int op, int nblk, const int *blocks, const int *blkq, const int *ia, const int *ja, const T *val,
+                   const T *x, T *y, const T *b, hipStream_t st) {
   if (nblk <= 0) return;
   const int g = grid_rows(nblk);
+  const T *no_dsc = nullptr;
+  T *no_ys = nullptr;
   if (op == BN_OP_SET)
-    hipLaunchKernelGGL((k_csr_blocks<T, BN_OP_SET>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, ia, ja, val, x, y, b);
+    BN_LAUNCH((k_csr_blocks<T, BN_OP_SET>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
+              no_dsc, no_ys);
   else if (op == BN_OP_RESID)
-    hipLaunchKernelGGL((k_csr_blocks<T, BN_OP_RESID>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, ia, ja, val, x, y, b);
+    BN_LAUNCH((k_csr_blocks<T, BN_OP_RESID>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
+              no_dsc, no_ys);
   else
-    hipLaunchKernelGGL((k_csr_blocks<T, BN_OP_ADD>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, ia, ja, val, x, y, b);
+    BN_LAUNCH((k_csr_blocks<T, BN_OP_ADD>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
+              no_dsc, no_ys);
+}
+
+// ---------------------------------------------------------------------
+// One multicolour Gauss-Seidel colour by row blocks.  The colour's rows
+// are rows lo..hi-1 of a colour-ordered copy of the level operator that
+// holds only the off-diagonal entries (row r updates level row map[r]);
+// blocks b0 .. b0 + nb - 1 of `blocks`/`blkq` cover exactly those rows.
+// Same three round trips as k_csr_blocks: block bounds, then the entries
+// with the row's map, b, dinv (and zbase), then the x gathers; each row
+// is summed from LDS in a fixed order.
+//   x_i = (b_i - sum_j a_rj x_j) dinv_r,   i = map[r]
+// ---------------------------------------------------------------------
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_gs_blocks(int b0, int nb, const int *__restrict__ blocks,
+                                                     const int *__restrict__ blkq, const int *__restrict__ ja,
+                                                     const T *__restrict__ val, const int *__restrict__ ia,
+                                                     const int *__restrict__ map, const T *__restrict__ dinv,
+                                                     const T *__restrict__ b, T *__restrict__ x,
+                                                     const T *__restrict__ zbase, T *__restrict__ zsum) {
+  constexpr int PER = BN_ROWBLOCK_NNZ / BLOCK;
+  __shared__ T prod[BN_ROWBLOCK_NNZ + 1];
+  const int lb = logical_block();
+  if (lb >= nb) return;
+  const int bid = b0 + lb;
+  const int r0 = blocks[bid], r1 = blocks[bid + 1];
+  const int q0 = blkq[bid], q1 = blkq[bid + 1];
+  const int cnt = q1 - q0;
+  if (cnt > BN_ROWBLOCK_NNZ) {
+    // one long row
+    const int i = map[r0];
+    const T bi = b[i], dr = dinv[r0];
+    const T s = row_dot_deep<T>(q0 + threadIdx.x, q1, ja, val, x);
+    const double t = block_sum((double)s);
+    if (threadIdx.x == 0) {
+      const T v = (bi - (T)t) * dr;
+      x[i] = v;
+      if (zsum) zsum[i] = zbase[i] + v;
+    }
+    return;
+  }
+  const int nr = r1 - r0;
+  int lg = nr >= BLOCK ? 0 : 31 - __clz(BLOCK / nr);
+  if (lg > 6) lg = 6;
+  const int tpr = 1 << lg;
+  const int sub = threadIdx.x & (tpr - 1);
+  const int rf = r0 + (threadIdx.x >> lg);
+  const int rc = rf < r1 ? rf : r1 - 1;
+  const int e0 = ia[rc] - q0, e1 = ia[rc + 1] - q0;
+  const int i0 = map[rc];
+  const T b0v = b[i0], d0v = dinv[rc];
+  const T z0v = zsum ? zbase[i0] : T(0);
+  if (cnt > 0) {
+    int jj[PER];
+    T aa[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int k = threadIdx.x + u * BLOCK;
+      const int kk = k < cnt ? k : cnt - 1;
+      jj[u] = ja[q0 + kk];
+      aa[u] = val[q0 + kk];
+    }
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+      const int k = threadIdx.x + u * BLOCK;
+      const T hm = k < cnt ? T(1) : T(0);
+      prod[k < cnt ? k : BN_ROWBLOCK_NNZ] = (aa[u] * hm) * x[jj[u]];
+    }
+  }
+  // every gather of this colour is done before any of its rows is written
+  // (rows of one colour are uncoupled, so no row reads another's result)
+  __syncthreads();
+  for (int rb = r0; rb < r1; rb += BLOCK >> lg) {
+    const int r = rb + (threadIdx.x >> lg);
+    const bool first = (rb == r0);
+    int ks = e0, ke = e1;
+    if (!first && r < r1) {
+      ks = ia[r] - q0;
+      ke = ia[r + 1] - q0;
+    }
+    T s = 0;
+    if (r < r1)
+      for (int k = ks + sub; k < ke; k += tpr) s += prod[k];
+    for (int o = tpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    if (sub == 0 && r < r1) {
+      const int i = first ? i0 : map[r];
+      const T v = ((first ? b0v : b[i]) - s) * (first ? d0v : dinv[r]);
+      x[i] = v;
+      if (zsum) zsum[i] = (first ? z0v : zbase[i]) + v;
+    }
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -505,7 +641,7 @@ void dispatch_rowsolve(int lo, int cnt, const int *rows, const int *ia, const in
                        const T *dinv, const T *b, T *y, int lanes, const T *zb, T *zs, hipStream_t st) {
   if (cnt <= 0) return;
 #define BN_RS(L)                                                                                   \
-  hipLaunchKernelGGL((k_rowsolve<T, L, MODE>), dim3(grid_rows((cnt + BLOCK / L - 1) / (BLOCK / L))), dim3(BLOCK), \
+  BN_LAUNCH((k_rowsolve<T, L, MODE>), dim3(grid_rows((cnt + BLOCK / L - 1) / (BLOCK / L))), dim3(BLOCK), \
                      0, st, lo, cnt, rows, ia, ja, val, dinv, b, y, zb, zs)
   switch (lanes) {
     case 1: BN_RS(1); break;
@@ -599,7 +735,7 @@ void launch_block_gs(int nchunk, int max_rows, const int *chunk, const int *ccol
                      const T *zb, T *zs, hipStream_t st) {
   const size_t sh = (size_t)max_rows * sizeof(T);
 #define BN_BGS(BW, Z)                                                                                        \
-  hipLaunchKernelGGL((k_block_gs<T, LANES, BW, Z>), dim3(nchunk), dim3(GS_BLOCK), sh, st, chunk, ccol, cbnd, ia, \
+  BN_LAUNCH((k_block_gs<T, LANES, BW, Z>), dim3(nchunk), dim3(GS_BLOCK), sh, st, chunk, ccol, cbnd, ia, \
                      ja, val, dinv, b, xold, xnew, zb, zs)
   if (backward) {
     if (xold) BN_BGS(true, false);
@@ -656,8 +792,8 @@ template <typename T>
 void dispatch_jacobi_rows(int n, const int *ia, const int *ja, const T *val, const T *dinv, const T *b,
                           const T *xin, T *xout, int lanes, const T *zb, T *zs, hipStream_t st) {
 #define BN_JR(L)                                                                                       \
-  hipLaunchKernelGGL((k_jacobi_rows<T, L>), dim3(grid_rows((n + BLOCK / L - 1) / (BLOCK / L))), dim3(BLOCK), 0, st, \
-                     n, ia, ja, val, dinv, b, xin, xout, zb, zs)
+  BN_LAUNCH((k_jacobi_rows<T, L>), dim3(grid_rows((n + BLOCK / L - 1) / (BLOCK / L))), dim3(BLOCK), 0, st, \
+                     n, ia, ja, val, dinv, b, xin, xout, zb, zs, (T *)nullptr)
   switch (lanes) {
     case 1: BN_JR(1); break;
     case 2: BN_JR(2); break;
@@ -674,7 +810,7 @@ template <typename T>
 void dispatch_jacobi_rows_scaled(int n, const int *ia, const int *ja, const T *val, const T *dinv, const T *b,
                                  const T *xin, T *xout, T *xs, int lanes, hipStream_t st) {
 #define BN_JRS(L)                                                                                        \
-  hipLaunchKernelGGL((k_jacobi_rows<T, L, true>), dim3(grid_rows((n + BLOCK / L - 1) / (BLOCK / L))), dim3(BLOCK), \
+  BN_LAUNCH((k_jacobi_rows<T, L, true>), dim3(grid_rows((n + BLOCK / L - 1) / (BLOCK / L))), dim3(BLOCK), \
                      0, st, n, ia, ja, val, dinv, b, xin, xout, (const T *)nullptr, (T *)nullptr, xs)
   switch (lanes) {
     case 1: BN_JRS(1); break;
@@ -949,6 +1085,11 @@ inline dim3 g1(long long n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); 
 // =====================================================================
 extern "C" {
 
+void bn_time_next_launch(hipEvent_t start, hipEvent_t stop) {
+  g_time_start = start;
+  g_time_stop = stop;
+}
+
 void bn_csr_apply(char vt, int op, int n, const int *ia, const int *ja, const void *val, const void *x,
                   void *y, const void *b, int lanes, const void *dotw, double *partial,
                   hipStream_t stream) {
@@ -982,27 +1123,28 @@ void bn_csr_apply_binned(char vt, int op, int n, const int *ia, const int *ja, c
   }
 }
 
-void bn_csr_resid_blocks_scaled(char vt, int nblk, const int *blocks, const int *ia, const int *ja, const void *val,
-                                const void *x, void *y, const void *b, const void *dsc, void *ys, hipStream_t stream) {
+void bn_csr_resid_blocks_scaled(char vt, int nblk, const int *blocks, const int *blkq, const int *ia, const int *ja,
+                                const void *val, const void *x, void *y, const void *b, const void *dsc, void *ys,
+                                hipStream_t stream) {
   if (nblk <= 0) return;
   const int g = grid_rows(nblk);
   if (vt == 's')
-    hipLaunchKernelGGL((k_csr_blocks<float, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, stream, nblk, blocks, ia, ja,
-                       (const float *)val, (const float *)x, (float *)y, (const float *)b, (const float *)dsc,
+    BN_LAUNCH((k_csr_blocks<float, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, stream, nblk, blocks, blkq,
+                       ia, ja, (const float *)val, (const float *)x, (float *)y, (const float *)b, (const float *)dsc,
                        (float *)ys);
   else
-    hipLaunchKernelGGL((k_csr_blocks<double, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, stream, nblk, blocks, ia,
-                       ja, (const double *)val, (const double *)x, (double *)y, (const double *)b,
+    BN_LAUNCH((k_csr_blocks<double, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, stream, nblk, blocks, blkq,
+                       ia, ja, (const double *)val, (const double *)x, (double *)y, (const double *)b,
                        (const double *)dsc, (double *)ys);
 }
 
-void bn_csr_apply_blocks(char vt, int op, int nblk, const int *blocks, const int *ia, const int *ja,
+void bn_csr_apply_blocks(char vt, int op, int nblk, const int *blocks, const int *blkq, const int *ia, const int *ja,
                          const void *val, const void *x, void *y, const void *b, hipStream_t stream) {
   if (vt == 's')
-    launch_blocks<float>(op, nblk, blocks, ia, ja, (const float *)val, (const float *)x, (float *)y,
+    launch_blocks<float>(op, nblk, blocks, blkq, ia, ja, (const float *)val, (const float *)x, (float *)y,
                          (const float *)b, stream);
   else
-    launch_blocks<double>(op, nblk, blocks, ia, ja, (const double *)val, (const double *)x, (double *)y,
+    launch_blocks<double>(op, nblk, blocks, blkq, ia, ja, (const double *)val, (const double *)x, (double *)y,
                           (const double *)b, stream);
 }
 
@@ -1053,6 +1195,21 @@ void bn_gs_color(char vt, int lo, int hi, const int *ia, const int *ja, const vo
                                      (const double *)zbase, (double *)zsum, stream);
 }
 
+void bn_gs_color_blocks(char vt, int b0, int nb, const int *blocks, const int *blkq, const int *ia, const int *ja,
+                        const void *val, const int *map, const void *dinv, const void *b, void *x, const void *zbase,
+                        void *zsum, hipStream_t stream) {
+  if (nb <= 0) return;
+  const int g = grid_rows(nb);
+  if (vt == 's')
+    BN_LAUNCH(k_gs_blocks<float>, dim3(g), dim3(BLOCK), 0, stream, b0, nb, blocks, blkq, ja,
+                       (const float *)val, ia, map, (const float *)dinv, (const float *)b, (float *)x,
+                       (const float *)zbase, (float *)zsum);
+  else
+    BN_LAUNCH(k_gs_blocks<double>, dim3(g), dim3(BLOCK), 0, stream, b0, nb, blocks, blkq, ja,
+                       (const double *)val, ia, map, (const double *)dinv, (const double *)b, (double *)x,
+                       (const double *)zbase, (double *)zsum);
+}
+
 void bn_gs_color_map(char vt, int lo, int hi, const int *map, const int *ia, const int *ja, const void *val,
                      const void *dinv, const void *b, void *x, int lanes, const void *zbase, void *zsum,
                      hipStream_t stream) {
@@ -1072,10 +1229,10 @@ void bn_gs_color_zero_map(char vt, int lo, int hi, int all_n, const int *map, co
   const int cnt = all_n - lo;
   if (cnt <= 0) return;
   if (vt == 's')
-    hipLaunchKernelGGL(k_diag_scale_map<float>, g1(cnt), dim3(BLOCK), 0, stream, lo, hi, all_n, map,
+    BN_LAUNCH(k_diag_scale_map<float>, g1(cnt), dim3(BLOCK), 0, stream, lo, hi, all_n, map,
                        (const float *)dinv, (const float *)b, (float *)x);
   else
-    hipLaunchKernelGGL(k_diag_scale_map<double>, g1(cnt), dim3(BLOCK), 0, stream, lo, hi, all_n, map,
+    BN_LAUNCH(k_diag_scale_map<double>, g1(cnt), dim3(BLOCK), 0, stream, lo, hi, all_n, map,
                        (const double *)dinv, (const double *)b, (double *)x);
 }
 
@@ -1083,10 +1240,10 @@ void bn_gs_color_zero_colour(char vt, int n, int c, const unsigned char *colour,
                              void *x, hipStream_t stream) {
   if (n <= 0) return;
   if (vt == 's')
-    hipLaunchKernelGGL(k_diag_scale_colour<float>, g1(n), dim3(BLOCK), 0, stream, n, c, colour,
+    BN_LAUNCH(k_diag_scale_colour<float>, g1(n), dim3(BLOCK), 0, stream, n, c, colour,
                        (const float *)dinv, (const float *)b, (float *)x);
   else
-    hipLaunchKernelGGL(k_diag_scale_colour<double>, g1(n), dim3(BLOCK), 0, stream, n, c, colour,
+    BN_LAUNCH(k_diag_scale_colour<double>, g1(n), dim3(BLOCK), 0, stream, n, c, colour,
                        (const double *)dinv, (const double *)b, (double *)x);
 }
 
@@ -1095,10 +1252,10 @@ void bn_gs_color_zero(char vt, int lo, int hi, const void *dinv, const void *b, 
   const int cnt = hi - lo;
   if (cnt <= 0) return;
   if (vt == 's')
-    hipLaunchKernelGGL(k_diag_scale<float>, g1(cnt), dim3(BLOCK), 0, stream, lo, cnt, (const float *)dinv,
+    BN_LAUNCH(k_diag_scale<float>, g1(cnt), dim3(BLOCK), 0, stream, lo, cnt, (const float *)dinv,
                        (const float *)b, (float *)x);
   else
-    hipLaunchKernelGGL(k_diag_scale<double>, g1(cnt), dim3(BLOCK), 0, stream, lo, cnt,
+    BN_LAUNCH(k_diag_scale<double>, g1(cnt), dim3(BLOCK), 0, stream, lo, cnt,
                        (const double *)dinv, (const double *)b, (double *)x);
 }
 
@@ -1135,10 +1292,10 @@ void bn_jacobi_update(char vt, int n, const void *r, const void *dscale, double 
                       hipStream_t stream) {
   if (n <= 0) return;
   if (vt == 's')
-    hipLaunchKernelGGL(k_jacobi_update<float>, g1(n), dim3(BLOCK), 0, stream, n, (const float *)r,
+    BN_LAUNCH(k_jacobi_update<float>, g1(n), dim3(BLOCK), 0, stream, n, (const float *)r,
                        (const float *)dscale, (float)omega, (float *)x);
   else
-    hipLaunchKernelGGL(k_jacobi_update<double>, g1(n), dim3(BLOCK), 0, stream, n, (const double *)r,
+    BN_LAUNCH(k_jacobi_update<double>, g1(n), dim3(BLOCK), 0, stream, n, (const double *)r,
                        (const double *)dscale, omega, (double *)x);
 }
 
@@ -1146,10 +1303,10 @@ void bn_jacobi_zero(char vt, int n, const void *b, const void *dscale, double om
                     hipStream_t stream) {
   if (n <= 0) return;
   if (vt == 's')
-    hipLaunchKernelGGL(k_jacobi_zero<float>, g1(n), dim3(BLOCK), 0, stream, n, (const float *)b,
+    BN_LAUNCH(k_jacobi_zero<float>, g1(n), dim3(BLOCK), 0, stream, n, (const float *)b,
                        (const float *)dscale, (float)omega, (float *)x);
   else
-    hipLaunchKernelGGL(k_jacobi_zero<double>, g1(n), dim3(BLOCK), 0, stream, n, (const double *)b,
+    BN_LAUNCH(k_jacobi_zero<double>, g1(n), dim3(BLOCK), 0, stream, n, (const double *)b,
                        (const double *)dscale, omega, (double *)x);
 }
 
@@ -1169,36 +1326,36 @@ void bn_trsv_level(char vt, int cnt, const int *rows, const int *ia, const int *
 void bn_dense_gemv(char vt, int n, const void *M, const void *x, void *y, hipStream_t stream) {
   if (n <= 0) return;
   if (vt == 's')
-    hipLaunchKernelGGL(k_gemv<float>, dim3(n), dim3(BLOCK), 0, stream, n, (const float *)M,
+    BN_LAUNCH(k_gemv<float>, dim3(n), dim3(BLOCK), 0, stream, n, (const float *)M,
                        (const float *)x, (float *)y);
   else
-    hipLaunchKernelGGL(k_gemv<double>, dim3(n), dim3(BLOCK), 0, stream, n, (const double *)M,
+    BN_LAUNCH(k_gemv<double>, dim3(n), dim3(BLOCK), 0, stream, n, (const double *)M,
                        (const double *)x, (double *)y);
 }
 
 void bn_fill(char vt, int n, void *x, double v, hipStream_t stream) {
   if (n <= 0) return;
   if (vt == 's')
-    hipLaunchKernelGGL(k_fill<float>, g1(n), dim3(BLOCK), 0, stream, n, (float *)x, (float)v);
+    BN_LAUNCH(k_fill<float>, g1(n), dim3(BLOCK), 0, stream, n, (float *)x, (float)v);
   else
-    hipLaunchKernelGGL(k_fill<double>, g1(n), dim3(BLOCK), 0, stream, n, (double *)x, v);
+    BN_LAUNCH(k_fill<double>, g1(n), dim3(BLOCK), 0, stream, n, (double *)x, v);
 }
 
 void bn_copy(char vt, int n, const void *x, void *y, hipStream_t stream) {
   if (n <= 0) return;
   if (vt == 's')
-    hipLaunchKernelGGL(k_copy<float>, g1(n), dim3(BLOCK), 0, stream, n, (const float *)x, (float *)y);
+    BN_LAUNCH(k_copy<float>, g1(n), dim3(BLOCK), 0, stream, n, (const float *)x, (float *)y);
   else
-    hipLaunchKernelGGL(k_copy<double>, g1(n), dim3(BLOCK), 0, stream, n, (const double *)x, (double *)y);
+    BN_LAUNCH(k_copy<double>, g1(n), dim3(BLOCK), 0, stream, n, (const double *)x, (double *)y);
 }
 
 void bn_convert(char vt_in, char vt_out, int n, const void *x, void *y, hipStream_t stream) {
   if (n <= 0) return;
   if (vt_in == 'd' && vt_out == 's')
-    hipLaunchKernelGGL((k_convert<double, float>), g1(n), dim3(BLOCK), 0, stream, n, (const double *)x,
+    BN_LAUNCH((k_convert<double, float>), g1(n), dim3(BLOCK), 0, stream, n, (const double *)x,
                        (float *)y);
   else if (vt_in == 's' && vt_out == 'd')
-    hipLaunchKernelGGL((k_convert<float, double>), g1(n), dim3(BLOCK), 0, stream, n, (const float *)x,
+    BN_LAUNCH((k_convert<float, double>), g1(n), dim3(BLOCK), 0, stream, n, (const float *)x,
                        (double *)y);
   else
     bn_copy(vt_in, n, x, y, stream);
@@ -1207,9 +1364,9 @@ void bn_convert(char vt_in, char vt_out, int n, const void *x, void *y, hipStrea
 void bn_axpy1(char vt, int n, const void *x, void *y, hipStream_t stream) {
   if (n <= 0) return;
   if (vt == 's')
-    hipLaunchKernelGGL(k_axpy1<float>, g1(n), dim3(BLOCK), 0, stream, n, (const float *)x, (float *)y);
+    BN_LAUNCH(k_axpy1<float>, g1(n), dim3(BLOCK), 0, stream, n, (const float *)x, (float *)y);
   else
-    hipLaunchKernelGGL(k_axpy1<double>, g1(n), dim3(BLOCK), 0, stream, n, (const double *)x, (double *)y);
+    BN_LAUNCH(k_axpy1<double>, g1(n), dim3(BLOCK), 0, stream, n, (const double *)x, (double *)y);
 }
 
 void bn_dot_partial(int n, const double *x, const double *y, const double *u, const double *v,
@@ -1222,7 +1379,7 @@ void bn_dot_partial(int n, const double *x, const double *y, const double *u, co
   const long long nb = ((long long)n + BLOCK - 1) / BLOCK;
   const int g = grid_for(nb);
   const int chunk = (int)((nb + g - 1) / g);
-  hipLaunchKernelGGL(k_dot2, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, y, u, v, partial, partial2);
+  BN_LAUNCH(k_dot2, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, y, u, v, partial, partial2);
 }
 
 void bn_pcg_update_xr(int n, double *x, const double *p, double *r, const double *q, const double *s,
@@ -1234,7 +1391,7 @@ void bn_pcg_update_xr(int n, double *x, const double *p, double *r, const double
   const long long nb = ((long long)n + BLOCK - 1) / BLOCK;
   const int g = grid_for(nb);
   const int chunk = (int)((nb + g - 1) / g);
-  hipLaunchKernelGGL(k_update_xr<false>, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, p, r, q, s, partial,
+  BN_LAUNCH(k_update_xr<false>, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, p, r, q, s, partial,
                      (float *)NULL);
 }
 
@@ -1247,7 +1404,7 @@ void bn_pcg_update_xr_s(int n, double *x, const double *p, double *r, const doub
   const long long nb = ((long long)n + BLOCK - 1) / BLOCK;
   const int g = grid_for(nb);
   const int chunk = (int)((nb + g - 1) / g);
-  hipLaunchKernelGGL(k_update_xr<true>, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, p, r, q, s, partial, rs);
+  BN_LAUNCH(k_update_xr<true>, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, p, r, q, s, partial, rs);
 }
 
 void bn_dot_rz_s(int n, const double *r, const float *zs, const double *q, double *partial, double *partial2,
@@ -1260,33 +1417,33 @@ void bn_dot_rz_s(int n, const double *r, const float *zs, const double *q, doubl
   const long long nb = ((long long)n + BLOCK - 1) / BLOCK;
   const int g = grid_for(nb);
   const int chunk = (int)((nb + g - 1) / g);
-  hipLaunchKernelGGL(k_dot_rz_s, dim3(g), dim3(BLOCK), 0, stream, n, chunk, r, zs, q, partial, partial2);
+  BN_LAUNCH(k_dot_rz_s, dim3(g), dim3(BLOCK), 0, stream, n, chunk, r, zs, q, partial, partial2);
 }
 
 void bn_pcg_update_p_s(int n, double *p, const float *zs, const double *s, hipStream_t stream) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_update_p_s, g1(n), dim3(BLOCK), 0, stream, n, p, zs, s);
+  BN_LAUNCH(k_update_p_s, g1(n), dim3(BLOCK), 0, stream, n, p, zs, s);
 }
 
 void bn_pcg_update_p(int n, double *p, const double *z, const double *s, hipStream_t stream) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_update_p, g1(n), dim3(BLOCK), 0, stream, n, p, z, s);
+  BN_LAUNCH(k_update_p, g1(n), dim3(BLOCK), 0, stream, n, p, z, s);
 }
 
 void bn_scalar_finish(int which, const double *p1, const double *p2, const double *p3, double *s, int slot,
                       hipStream_t stream) {
-  hipLaunchKernelGGL(k_scalar_finish, dim3(1), dim3(BLOCK), 0, stream, which, p1, p2, p3, s, slot);
+  BN_LAUNCH(k_scalar_finish, dim3(1), dim3(BLOCK), 0, stream, which, p1, p2, p3, s, slot);
 }
 
 void bn_permute(int n, const int *perm, const double *x, double *y, int scatter, hipStream_t stream) {
   if (n <= 0) return;
-  hipLaunchKernelGGL(k_permute, g1(n), dim3(BLOCK), 0, stream, n, perm, x, y, scatter);
+  BN_LAUNCH(k_permute, g1(n), dim3(BLOCK), 0, stream, n, perm, x, y, scatter);
 }
 
 void bn_edge_grad(int m, const int *eu, const int *ev, const double *g, const double *p, const double *lam,
                   const double *pdir, double *dg, double *grad_logr, hipStream_t stream) {
   if (m <= 0) return;
-  hipLaunchKernelGGL(k_edge_grad, g1(m), dim3(BLOCK), 0, stream, m, eu, ev, g, p, lam, pdir, dg, grad_logr);
+  BN_LAUNCH(k_edge_grad, g1(m), dim3(BLOCK), 0, stream, m, eu, ev, g, p, lam, pdir, dg, grad_logr);
 }
 
 }  // extern "C"

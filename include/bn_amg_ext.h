@@ -145,6 +145,27 @@ int bn_solver_ic_levels(void *h, int *fwd, int *bwd);
  * Returns 0 or a BN_ERR_* code. */
 int bn_solver_time_kernel(void *h, int which, int reps, double *ms_avg, double *bytes);
 
+/* Kernel census of one solve: the solve runs once (b, x as for
+ * bn_solver_solve_device; no step graph) with a HIP event pair on the
+ * solver's stream around every launch of its bandwidth kernels (sparse
+ * products, residuals and transfers, Gauss-Seidel colours, IC(0) Jacobi
+ * sweeps, the outer SpMV); afterwards every distinct launch shape is
+ * launched `reps` times more, each after a 320 MB read (cold).  Per kernel
+ * symbol (spelled as rocprofv3 prints it): launches, distinct shapes,
+ * algorithmic bytes (each array touched once), device microseconds as the
+ * solve ran, and as cold launches would take.  Up to max_out symbols;
+ * *n_out = how many.  No counterpart in the reference (measurement). */
+typedef struct {
+  char name[80];
+  int launches;
+  int shapes;
+  double bytes;
+  double us_solve;
+  double us_cold;
+} bn_kernel_stat;
+int bn_solver_kernel_census(void *h, const double *d_b, double *d_x, double tol, int maxiter, int reps,
+                            bn_kernel_stat *out, int max_out, int *n_out);
+
 /* Device memory helpers for callers that hold no HIP runtime of their
  * own (e.g. ctypes users): plain hipMalloc/hipMemcpy/hipFree on the
  * current device.  Return 0 or BN_ERR_HIP. */

@@ -109,21 +109,28 @@ def test_row_block_kernel_against_oracle(oracle, vt):
         oracle.lib.abyvam(dp(b), ip(ia), ip(ja), dp(val), dp(x), n, dp(ref_res))
         oracle.lib.abyvcs(ip(ia), ip(ja), dp(val), dp(x), n, dp(ref_add))
         dia, dja, dv, dblk = dev(ia), dev(ja), dev(val.astype(dt)), dev(blk[: nb + 1])
+        dblkq = dev(ia[blk[: nb + 1]].astype(np.int32))      # entry offsets of the block boundaries
         dx, db = dev(x.astype(dt)), dev(b.astype(dt))
+        bits = np.uint64 if vt == "d" else np.uint32
         for op, ref, init in ((0, ref_set, None), (1, ref_res, None), (2, ref_add, y0)):
-            dy = dev((init if init is not None else np.zeros(n)).astype(dt))
-            lib.bn_csr_apply_blocks(vt.encode(), op, nb, dblk.ptr, dia.ptr, dja.ptr, dv.ptr, dx.ptr, dy.ptr,
-                                    db.ptr, None)
-            got = dy.numpy().astype(np.float64)
-            assert relerr(got, ref) < tol, (name, op)
+            outs = []
+            for q in (None, dblkq.ptr):
+                dy = dev((init if init is not None else np.zeros(n)).astype(dt))
+                lib.bn_csr_apply_blocks(vt.encode(), op, nb, dblk.ptr, q, dia.ptr, dja.ptr, dv.ptr, dx.ptr, dy.ptr,
+                                        db.ptr, None)
+                outs.append(dy.numpy())
+                got = outs[-1].astype(np.float64)
+                assert relerr(got, ref) < tol, (name, op)
+            # with or without the stored entry offsets: the same sums
+            assert np.array_equal(outs[0].view(bits), outs[1].view(bits)), (name, op)
         # the scaled residual writes the plain residual bit for bit and its
         # product with dsc (hub rows take the one-long-row path)
-        bits = np.uint64 if vt == "d" else np.uint32
         dsc = (rng.random(n) + 0.5).astype(dt)
         dy1, dy2, dys, dd = dev(np.zeros(n, dt)), dev(np.zeros(n, dt)), dev(np.zeros(n, dt)), dev(dsc)
-        lib.bn_csr_apply_blocks(vt.encode(), 1, nb, dblk.ptr, dia.ptr, dja.ptr, dv.ptr, dx.ptr, dy1.ptr, db.ptr, None)
-        lib.bn_csr_resid_blocks_scaled(vt.encode(), nb, dblk.ptr, dia.ptr, dja.ptr, dv.ptr, dx.ptr, dy2.ptr, db.ptr,
-                                       dd.ptr, dys.ptr, None)
+        lib.bn_csr_apply_blocks(vt.encode(), 1, nb, dblk.ptr, None, dia.ptr, dja.ptr, dv.ptr, dx.ptr, dy1.ptr, db.ptr,
+                                None)
+        lib.bn_csr_resid_blocks_scaled(vt.encode(), nb, dblk.ptr, dblkq.ptr, dia.ptr, dja.ptr, dv.ptr, dx.ptr, dy2.ptr,
+                                       db.ptr, dd.ptr, dys.ptr, None)
         g1, g2 = dy1.numpy(), dy2.numpy()
         assert np.array_equal(g1.view(bits), g2.view(bits)), name
         assert np.array_equal(dys.numpy().view(bits), (g2 * dsc).astype(dt).view(bits)), name
@@ -307,6 +314,87 @@ def test_mapped_multicolour_sweep_is_exact_gauss_seidel(vt, lanes):
         rows = perm[lo:]  # the rows the colour-order form writes
         assert np.array_equal(gc[rows].view(bits), gm[rows].view(bits))
         assert np.array_equal(gc[colour != c].view(bits), np.zeros(int(np.sum(colour != c)), dt).view(bits))
+
+
+def colour_blocks(offd, col_ptr):
+    """Row blocks of a colour-ordered copy, colour by colour (the solver's
+    construction): boundaries and entry offsets, first block per colour."""
+    bk, cb = [0], []
+    for c in range(len(col_ptr) - 1):
+        lo, hi = int(col_ptr[c]), int(col_ptr[c + 1])
+        cb.append(len(bk) - 1)
+        if hi > lo:
+            sub = np.ascontiguousarray(offd.indptr[lo:hi + 1], np.int32)
+            out = np.zeros(hi - lo + 1, np.int32)
+            k = lib.bn_row_blocks(ip(sub), hi - lo, 2048, 256, ip(out))
+            bk.extend(int(v) + lo for v in out[1:k + 1])
+    cb.append(len(bk) - 1)
+    bk = np.array(bk, np.int32)
+    return bk, offd.indptr[bk].astype(np.int32), cb
+
+
+@pytest.mark.parametrize("vt", ["d", "s"])
+@pytest.mark.parametrize("which", ["network", "hubs"])
+def test_row_block_colour_kernel_is_exact_gauss_seidel(vt, which):
+    # the row-block colour kernel on the solver's layout: rows in colour
+    # order holding only their off-diagonal entries, vectors natural, row r
+    # updating map[r]; a forward sweep solves (D + L_pi) x_new = b - U_pi x_old
+    if which == "network":
+        net = make_network(14, 8, seed=3)
+        A = sp.csr_matrix((net.val, net.ja, net.ia), shape=(net.n, net.n))
+    else:
+        ia, ja, val = hub_matrix(3)
+        H = sp.csr_matrix((val, ja, ia), shape=(len(ia) - 1, len(ia) - 1))
+        A = (H + H.T).tocsr()
+        A.setdiag(np.asarray(abs(A).sum(axis=1)).ravel() + 1.0)
+        A = A.tocsr()
+        A.sort_indices()
+    n = A.shape[0]
+    aia, aja, ava = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy()
+    m = L.CSR(n, n, len(aja), ip(aia), ip(aja), dp(ava))
+    perm = np.zeros(n, np.int32)
+    cptr = C.POINTER(C.c_int)()
+    mis = (np.arange(n) % 3 == 0).astype(np.int32)
+    ncol = lib.bn_colour_order(C.byref(m), ip(mis), 1, ip(perm), C.byref(cptr))
+    col_ptr = np.ctypeslib.as_array(cptr, (ncol + 1,)).copy()
+    Ac = A[perm].tocsr()
+    Ac.sort_indices()
+    coo = Ac.tocoo()
+    keep = coo.col != perm[coo.row]                     # drop each row's own diagonal
+    offd = sp.csr_matrix((coo.data[keep], (coo.row[keep], coo.col[keep])), shape=(n, n))
+    offd.sort_indices()
+    bk, bq, cb = colour_blocks(offd, col_ptr)
+    lens = np.diff(offd.indptr)
+    case(f"row-block colour GS {which} n={n} colours={ncol} blocks={len(bk) - 1} "
+         f"longest row {lens.max()} vt={vt}")
+    rng = np.random.default_rng(4)
+    b = rng.random(n) * 1e-3
+    x_old = rng.random(n) * 1e3
+    zbase = rng.random(n)
+    D = A.diagonal()
+    dt = np.float64 if vt == "d" else np.float32
+    dia, dja, dv = dev(offd.indptr.astype(np.int32)), dev(offd.indices.astype(np.int32)), dev(offd.data.astype(dt))
+    dbk, dbq, dmap, ddinv = dev(bk), dev(bq), dev(perm), dev((1.0 / D[perm]).astype(dt))
+    db, dx, dx2 = dev(b.astype(dt)), dev(x_old.astype(dt)), dev(x_old.astype(dt))
+    dzb, dzs = dev(zbase.astype(dt)), dev(np.zeros(n, dt))
+    for c in range(ncol):
+        lib.bn_gs_color_blocks(vt.encode(), cb[c], cb[c + 1] - cb[c], dbk.ptr, dbq.ptr, dia.ptr, dja.ptr, dv.ptr,
+                               dmap.ptr, ddinv.ptr, db.ptr, dx.ptr, dzb.ptr, dzs.ptr, None)
+        lib.bn_gs_color_map(vt.encode(), int(col_ptr[c]), int(col_ptr[c + 1]), dmap.ptr, dia.ptr, dja.ptr, dv.ptr,
+                            ddinv.ptr, db.ptr, dx2.ptr, 4, None, None, None)
+    x_new = dx.numpy().astype(np.float64)
+    pos = np.empty(n, np.int64)
+    pos[perm] = np.arange(n)
+    ac = A.tocoo()
+    lower = pos[ac.col] < pos[ac.row]
+    upper = pos[ac.col] > pos[ac.row]
+    Lp = sp.csr_matrix((ac.data[lower], (ac.row[lower], ac.col[lower])), shape=(n, n))
+    Up = sp.csr_matrix((ac.data[upper], (ac.row[upper], ac.col[upper])), shape=(n, n))
+    tol = 1e-12 if vt == "d" else 2e-5
+    assert relerr(D * x_new + Lp @ x_new, b - Up @ x_old) < tol
+    assert relerr(dzs.numpy().astype(np.float64), zbase + x_new) < (1e-15 if vt == "d" else 1e-6)
+    # the lanes kernel on the same diagonal-free copy gives the same sweep
+    assert relerr(dx2.numpy().astype(np.float64), x_new) < (1e-12 if vt == "d" else 1e-5)
 
 
 @pytest.mark.parametrize("vt", ["d", "s"])
