@@ -214,6 +214,9 @@ def _load():
         "bn_row_blocks": (C.c_int, [IP, C.c_int, C.c_int, C.c_int, IP]),
         "bn_colour_order": (C.c_int, [C.POINTER(CSR), IP, C.c_int, IP, C.POINTER(IP)]),
         "bn_level_sets": (C.c_int, [IP, IP, C.c_int, C.c_int, IP, C.POINTER(IP)]),
+        "bn_dev_scan_i32": (C.c_int, [VP, VP, C.c_longlong, VP]),
+        "bn_dev_scan_i64": (C.c_int, [VP, VP, C.c_longlong, VP]),
+        "bn_dev_sort_pairs": (C.c_int, [VP, VP, VP, VP, C.c_int, C.c_int, VP]),
         "bn_sweep_levels": (C.c_int, [C.POINTER(CSR), IP, C.c_int, IP, C.POINTER(IP)]),
         "cf_gauss_seidel": (C.c_int, [C.c_int, IP, IP, DP, DP, DP, DP, C.c_int, C.c_int, DP]),
         "bn_cf_gauss_seidel_device": (C.c_int, [C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, C.c_int]),

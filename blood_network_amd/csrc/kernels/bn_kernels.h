@@ -48,6 +48,15 @@ enum {
  * device current); 0 or a BN_ERR_* code */
 int bn_scratch_ptr(void **ptr, size_t *bytes);
 
+/* Set-up helpers (devsort.hip), synchronous on `st`, 0 or -1: exclusive
+ * prefix sums of n int32 / int64 values, and a stable radix sort of n
+ * (key, value) pairs by the keys' low end_bit bits (higher bits ignored);
+ * inputs and outputs must not overlap. */
+int bn_dev_scan_i32(const int *in, int *out, long long n, hipStream_t st);
+int bn_dev_scan_i64(const long long *in, long long *out, long long n, hipStream_t st);
+int bn_dev_sort_pairs(const unsigned long long *kin, unsigned long long *kout, const int *vin, int *vout, int n,
+                      int end_bit, hipStream_t st);
+
 /* Arms a kernel timer: the calling thread's next launch from kernels.hip
  * records its start and end into the two events (the dispatch's own
  * timestamps); NULL, NULL disarms it. */

@@ -9,12 +9,11 @@
 //    starting from its first term (the diagonal from +0.0);
 //  - A_c keeps RAP's layout, each value (v_ij + v_ji) * 0.5.
 // Every term (R entry q1, A entry q2, P entry q3) becomes an item keyed by
-// (row, column); a stable radix sort groups the items of an entry in
+// (row, column); a stable radix sort (devsort.hip) groups the items of an entry in
 // traversal order, and a per-row virtual item ahead of all terms stands
 // for the diagonal the host reserves.  An entry's first item then gives
 // its place in the row.  Compiled without contraction.
 #include <hip/hip_runtime.h>
-#include <hipcub/hipcub.hpp>
 
 #include <stdlib.h>
 #include <string.h>
@@ -252,12 +251,8 @@ T *upload(Pool &pool, const T *h, size_t count) {
   } while (0)
 
 int sort_pairs(Pool &pool, const u64 *kin, u64 *kout, const int *vin, int *vout, int n, int end_bit) {
-  size_t tb = 0;
-  GCHECK(hipcub::DeviceRadixSort::SortPairs(nullptr, tb, kin, kout, vin, vout, n, 0, end_bit, (hipStream_t)0));
-  void *tmp = pool.get<char>(tb);
-  GNULL(tmp);
-  GCHECK(hipcub::DeviceRadixSort::SortPairs(tmp, tb, kin, kout, vin, vout, n, 0, end_bit, (hipStream_t)0));
-  return 0;
+  (void)pool;
+  return bn_dev_sort_pairs(kin, kout, vin, vout, n, end_bit, (hipStream_t)0);
 }
 
 int galerkin(const dcsr_mat *A, const dcsr_mat *P, const dcsr_mat *R, dcsr_mat *Ac) {
@@ -275,13 +270,7 @@ int galerkin(const dcsr_mat *A, const dcsr_mat *P, const dcsr_mat *R, dcsr_mat *
   GNULL(aia && aja && av && pia && pja && pv && ria && rja && rv && ap && w && off);
   hipLaunchKernelGGL(k_g_ap, dim3(nb(nf)), dim3(B), 0, 0, nf, aia, aja, pia, ap);
   hipLaunchKernelGGL(k_g_w, dim3(nb(nnzr)), dim3(B), 0, 0, nnzr, rja, ap, w);
-  {
-    size_t tb = 0;
-    GCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, w, off, nnzr, (hipStream_t)0));
-    void *tmp = pool.get<char>(tb);
-    GNULL(tmp);
-    GCHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tb, w, off, nnzr, (hipStream_t)0));
-  }
+  if (bn_dev_scan_i64(w, off, nnzr, (hipStream_t)0)) return -1;
   long long last[2];
   GCHECK(hipMemcpy(&last[0], off + nnzr - 1, sizeof(long long), hipMemcpyDeviceToHost));
   GCHECK(hipMemcpy(&last[1], w + nnzr - 1, sizeof(long long), hipMemcpyDeviceToHost));
@@ -303,13 +292,7 @@ int galerkin(const dcsr_mat *A, const dcsr_mat *P, const dcsr_mat *R, dcsr_mat *
   int *flag = pool.get<int>((size_t)N), *eid = pool.get<int>((size_t)N);
   GNULL(flag && eid);
   hipLaunchKernelGGL(k_g_heads, dim3(nb(N)), dim3(B), 0, 0, N, key_s, flag);
-  {
-    size_t tb = 0;
-    GCHECK(hipcub::DeviceScan::ExclusiveSum(nullptr, tb, flag, eid, N, (hipStream_t)0));
-    void *tmp = pool.get<char>(tb);
-    GNULL(tmp);
-    GCHECK(hipcub::DeviceScan::ExclusiveSum(tmp, tb, flag, eid, N, (hipStream_t)0));
-  }
+  if (bn_dev_scan_i32(flag, eid, N, (hipStream_t)0)) return -1;
   int lastf[2];
   GCHECK(hipMemcpy(&lastf[0], eid + N - 1, sizeof(int), hipMemcpyDeviceToHost));
   GCHECK(hipMemcpy(&lastf[1], flag + N - 1, sizeof(int), hipMemcpyDeviceToHost));
