@@ -1,6 +1,6 @@
 /*
  * bn_solver.h - internal solver state: host hierarchy plus its device
- * mirror (relabelled per level in smoother-colour order).
+ * mirror.
  */
 #ifndef BN_SOLVER_H
 #define BN_SOLVER_H

@@ -1,13 +1,18 @@
 /*
  * device.c - upload of the host hierarchy to the GPU.
  *
- * Each level is relabelled once in its smoother-colour order (C colours,
- * then F colours), so every colour of the multicolour sweep is a
- * contiguous row range of the level operator and all kernels of the level
- * stream contiguous CSR segments.  P and R = P^T are relabelled on both
- * sides, the solve matrix and its IC(0) factor in level-0 numbering.
- * Operators are stored in FP64 and FP32 so the V-cycle precision can be
- * switched without re-uploading.
+ * Levels keep the hierarchy's numbering, whose neighbouring rows share x
+ * cache lines in every operator of the level.  A level swept by the
+ * multicolour Gauss-Seidel smoother also gets a copy of its rows in colour
+ * order (C colours, then F colours) without the diagonal, so each colour's
+ * kernel streams one contiguous CSR segment and scatters into the level
+ * vectors.  (Renumbering such a level in colour order instead measured
+ * slower: its residual, restriction and interpolation lose the locality of
+ * their x gathers and the sweep gains little.)  Only the chunked layout
+ * of the block smoother relabels levels, P and R = P^T on both sides and
+ * the solve matrix and its IC(0) factor in level-0 numbering.  Operators
+ * are stored in FP64 and FP32 so the V-cycle precision can be switched
+ * without re-uploading.
  */
 #include <math.h>
 #include <stdio.h>

@@ -329,6 +329,34 @@ static REAL weight_draw(weight_rng *g) {
 /* PMIS, C_amg1/src/rs_classical.c:246-371.  Weights lambda_i +
  * rand()/RAND_MAX after srand(12345), drawn in index order, so the split
  * matches the reference on the same C library. */
+/* the per-row set-up stages on the device (setup_dev.hip), entry for entry
+ * the host routines' results; absent from builds without the device code */
+typedef struct bn_dsetup bn_dsetup;
+bn_dsetup *bn_dsetup_begin(const dcsr_mat *A) __attribute__((weak));
+void bn_dsetup_end(bn_dsetup *c) __attribute__((weak));
+int bn_dsetup_strength(bn_dsetup *c, double theta, int clamp, dcsr_mat *S, int *colcount) __attribute__((weak));
+int bn_dsetup_rounds(bn_dsetup *c, const double *w, int *cf, int undecided) __attribute__((weak));
+int bn_dsetup_interp(bn_dsetup *c, const int *cf, dcsr_mat *P) __attribute__((weak));
+
+/* PMIS with the rounds on the device: the weights are the host's (column
+ * counts from the device, draws in index order); 0, or -1 to run the host
+ * routine instead */
+static int pmis_device(bn_dsetup *dc, const INT *colcnt, const dcsr_mat *S, INT *cf, INT *n_coarse) {
+  const INT n = S->row;
+  REAL *w = (REAL *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(REAL));
+  weight_rng g;
+  weight_rng_init(&g);
+  for (INT i = 0; i < n; i++) {
+    w[i] = (REAL)colcnt[i] + weight_draw(&g);
+    cf[i] = 0;
+  }
+  const int rc = bn_dsetup_rounds(dc, w, cf, n);
+  free(w);
+  if (rc) return -1;
+  *n_coarse = promote_and_count(S, cf);
+  return 0;
+}
+
 void pmis_coarsening(const dcsr_mat *S, INT *cf, INT *n_coarse) {
   const INT n = S->row;
   dcsr_mat ST;
@@ -705,6 +733,36 @@ void ichol_free(ichol_factor *L) {
 int bn_galerkin_device(const dcsr_mat *A, const dcsr_mat *P, const dcsr_mat *R, dcsr_mat *Ac)
     __attribute__((weak));
 
+/* A device context for the per-row stages of a level.  BN_DEVICE_SETUP
+ * unset (automatic): PMIS levels of at least 4,000,000 operator entries,
+ * where the device rounds more than pay for the transfers (at 1.07M
+ * unknowns, level 0: 71 ms of host strength, split and interpolation on
+ * 16 threads against 37 ms); the other coarsenings keep the per-row
+ * stages on the host, whose threads form strength and interpolation in
+ * about the time the operator takes to cross to the device.  "all" (or
+ * "1"): every level of at least BN_DEVICE_SETUP_MIN entries (default 0),
+ * whatever the coarsening; "off" (or "0"): none.  No device: none. */
+static bn_dsetup *device_setup_begin(const dcsr_mat *A, int coarsening) {
+  if (!bn_dsetup_begin || !bn_dsetup_end || !bn_dsetup_strength || !bn_dsetup_rounds || !bn_dsetup_interp)
+    return NULL;
+  const char *mode = getenv("BN_DEVICE_SETUP");
+  long long min_nnz = 4000000;
+  if (mode && *mode) {
+    if (!strcmp(mode, "off") || !strcmp(mode, "0")) return NULL;
+    if (strcmp(mode, "all") && strcmp(mode, "1")) return NULL;
+    const char *mn = getenv("BN_DEVICE_SETUP_MIN");
+    min_nnz = (mn && *mn) ? atoll(mn) : 0;
+  } else if (coarsening != COARSEN_PMIS) {
+    return NULL;
+  }
+  if (A->nnz < min_nnz) return NULL;
+  return bn_dsetup_begin(A);
+}
+
+static void dsetup_end(bn_dsetup *dc) {
+  if (dc) bn_dsetup_end(dc);
+}
+
 static void galerkin_level(const amg_level *cur, dcsr_mat *Ac) {
   dcsr_mat Pt, rap;
   bn_csr_transpose(&cur->P, &Pt);
@@ -742,16 +800,36 @@ void bn_build_hierarchy(amg_data *amg, const dcsr_mat *A, int strength) {
     INT ml = 0, nc = 0;
     double t[5];
     t[0] = setup_ms();
-    strength_rule(&cur->A, amg->threshold, strength == 1, &cur->A_filtered, &ml);
+    /* large levels: strength, PMIS rounds and interpolation on the device */
+    bn_dsetup *dc = device_setup_begin(&cur->A, amg->coarsening);
+    INT *colcnt = (dc && amg->coarsening == COARSEN_PMIS) ? (INT *)bn_xmalloc((size_t)nf * sizeof(INT)) : NULL;
+    if (dc && bn_dsetup_strength(dc, amg->threshold, strength == 1, &cur->A_filtered, colcnt) == 0) {
+      INT longest = 0;
+      for (INT i = 0; i < nf; i++)
+        if (cur->A.ia[i + 1] - cur->A.ia[i] > longest) longest = cur->A.ia[i + 1] - cur->A.ia[i];
+      ml = longest < 1 ? 1 : longest;
+    } else {
+      dsetup_end(dc);
+      dc = NULL;
+      free(colcnt);
+      colcnt = NULL;
+      strength_rule(&cur->A, amg->threshold, strength == 1, &cur->A_filtered, &ml);
+    }
     t[1] = setup_ms();
     INT *cf = (INT *)bn_xcalloc((size_t)nf, sizeof(INT));
-    if (amg->coarsening == COARSEN_PMIS) pmis_coarsening(&cur->A_filtered, cf, &nc);
-    else if (amg->coarsening == COARSEN_HMIS) hmis_coarsening(&cur->A_filtered, cf, &nc);
-    else rs_coarsening(&cur->A_filtered, ml, cf, &nc);
+    if (amg->coarsening == COARSEN_PMIS) {
+      if (!colcnt || pmis_device(dc, colcnt, &cur->A_filtered, cf, &nc)) pmis_coarsening(&cur->A_filtered, cf, &nc);
+    } else if (amg->coarsening == COARSEN_HMIS) {
+      hmis_coarsening(&cur->A_filtered, cf, &nc);
+    } else {
+      rs_coarsening(&cur->A_filtered, ml, cf, &nc);
+    }
+    free(colcnt);
     t[2] = setup_ms();
     cur->n_coarse = nc;
     if (nc == 0 || nc >= nf) {
       /* stalled (build_hierarchy.c:78-83): this level becomes coarsest */
+      dsetup_end(dc);
       bn_csr_free(&cur->A_filtered);
       free(cf);
       break;
@@ -766,7 +844,9 @@ void bn_build_hierarchy(amg_data *amg, const dcsr_mat *A, int strength) {
     }
     for (INT i = 0; i < nf; i++)
       if (cf[i] != CPT) cur->cf_order[q++] = i;
-    rs_standard_interpolation(&cur->A, &cur->A_filtered, cf, &cur->P);
+    if (!dc || bn_dsetup_interp(dc, cf, &cur->P)) rs_standard_interpolation(&cur->A, &cur->A_filtered, cf, &cur->P);
+    const int on_device = dc != NULL;
+    dsetup_end(dc);
     free(cf);
     t[3] = setup_ms();
     amg_level *nx = &amg->levels[nlev];
@@ -774,9 +854,9 @@ void bn_build_hierarchy(amg_data *amg, const dcsr_mat *A, int strength) {
     galerkin_level(cur, &nx->A);
     t[4] = setup_ms();
     if (log)
-      fprintf(stderr, "[setup] level %d (%d rows, %d nnz): strength %.1f ms, split %.1f ms, interpolation %.1f ms, "
-              "Galerkin %.1f ms -> %d rows, %d nnz\n", nlev - 1, nf, cur->A.nnz, t[1] - t[0], t[2] - t[1],
-              t[3] - t[2], t[4] - t[3], nc, nx->A.nnz);
+      fprintf(stderr, "[setup] level %d (%d rows, %d nnz%s): strength %.1f ms, split %.1f ms, interpolation %.1f ms, "
+              "Galerkin %.1f ms -> %d rows, %d nnz\n", nlev - 1, nf, cur->A.nnz, on_device ? ", device stages" : "",
+              t[1] - t[0], t[2] - t[1], t[3] - t[2], t[4] - t[3], nc, nx->A.nnz);
     nx->n = nc;
     nx->l1_diag = (REAL *)bn_xmalloc((size_t)nc * sizeof(REAL));
     bn_csr_l1(&nx->A, nx->l1_diag);
