@@ -83,7 +83,9 @@ def test_jacobi_ic_flexible(net30):
     _case("lattice30 Jacobi IC, flexible")
     s = AMGSolver(net30.ia, net30.ja, net30.val, coarsening="pmis", min_size=500, fine_ic="jacobi",
                   stop="relres")
-    x, it = s.solve(net30.b, tol=1e-10, maxiter=1000)
+    # the error bound is kappa times the residual bound: solved to 1e-12 so
+    # that the 1e-8 error test holds for kappa up to 1e4
+    x, it = s.solve(net30.b, tol=1e-12, maxiter=1000)
     assert it > 0
     assert rel(x, direct(net30)) < 1e-8
 
