@@ -266,3 +266,46 @@ def test_preconditioner_is_symmetric(oracle):
     u, v = rng.random(net.n), rng.random(net.n)
     a, b = u @ h.precond(v), v @ h.precond(u)
     assert abs(a - b) <= 1e-6 * abs(a)
+
+
+def _sweep_closed_form(A, b, x0, order, nu, direction):
+    # nu sequential sweeps in `order` (forward) or its reverse (backward):
+    # with B = A[pi, pi] = D + L + U in visiting order, a forward sweep
+    # solves (D + L) x'[pi] = b[pi] - U x[pi], a backward one
+    # (D + U) x'[pi] = b[pi] - L x[pi]
+    Ap = A.toarray()[np.ix_(order, order)]
+    x = x0[order].copy()
+    for _ in range(nu):
+        if direction == 0:
+            x = np.linalg.solve(np.tril(Ap), b[order] - np.triu(Ap, 1) @ x)
+        else:
+            x = np.linalg.solve(np.triu(Ap), b[order] - np.tril(Ap, -1) @ x)
+    out = np.empty_like(x)
+    out[order] = x
+    return out
+
+
+@pytest.mark.parametrize("nu,direction", [(1, 0), (1, 1), (2, 0), (3, 1)])
+def test_mex_smoother_restatement_closed_form(oracle, nu, direction):
+    """oracle_cf_gauss_seidel (Matlab_amg/cf_gauss_seidel_mex.c:27-105:
+    1-based cf_order in doubles, direction 0 forward / 1 backward, nu
+    sweeps) against the dense closed form of the sequential sweep; a
+    dropped term, a wrong diagonal or a reversed direction fails it."""
+    import ctypes as C
+    rng = np.random.default_rng(20 + nu + direction)
+    n = 70
+    M = sp.random(n, n, density=0.07, random_state=nu + 10 * direction)
+    A = (M + M.T).tocsr()
+    A.setdiag(np.abs(A).sum(axis=1).A.ravel() + 0.5)
+    A.sort_indices()
+    ia, ja, va = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.copy()
+    order = rng.permutation(n)
+    b, x0 = rng.random(n), rng.random(n)
+    f = oracle.lib.oracle_cf_gauss_seidel
+    f.argtypes = [C.c_int, C.POINTER(C.c_int), C.POINTER(C.c_int), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                  C.POINTER(C.c_double), C.POINTER(C.c_double), C.c_int, C.c_int, C.POINTER(C.c_double)]
+    x = np.zeros(n)
+    o1 = order.astype(np.float64) + 1.0
+    f(n, ip(ia), ip(ja), dp(va), dp(b), dp(x0), dp(o1), nu, direction, dp(x))
+    ref = _sweep_closed_form(A, b, x0, order, nu, direction)
+    assert np.linalg.norm(x - ref) / np.linalg.norm(ref) < 1e-13
