@@ -41,6 +41,23 @@ typedef struct {
   int deep_s;      /* FP32 likewise                                    */
 } bn_dcsr;
 
+/* Level-ordered ELL copy of one triangle for its exact solve.  Level l
+ * (rows rows[ptr[l] .. ptr[l+1]) of the level schedule) holds w[l] slots
+ * per row; slot s of the level's k-th row sits at q[l] + s * cnt + k (cnt:
+ * the level's rows), so a wavefront's slot loads are contiguous and do
+ * not wait for the row index.  Padding slots name the row itself with a
+ * zero coefficient.  w[l] = 0: the level keeps the CSR rows (wider than
+ * BN_ELL_MAX_W). */
+typedef struct {
+  int *w, *q;          /* host, per level */
+  int *src, *ja;       /* device, per slot: CSR entry (-1: padding), column */
+  double *vd;          /* device, per slot: values gathered from the CSR copy */
+  float *vs;
+  double *dd;          /* device: 1/l_ii in level order */
+  float *ds;
+  long long slots;
+} bn_tell;
+
 /* level-scheduled triangular factor L (forward) and L^T (backward) */
 typedef struct {
   int n, nlev_f, nlev_b;
@@ -49,6 +66,9 @@ typedef struct {
   float *dinv_s;
   int *rows_f, *rows_b;   /* device: rows grouped by level */
   int *ptr_f, *ptr_b;     /* host: level boundaries */
+  int ell;                /* ELL copies of both triangles built */
+  int ell_tried;          /* a build was attempted (not retried) */
+  bn_tell ef, eb;         /* forward (L) and backward (L^T) copies */
 } bn_dtri;
 
 /* exact-order Gauss-Seidel sweep of a level: the rows of the sequential
@@ -288,6 +308,10 @@ int bn_device_ok(void);
 /* Builds what the current smoother needs before any V-cycle is enqueued
  * or captured (the exact sweeps' schedules); 0 or a BN_ERR_* code. */
 int bn_vc_prepare(struct bn_solver *s);
+/* the ELL copies of a triangle (device.c): built from the device CSR
+ * copies, values refreshed from them after a value change */
+int bn_tri_ell_build(struct bn_solver *s, bn_dtri *t);
+void bn_tri_ell_refresh(struct bn_solver *s, bn_dtri *t);
 /* -1: the smoother runs no exact-order sweep, else the bn_sweep slot */
 int bn_vc_exact_kind(const struct bn_solver *s);
 /* one colour c of level d's multicolour sweep with the given colour-kernel

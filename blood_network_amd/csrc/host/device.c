@@ -619,6 +619,114 @@ static double now_ms(void) {
   return 1e3 * (double)t.tv_sec + 1e-6 * (double)t.tv_nsec;
 }
 
+/* One direction's ELL copy from the CSR pattern (host copies ia, ja) and
+ * the level schedule (ptr, rows).  Values come later, gathered on the
+ * device from the CSR copy (bn_tri_ell_refresh). */
+static int ell_build_dir(struct bn_solver *s, int nlev, const int *ptr, const int *rows, const int *ia,
+                         const int *ja, bn_tell *e) {
+  e->w = (int *)bn_xmalloc((size_t)(nlev > 0 ? nlev : 1) * sizeof(int));
+  e->q = (int *)bn_xmalloc((size_t)(nlev > 0 ? nlev : 1) * sizeof(int));
+  bn_track_host(s, e->w);
+  bn_track_host(s, e->q);
+  long long slots = 0;
+  for (int l = 0; l < nlev; l++) {
+    int w = 0;
+    for (int k = ptr[l]; k < ptr[l + 1]; k++) {
+      const int len = ia[rows[k] + 1] - ia[rows[k]];
+      if (len > w) w = len;
+    }
+    if (w > BN_ELL_MAX_W) w = 0; /* this level keeps the CSR rows */
+    e->w[l] = w;
+    e->q[l] = (int)slots;
+    slots += (long long)w * (ptr[l + 1] - ptr[l]);
+    if (slots > 0x7fffffffLL) return -1;
+  }
+  e->slots = slots;
+  int *src = (int *)bn_xmalloc((size_t)(slots > 0 ? slots : 1) * sizeof(int));
+  int *col = (int *)bn_xmalloc((size_t)(slots > 0 ? slots : 1) * sizeof(int));
+#pragma omp parallel for schedule(dynamic, 4)
+  for (int l = 0; l < nlev; l++) {
+    const int w = e->w[l], cnt = ptr[l + 1] - ptr[l];
+    for (int k = 0; k < cnt && w > 0; k++) {
+      const int r = rows[ptr[l] + k], q0 = ia[r], len = ia[r + 1] - q0;
+      for (int t = 0; t < w; t++) {
+        const long long at = (long long)e->q[l] + (long long)t * cnt + k;
+        src[at] = t < len ? q0 + t : -1;
+        col[at] = t < len ? ja[q0 + t] : r;
+      }
+    }
+  }
+  int rc = 0;
+  e->src = (int *)dmalloc(s, (size_t)(slots > 0 ? slots : 1) * sizeof(int));
+  e->ja = (int *)dmalloc(s, (size_t)(slots > 0 ? slots : 1) * sizeof(int));
+  e->vd = (double *)dmalloc(s, (size_t)(slots > 0 ? slots : 1) * sizeof(double));
+  e->vs = (float *)dmalloc(s, (size_t)(slots > 0 ? slots : 1) * sizeof(float));
+  if (!e->src || !e->ja || !e->vd || !e->vs) rc = -1;
+  if (!rc && slots > 0 &&
+      (hipMemcpy(e->src, src, (size_t)slots * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(e->ja, col, (size_t)slots * sizeof(int), hipMemcpyHostToDevice) != hipSuccess))
+    rc = -1;
+  free(src);
+  free(col);
+  return rc;
+}
+
+/* BN_TRI_ELL=0 keeps every exact triangular level on the CSR rows */
+int bn_tri_ell_build(struct bn_solver *s, bn_dtri *t) {
+  if (t->ell || t->ell_tried || !t->lo.ia || t->n <= 0) return 0;
+  /* never inside a stream capture (the build copies synchronously) */
+  hipStreamCaptureStatus cs = hipStreamCaptureStatusNone;
+  if (hipStreamIsCapturing(s->stream, &cs) != hipSuccess || cs != hipStreamCaptureStatusNone) return 0;
+  t->ell_tried = 1;
+  const char *env = getenv("BN_TRI_ELL");
+  if (env && env[0] == '0') return 0;
+  const int n = t->n, nz = t->lo.nnz;
+  int *lia = (int *)bn_xmalloc(((size_t)n + 1) * sizeof(int)), *uia = (int *)bn_xmalloc(((size_t)n + 1) * sizeof(int));
+  int *lja = (int *)bn_xmalloc((size_t)(nz > 0 ? nz : 1) * sizeof(int));
+  int *uja = (int *)bn_xmalloc((size_t)(nz > 0 ? nz : 1) * sizeof(int));
+  int *rf = (int *)bn_xmalloc((size_t)n * sizeof(int)), *rb = (int *)bn_xmalloc((size_t)n * sizeof(int));
+  int rc = 0;
+  if (hipMemcpy(lia, t->lo.ia, ((size_t)n + 1) * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(uia, t->up.ia, ((size_t)n + 1) * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(lja, t->lo.ja, (size_t)nz * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(uja, t->up.ja, (size_t)nz * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(rf, t->rows_f, (size_t)n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
+      hipMemcpy(rb, t->rows_b, (size_t)n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
+    rc = -1;
+  if (!rc) rc = ell_build_dir(s, t->nlev_f, t->ptr_f, rf, lia, lja, &t->ef);
+  if (!rc) rc = ell_build_dir(s, t->nlev_b, t->ptr_b, rb, uia, uja, &t->eb);
+  if (!rc) {
+    t->ef.dd = (double *)dmalloc(s, (size_t)n * sizeof(double));
+    t->ef.ds = (float *)dmalloc(s, (size_t)n * sizeof(float));
+    t->eb.dd = (double *)dmalloc(s, (size_t)n * sizeof(double));
+    t->eb.ds = (float *)dmalloc(s, (size_t)n * sizeof(float));
+    if (!t->ef.dd || !t->ef.ds || !t->eb.dd || !t->eb.ds) rc = -1;
+  }
+  free(lia);
+  free(uia);
+  free(lja);
+  free(uja);
+  free(rf);
+  free(rb);
+  if (rc) {
+    (void)hipGetLastError();
+    return 0; /* the CSR rows serve: same results */
+  }
+  t->ell = 1;
+  bn_tri_ell_refresh(s, t);
+  return hipStreamSynchronize(s->stream) == hipSuccess ? 0 : BN_ERR_HIP;
+}
+
+/* the ELL values and level-ordered 1/t_ii from the device CSR copy (after
+ * every change of the factor's values) */
+void bn_tri_ell_refresh(struct bn_solver *s, bn_dtri *t) {
+  if (!t->ell) return;
+  bn_ell_gather(t->ef.slots, t->ef.src, t->lo.vd, t->ef.vd, t->ef.vs, s->stream);
+  bn_ell_gather(t->eb.slots, t->eb.src, t->up.vd, t->eb.vd, t->eb.vs, s->stream);
+  bn_rb_gather(t->n, t->rows_f, t->dinv_d, t->ef.dd, t->ef.ds, s->stream);
+  bn_rb_gather(t->n, t->rows_b, t->dinv_d, t->eb.dd, t->eb.ds, s->stream);
+}
+
 int bn_tri_upload(struct bn_solver *s, const ichol_factor *F, bn_dtri *t) {
   return upload_tri(s, F, NULL, NULL, t);
 }

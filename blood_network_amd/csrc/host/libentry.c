@@ -333,7 +333,7 @@ void ichol_solve(const ichol_factor *L, const REAL *b, REAL *x) {
   if (!e) {
     e = new_entry(KIND_TRI, L, NULL, NULL);
     e->n = L->n;
-    if (new_stream_solver(&e->s) || bn_tri_upload(e->s, L, &e->tri)) {
+    if (new_stream_solver(&e->s) || bn_tri_upload(e->s, L, &e->tri) || bn_tri_ell_build(e->s, &e->tri)) {
       pthread_mutex_unlock(&g_mu);
       fatal("ichol_solve device set-up failed");
     }

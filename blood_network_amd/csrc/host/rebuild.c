@@ -784,6 +784,9 @@ int bn_solver_rebuild_values_device(void *h, int nnz_in, const double *d_val) {
     return rc;
   }
   if (!s->coarse_dense && (rc = refresh_tri(s, &C->L_ichol, &s->coarse_ic))) return rc;
+  /* the exact solves' ELL copies follow their factors' new values */
+  bn_tri_ell_refresh(s, &s->fine_ic);
+  if (!s->coarse_dense) bn_tri_ell_refresh(s, &s->coarse_ic);
   tm[5] = rb_ms();
   if (log)
     fprintf(stderr,

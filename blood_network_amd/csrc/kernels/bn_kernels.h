@@ -195,6 +195,17 @@ void bn_trsv_level(char vt, int cnt, const int *rows, const int *ia,
                    const void *b, void *y, int lanes, const void *zbase,
                    void *zsum, hipStream_t stream);
 
+#define BN_ELL_MAX_W 8    /* widest level of the ELL triangular kernels */
+
+/* The same level from a level-ordered ELL copy: rows[0..cnt) are the
+ * level's rows, slot s of its k-th row at ja/val[s * cnt + k] (w slots,
+ * w <= 8, padding naming the row itself), dl[k] its 1/t_ii.  Row sums as
+ * bn_trsv_level's with one lane per row, to the last bit. */
+void bn_trsv_ell(char vt, int w, int cnt, const int *rows, const int *ja, const void *val, const void *dl,
+                 const void *b, void *y, const void *zbase, void *zsum, hipStream_t stream);
+/* out[q] = from[src[q]] (0 where src[q] < 0), and its FP32 rounding */
+void bn_ell_gather(long long n, const int *src, const double *from, double *out, float *outs, hipStream_t stream);
+
 /* One Jacobi sweep of a triangular (or any) system over all n rows:
  * xout_i = (b_i - sum_{j != i} T_ij xin_j) * dinv_i.  xin == NULL means
  * xin = 0 (xout = b * dinv).  With zsum != NULL (and xin != NULL) the

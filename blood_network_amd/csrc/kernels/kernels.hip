@@ -656,6 +656,62 @@ void dispatch_rowsolve(int lo, int cnt, const int *rows, const int *ia, const in
 }
 
 // ---------------------------------------------------------------------
+// Exact triangular levels from a level-ordered ELL copy.  Thread k of a
+// level updates row i = rows[k]; slot s of its row sits at s * cnt + k, so
+// the slot loads are issued with the row index and 1/t_ii instead of after
+// them: one round trip for those, then the gathers of y (two dependent
+// rounds per level against four through the CSR rows).  The sums are
+// k_rowsolve<T, 1, RS_LIST>'s: the row's t-th entry goes to accumulator
+// t & 3, the diagonal and the padding (which names the row itself)
+// contribute zero, and the accumulators add as (0 + 1) + (2 + 3), so
+// every y_i is the same to the last bit.
+// ---------------------------------------------------------------------
+template <typename T, int W>
+__device__ __forceinline__ T ell_row(int k, int cnt, int i, const int *__restrict__ ja, const T *__restrict__ val,
+                                     const T *y) {
+  T acc[4] = {0, 0, 0, 0};
+  int j[W];
+  T a[W];
+#pragma unroll
+  for (int s = 0; s < W; s++) {
+    j[s] = ja[s * cnt + k];
+    a[s] = val[s * cnt + k];
+  }
+#pragma unroll
+  for (int s = 0; s < W; s++) {
+    // select the product, not the gathered value: y_i itself may be stale
+    const T p = a[s] * y[j[s]];
+    acc[s & 3] += (j[s] == i) ? T(0) : p;
+  }
+  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
+template <typename T, int W>
+__global__ __launch_bounds__(BLOCK) void k_trsv_ell(int cnt, const int *__restrict__ rows,
+                                                    const int *__restrict__ ja, const T *__restrict__ val,
+                                                    const T *__restrict__ dl, const T *__restrict__ b, T *y,
+                                                    const T *__restrict__ zbase, T *__restrict__ zsum) {
+  const int k = logical_block() * BLOCK + threadIdx.x;
+  if (k >= cnt) return;
+  const int i = rows[k];
+  const T d = dl[k];
+  const T s = ell_row<T, W>(k, cnt, i, ja, val, y);
+  const T v = (b[i] - s) * d;
+  y[i] = v;
+  if (zsum) zsum[i] = zbase[i] + v;
+}
+
+__global__ void k_ell_gather(long long n, const int *__restrict__ src, const double *__restrict__ from,
+                             double *__restrict__ out, float *__restrict__ outs) {
+  const long long q = (long long)blockIdx.x * BLOCK + threadIdx.x;
+  if (q >= n) return;
+  const int k = src[q];
+  const double v = k >= 0 ? from[k] : 0.0;
+  out[q] = v;
+  outs[q] = (float)v;
+}
+
+// ---------------------------------------------------------------------
 // Block Gauss-Seidel (l1 hybrid): one workgroup per chunk of consecutive
 // rows.  The chunk's iterate lives in LDS for the whole sweep; the chunk's
 // local colours are swept in order with workgroup barriers between them,
@@ -1078,6 +1134,24 @@ __global__ void k_edge_grad(int m, const int *__restrict__ eu, const int *__rest
 
 inline dim3 g1(long long n) { return dim3((unsigned)((n + BLOCK - 1) / BLOCK)); }
 
+template <typename T>
+void trsv_ell_t(int w, int cnt, const int *rows, const int *ja, const T *val, const T *dl, const T *b, T *y,
+                       const T *zb, T *zs, hipStream_t st) {
+  const dim3 g(grid_rows((cnt + BLOCK - 1) / BLOCK));
+#define BN_TE(W) BN_LAUNCH((k_trsv_ell<T, W>), g, dim3(BLOCK), 0, st, cnt, rows, ja, val, dl, b, y, zb, zs)
+  switch (w) {
+    case 1: BN_TE(1); break;
+    case 2: BN_TE(2); break;
+    case 3: BN_TE(3); break;
+    case 4: BN_TE(4); break;
+    case 5: BN_TE(5); break;
+    case 6: BN_TE(6); break;
+    case 7: BN_TE(7); break;
+    default: BN_TE(8); break;
+  }
+#undef BN_TE
+}
+
 }  // namespace
 
 // =====================================================================
@@ -1321,6 +1395,22 @@ void bn_trsv_level(char vt, int cnt, const int *rows, const int *ia, const int *
     dispatch_rowsolve<double, RS_LIST>(0, cnt, rows, ia, ja, (const double *)val, (const double *)dinv,
                                     (const double *)b, (double *)y, lanes, (const double *)zbase,
                                     (double *)zsum, stream);
+}
+
+void bn_trsv_ell(char vt, int w, int cnt, const int *rows, const int *ja, const void *val, const void *dl,
+                 const void *b, void *y, const void *zbase, void *zsum, hipStream_t stream) {
+  if (cnt <= 0 || w <= 0 || w > BN_ELL_MAX_W) return;
+  if (vt == 's')
+    trsv_ell_t<float>(w, cnt, rows, ja, (const float *)val, (const float *)dl, (const float *)b, (float *)y,
+                      (const float *)zbase, (float *)zsum, stream);
+  else
+    trsv_ell_t<double>(w, cnt, rows, ja, (const double *)val, (const double *)dl, (const double *)b, (double *)y,
+                       (const double *)zbase, (double *)zsum, stream);
+}
+
+void bn_ell_gather(long long n, const int *src, const double *from, double *out, float *outs, hipStream_t stream) {
+  if (n <= 0) return;
+  BN_LAUNCH(k_ell_gather, dim3((unsigned)((n + BLOCK - 1) / BLOCK)), dim3(BLOCK), 0, stream, n, src, from, out, outs);
 }
 
 void bn_dense_gemv(char vt, int n, const void *M, const void *x, void *y, hipStream_t stream) {
