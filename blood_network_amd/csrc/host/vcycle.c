@@ -278,20 +278,17 @@ static void coarse_solve(struct bn_solver *s, const void *b, void *x) {
     return;
   }
   /* FP32 V-cycle: the coarsest solve stays FP64 */
+  if (s->coarse_dense) {
+    bn_dense_gemv_ds(n, s->cinv_d, (const float *)b, (float *)x, s->stream);
+    return;
+  }
   double *bd = s->cvec[4], *xd = s->cvec[5];
   bn_convert('s', 'd', n, b, bd, s->stream);
-  if (s->coarse_dense) {
-    bn_dense_gemv('d', n, s->cinv_d, bd, xd, s->stream);
-  } else {
-    bn_fill('d', n, xd, 0.0, s->stream);
-    coarse_icpcg(s, bd, xd);
-  }
+  bn_fill('d', n, xd, 0.0, s->stream);
+  coarse_icpcg(s, bd, xd);
   bn_convert('d', 's', n, xd, x, s->stream);
 }
 
-/* IC(0) application with level-scheduled forward and backward sweeps:
- * x = (L L^T)^{-1} b  (C_amg1/src/ichol.c:121-165).  If zsum, also
- * zsum = zbase + x. */
 /* one triangle, level by level: from the ELL copy where it exists,
  * otherwise from the CSR rows.  The ELL kernels form the one-lane CSR
  * kernel's sums, whatever lanes the CSR rows would run with (the tuner
@@ -316,6 +313,9 @@ static void tri_dir(struct bn_solver *s, const bn_dtri *t, const bn_dcsr *m, con
   }
 }
 
+/* IC(0) application with level-scheduled forward and backward sweeps:
+ * x = (L L^T)^{-1} b  (C_amg1/src/ichol.c:121-165).  If zsum, also
+ * zsum = zbase + x. */
 static void tri_solve(struct bn_solver *s, bn_dtri *t, char vt, const void *b, void *y, void *x,
                       const void *zbase, void *zsum) {
   const void *dinv = vt == 's' ? (const void *)t->dinv_s : (const void *)t->dinv_d;

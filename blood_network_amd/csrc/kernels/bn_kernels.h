@@ -223,6 +223,8 @@ void bn_jacobi_rows_scaled(char vt, int n, const int *ia, const int *ja, const v
 /* dense y = M x (row-major n x n), M in vt, x,y in vt */
 void bn_dense_gemv(char vt, int n, const void *M, const void *x, void *y,
                    hipStream_t stream);
+/* y = (float)(M (double)x): the FP64 dense product on FP32 vectors */
+void bn_dense_gemv_ds(int n, const double *M, const float *x, float *y, hipStream_t stream);
 
 /* ---- BLAS-1 (FP64 unless noted) ---- */
 void bn_fill(char vt, int n, void *x, double v, hipStream_t stream);

@@ -939,6 +939,19 @@ __global__ __launch_bounds__(BLOCK) void k_gemv(int n, const T *__restrict__ M, 
   if (threadIdx.x == 0) y[row] = (T)s;
 }
 
+// the FP32 V-cycle's coarsest solve: the FP64 inverse applied to an FP32
+// right-hand side, the FP64 result rounded to FP32 - the widening, the
+// FP64 product and the rounding that three launches form, in one
+__global__ __launch_bounds__(BLOCK) void k_gemv_ds(int n, const double *__restrict__ M, const float *__restrict__ x,
+                                                   float *__restrict__ y) {
+  const int row = blockIdx.x;
+  const double *m = M + (size_t)row * (size_t)n;
+  double s = 0.0;
+  for (int j = threadIdx.x; j < n; j += BLOCK) s += m[j] * (double)x[j];
+  s = block_sum(s);
+  if (threadIdx.x == 0) y[row] = (float)s;
+}
+
 // ---------------------------------------------------------------------
 // BLAS-1
 // ---------------------------------------------------------------------
@@ -1421,6 +1434,11 @@ void bn_dense_gemv(char vt, int n, const void *M, const void *x, void *y, hipStr
   else
     BN_LAUNCH(k_gemv<double>, dim3(n), dim3(BLOCK), 0, stream, n, (const double *)M,
                        (const double *)x, (double *)y);
+}
+
+void bn_dense_gemv_ds(int n, const double *M, const float *x, float *y, hipStream_t stream) {
+  if (n <= 0) return;
+  BN_LAUNCH(k_gemv_ds, dim3(n), dim3(BLOCK), 0, stream, n, M, x, y);
 }
 
 void bn_fill(char vt, int n, void *x, double v, hipStream_t stream) {
