@@ -51,6 +51,13 @@ typedef struct {
 typedef struct {
   int *w, *q;          /* host, per level */
   int *src, *ja;       /* device, per slot: CSR entry (-1: padding), column */
+  /* one-launch solve (bn_trsv_parts) of each maximal run of levels with
+   * ELL slots: run r covers levels run_lo[r] .. run_hi[r]-1 (host); the np
+   * parts' steps of run r start at sp[r * (np + 1) + p] (device) in steps
+   * (device, int4 per step) */
+  int np, nrun;
+  int *run_lo, *run_hi;
+  int *sp, *steps;
   double *vd;          /* device, per slot: values gathered from the CSR copy */
   float *vs;
   double *dd;          /* device: 1/l_ii in level order */
@@ -171,6 +178,11 @@ struct bn_solver {
                            known, 1 the same order, 2 gathered by vperm_d */
   int *vperm_d;         /* device: Asolve slot k takes caller slot vperm_d[k] */
   bn_dtri fine_ic;      /* IC(0) of the solve matrix, level-0 numbering */
+  /* exact triangular solves inside bn_pcg_run as one launch per run of ELL
+   * levels (bn_trsv_parts): usable, in use by the running solve, compute
+   * units (the most parts), wait limit in wall_clock64() ticks */
+  int walk_ok, walk_on, walk_blocks;
+  long long walk_limit;
   int coarse_dense;     /* coarsest solve: 1 dense inverse, 0 IC-PCG */
   double *cinv_d;       /* dense inverse of the coarsest operator */
   float *cinv_s;

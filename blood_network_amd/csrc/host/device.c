@@ -619,11 +619,21 @@ static double now_ms(void) {
   return 1e3 * (double)t.tv_sec + 1e-6 * (double)t.tv_nsec;
 }
 
+static int cmp_int(const void *a, const void *b) {
+  const int x = *(const int *)a, y = *(const int *)b;
+  return (x > y) - (x < y);
+}
+
+static void sort_levels(int nlev, const int *ptr, int *rows) {
+#pragma omp parallel for schedule(dynamic, 16)
+  for (int l = 0; l < nlev; l++) qsort(rows + ptr[l], (size_t)(ptr[l + 1] - ptr[l]), sizeof(int), cmp_int);
+}
+
 /* One direction's ELL copy from the CSR pattern (host copies ia, ja) and
  * the level schedule (ptr, rows).  Values come later, gathered on the
  * device from the CSR copy (bn_tri_ell_refresh). */
 static int ell_build_dir(struct bn_solver *s, int nlev, const int *ptr, const int *rows, const int *ia,
-                         const int *ja, bn_tell *e) {
+                         const int *ja, int n, int np, bn_tell *e) {
   e->w = (int *)bn_xmalloc((size_t)(nlev > 0 ? nlev : 1) * sizeof(int));
   e->q = (int *)bn_xmalloc((size_t)(nlev > 0 ? nlev : 1) * sizeof(int));
   bn_track_host(s, e->w);
@@ -656,16 +666,78 @@ static int ell_build_dir(struct bn_solver *s, int nlev, const int *ptr, const in
       }
     }
   }
+  /* the one-launch solve's steps (bn_trsv_parts): part p holds rows
+   * [p n / np, (p + 1) n / np); in each run of ELL levels, its steps are its
+   * rows of each level, contiguous in the level's ascending row list */
+  e->np = np;
+  e->nrun = 0;
+  e->run_lo = (int *)bn_xmalloc((size_t)(nlev > 0 ? nlev : 1) * sizeof(int));
+  e->run_hi = (int *)bn_xmalloc((size_t)(nlev > 0 ? nlev : 1) * sizeof(int));
+  bn_track_host(s, e->run_lo);
+  bn_track_host(s, e->run_hi);
+  for (int l = 0; l < nlev;) {
+    if (e->w[l] == 0) {
+      l++;
+      continue;
+    }
+    int l1 = l;
+    while (l1 < nlev && e->w[l1] > 0) l1++;
+    e->run_lo[e->nrun] = l;
+    e->run_hi[e->nrun] = l1;
+    e->nrun++;
+    l = l1;
+  }
+  /* kb[l * (np + 1) + p]: part p's first entry in level l's row list */
+  int *kb = (int *)bn_xmalloc((size_t)(nlev > 0 ? nlev : 1) * (size_t)(np + 1) * sizeof(int));
+  long long nsteps = 0;
+  for (int l = 0; l < nlev; l++) {
+    const int cnt = ptr[l + 1] - ptr[l];
+    int k = 0;
+    for (int p = 0; p <= np; p++) {
+      const long long lo = (long long)p * n / np;
+      while (k < cnt && rows[ptr[l] + k] < lo) k++;
+      kb[(size_t)l * (np + 1) + p] = p == np ? cnt : k;
+    }
+    if (e->w[l] > 0)
+      for (int p = 0; p < np; p++) nsteps += kb[(size_t)l * (np + 1) + p + 1] > kb[(size_t)l * (np + 1) + p];
+  }
+  int *sp = (int *)bn_xmalloc((size_t)(e->nrun > 0 ? e->nrun : 1) * (size_t)(np + 1) * sizeof(int));
+  int *st = (int *)bn_xmalloc((size_t)(nsteps > 0 ? nsteps : 1) * 4 * sizeof(int));
+  long long at = 0;
+  for (int r = 0; r < e->nrun; r++) {
+    for (int p = 0; p < np; p++) {
+      sp[(size_t)r * (np + 1) + p] = (int)at;
+      for (int l = e->run_lo[r]; l < e->run_hi[r]; l++) {
+        const int k0 = kb[(size_t)l * (np + 1) + p], k1 = kb[(size_t)l * (np + 1) + p + 1];
+        if (k1 == k0) continue;
+        st[4 * at] = ptr[l] + k0;
+        st[4 * at + 1] = k1 - k0;
+        st[4 * at + 2] = (ptr[l + 1] - ptr[l]) * 16 + e->w[l];
+        st[4 * at + 3] = e->q[l] + k0;
+        at++;
+      }
+    }
+    sp[(size_t)r * (np + 1) + np] = (int)at;
+  }
+  free(kb);
   int rc = 0;
   e->src = (int *)dmalloc(s, (size_t)(slots > 0 ? slots : 1) * sizeof(int));
   e->ja = (int *)dmalloc(s, (size_t)(slots > 0 ? slots : 1) * sizeof(int));
   e->vd = (double *)dmalloc(s, (size_t)(slots > 0 ? slots : 1) * sizeof(double));
   e->vs = (float *)dmalloc(s, (size_t)(slots > 0 ? slots : 1) * sizeof(float));
-  if (!e->src || !e->ja || !e->vd || !e->vs) rc = -1;
+  e->sp = (int *)dmalloc(s, (size_t)(e->nrun > 0 ? e->nrun : 1) * (size_t)(np + 1) * sizeof(int));
+  e->steps = (int *)dmalloc(s, (size_t)(nsteps > 0 ? nsteps : 1) * 4 * sizeof(int));
+  if (!e->src || !e->ja || !e->vd || !e->vs || !e->sp || !e->steps) rc = -1;
   if (!rc && slots > 0 &&
       (hipMemcpy(e->src, src, (size_t)slots * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
        hipMemcpy(e->ja, col, (size_t)slots * sizeof(int), hipMemcpyHostToDevice) != hipSuccess))
     rc = -1;
+  if (!rc && e->nrun > 0 &&
+      (hipMemcpy(e->sp, sp, (size_t)e->nrun * (np + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+       hipMemcpy(e->steps, st, (size_t)nsteps * 4 * sizeof(int), hipMemcpyHostToDevice) != hipSuccess))
+    rc = -1;
+  free(sp);
+  free(st);
   free(src);
   free(col);
   return rc;
@@ -693,8 +765,35 @@ int bn_tri_ell_build(struct bn_solver *s, bn_dtri *t) {
       hipMemcpy(rf, t->rows_f, (size_t)n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess ||
       hipMemcpy(rb, t->rows_b, (size_t)n * sizeof(int), hipMemcpyDeviceToHost) != hipSuccess)
     rc = -1;
-  if (!rc) rc = ell_build_dir(s, t->nlev_f, t->ptr_f, rf, lia, lja, &t->ef);
-  if (!rc) rc = ell_build_dir(s, t->nlev_b, t->ptr_b, rb, uia, uja, &t->eb);
+  /* rows ascending within each level (a relabelled factor's level lists
+   * come in the original order): the one-launch solve's parts are then
+   * contiguous in every level; rows of one level are independent, so the
+   * order changes no value */
+  if (!rc) {
+    sort_levels(t->nlev_f, t->ptr_f, rf);
+    sort_levels(t->nlev_b, t->ptr_b, rb);
+    if (hipMemcpy(t->rows_f, rf, (size_t)n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(t->rows_b, rb, (size_t)n * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+      rc = -1;
+  }
+  /* parts of the one-launch solve: BN_TRI_PARTS, else two per compute
+   * unit; never more than fit on the device at once (both precisions' kernels,
+   * so every part is resident), nor more than one per 1,024 rows */
+  int fit = bn_trsv_parts_per_cu('d');
+  const int fit_s = bn_trsv_parts_per_cu('s');
+  if (fit_s < fit) fit = fit_s;
+  if (fit > 2) fit = 2;
+  int np = fit * s->walk_blocks;
+  const char *penv = getenv("BN_TRI_PARTS");
+  if (penv && atoi(penv) > 0) np = atoi(penv);
+  if (np > fit * s->walk_blocks) np = fit * s->walk_blocks;
+  if (np > (n + 1023) / 1024) np = (n + 1023) / 1024;
+  if (np < 1) {
+    np = 1;
+    s->walk_ok = 0; /* nothing known to fit: keep a launch per level */
+  }
+  if (!rc) rc = ell_build_dir(s, t->nlev_f, t->ptr_f, rf, lia, lja, n, np, &t->ef);
+  if (!rc) rc = ell_build_dir(s, t->nlev_b, t->ptr_b, rb, uia, uja, n, np, &t->eb);
   if (!rc) {
     t->ef.dd = (double *)dmalloc(s, (size_t)n * sizeof(double));
     t->ef.ds = (float *)dmalloc(s, (size_t)n * sizeof(float));
@@ -928,6 +1027,23 @@ int bn_device_upload(struct bn_solver *s) {
   if (!s->ev0) {
     CK(hipEventCreate(&s->ev0));
     CK(hipEventCreate(&s->ev1));
+  }
+  /* one-launch triangular solves (bn_trsv_parts): at most one workgroup per CU, every
+   * wait bounded at 0.5 s of the constant-rate clock; BN_TRI_WALK=0 keeps a
+   * launch per level */
+  {
+    int cus = 0, khz = 0;
+    const char *wenv = getenv("BN_TRI_WALK");
+    s->walk_ok = !(wenv && wenv[0] == '0') &&
+                 hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, s->device) == hipSuccess &&
+                 hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, s->device) == hipSuccess &&
+                 cus > 0 && khz > 0;
+    s->walk_blocks = cus;
+    s->walk_limit = (long long)khz * 500;
+    /* BN_TRI_WALK_LIMIT: the wait limit in clock ticks; < 0 gives up at the
+     * first wait, which exercises the re-run with a launch per level */
+    const char *lenv = getenv("BN_TRI_WALK_LIMIT");
+    if (lenv && lenv[0]) s->walk_limit = atoll(lenv);
   }
   s->vt = (s->opt.precision == BN_PREC_FP32) ? 's' : 'd';
   const amg_data *amg = s->amg;

@@ -60,7 +60,8 @@ static void enqueue_step(struct bn_solver *s) {
 
 static int mode_key(const struct bn_solver *s) {
   return (s->vt == 's') | (s->opt.fine_ic << 1) | (s->opt.flexible << 3) | (s->opt.smoother << 4) |
-         (s->opt.nu << 8) | (s->opt.ic_sweeps << 16) | ((s->opt.gs_exact != 0) << 28);
+         (s->opt.nu << 8) | (s->opt.ic_sweeps << 16) | ((s->opt.gs_exact != 0) << 28) |
+         (s->walk_on << 29);
 }
 
 /* The captured step for the current mode, capturing it on first use.
@@ -92,9 +93,36 @@ static hipGraphExec_t step_graph(struct bn_solver *s) {
   return e;
 }
 
-/* d_b, d_x in caller (original) numbering; x is the initial guess. */
+/* A one-launch triangular walk gave up waiting (its grid was not resident
+ * at once): nothing of the caller's has been written yet (d_x is only
+ * written at the end), so the solve starts again with a launch per level,
+ * which the solver keeps from now on. */
+static int walk_gave_up(struct bn_solver *s, const double *d_b, double *d_x, double tol, int maxiter,
+                        int print_level, bn_solve_info *info) {
+  fprintf(stderr, "  PCG: a one-launch triangular solve timed out; solving again with a launch per level\n");
+  s->walk_on = 0;
+  s->walk_ok = 0;
+  bn_drop_graphs(s);
+  return bn_pcg_run(s, d_b, d_x, tol, maxiter, print_level, info);
+}
+
+static int pcg_body(struct bn_solver *s, const double *d_b, double *d_x, double tol, int maxiter,
+                    int print_level, bn_solve_info *info);
+
+/* d_b, d_x in caller (original) numbering; x is the initial guess.  Exact
+ * triangular solves walk their levels in one launch per run of ELL levels
+ * inside this solve only; the launches flag a wait that timed out in the
+ * scalar block, which is read back with every step. */
 int bn_pcg_run(struct bn_solver *s, const double *d_b, double *d_x, double tol, int maxiter,
                int print_level, bn_solve_info *info) {
+  s->walk_on = s->walk_ok;
+  const int it = pcg_body(s, d_b, d_x, tol, maxiter, print_level, info);
+  s->walk_on = 0;
+  return it;
+}
+
+static int pcg_body(struct bn_solver *s, const double *d_b, double *d_x, double tol, int maxiter,
+                    int print_level, bn_solve_info *info) {
   const double w0 = now_ms();
   hipStream_t st = s->stream;
   const int n = s->lev[0].n;
@@ -108,6 +136,7 @@ int bn_pcg_run(struct bn_solver *s, const double *d_b, double *d_x, double tol, 
   hipGraphExec_t gexec = step_graph(s);
 
   hipEventRecord(s->ev0, st);
+  bn_fill('d', 1, sc + BN_S_WALKERR, 0.0, st);
   bn_permute(n, s->lev[0].perm_d, d_b, s->bb, 0, st);
   bn_permute(n, s->lev[0].perm_d, d_x, s->x, 0, st);
   /* r = b - A x ; ||b||^2 */
@@ -123,6 +152,7 @@ int bn_pcg_run(struct bn_solver *s, const double *d_b, double *d_x, double tol, 
     bn_set_error("device failure during PCG set-up");
     return BN_ERR_HIP;
   }
+  if (hs[BN_S_WALKERR] != 0.0) return walk_gave_up(s, d_b, d_x, tol, maxiter, print_level, info);
   double bnorm = sqrt(hs[BN_S_BB]);
   if (bnorm == 0.0) bnorm = 1.0; /* amg_pcg.c:29 */
   double rnorm = sqrt(hs[BN_S_RR]);
@@ -145,6 +175,7 @@ int bn_pcg_run(struct bn_solver *s, const double *d_b, double *d_x, double tol, 
         bn_set_error("device failure in PCG step %d", it);
         return BN_ERR_HIP;
       }
+      if (hs[BN_S_WALKERR] != 0.0) return walk_gave_up(s, d_b, d_x, tol, maxiter, print_level, info);
       rel = hs[BN_S_REL];
       rnorm = sqrt(hs[BN_S_RR]);
       if (print_level >= 1)

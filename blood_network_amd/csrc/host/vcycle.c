@@ -303,6 +303,24 @@ static void tri_dir(struct bn_solver *s, const bn_dtri *t, const bn_dcsr *m, con
   const size_t vb = vt == 's' ? sizeof(float) : sizeof(double);
   const void *ev = vt == 's' ? (const void *)e->vs : (const void *)e->vd;
   const void *dl = vt == 's' ? (const void *)e->ds : (const void *)e->dd;
+  if (s->walk_on && t->ell && e->steps) {
+    /* each run of ELL levels in one launch (bn_trsv_parts); levels with CSR
+     * rows one launch each */
+    bn_walk_fill(vt, t->n, y, s->stream);
+    int r = 0;
+    for (int l = 0; l < nl;) {
+      if (r < e->nrun && e->run_lo[r] == l) {
+        bn_trsv_parts(vt, e->np, e->sp + (size_t)r * (e->np + 1), e->steps, rows, e->ja, ev, dl, b, y, zbase, zsum,
+                      s->scal + BN_S_WALKERR, s->walk_limit, s->stream);
+        l = e->run_hi[r++];
+        continue;
+      }
+      bn_trsv_level(vt, ptr[l + 1] - ptr[l], rows + ptr[l], m->ia, m->ja, opv(m, vt), dinv, b, y, lanes, zbase,
+                    zsum, s->stream);
+      l++;
+    }
+    return;
+  }
   for (int l = 0; l < nl; l++) {
     const int cnt = ptr[l + 1] - ptr[l];
     if (t->ell && e->w[l] > 0)

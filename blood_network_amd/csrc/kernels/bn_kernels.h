@@ -27,6 +27,8 @@ enum {
   BN_S_RZOLD = 9,  /* previous r^T z (flexible beta)            */
   BN_S_ZRPREV = 10,/* z^T r_prev (flexible beta)                */
   BN_S_DONE = 11,  /* block solve: column converged (frozen)    */
+  BN_S_WALKERR = 11,/* single-vector solve: 1 when a one-launch
+                       triangular solve gave up (bn_trsv_parts)   */
   BN_S_ITS = 12,   /* block solve: iteration it converged at    */
   BN_S_ITER = 13,  /* block solve: iterations done              */
   BN_S_TOL = 14,   /* block solve: stop tolerance               */
@@ -203,6 +205,28 @@ void bn_trsv_level(char vt, int cnt, const int *rows, const int *ia,
  * bn_trsv_level's with one lane per row, to the last bit. */
 void bn_trsv_ell(char vt, int w, int cnt, const int *rows, const int *ja, const void *val, const void *dl,
                  const void *b, void *y, const void *zbase, void *zsum, hipStream_t stream);
+/* A run of levels of the same ELL copy in one launch of np workgroups of
+ * BN_PART_BLOCK threads (np at most one per CU, so all resident).  The
+ * rows are cut into np contiguous ranges; workgroup p computes its range's
+ * rows level by level, with a barrier between levels: its steps are
+ * steps[sp[p] .. sp[p+1]) (int4 {first entry of the level's row list,
+ * rows, level rows * 16 + slots per row (1..8), first slot}); rows, dl and
+ * ja/val are the copy's base arrays.  y must hold the sentinel
+ * (bn_walk_fill) on every row the steps compute and the final values on
+ * every row they read from earlier levels.  A row reads its y_j with plain
+ * loads and, where one still holds the sentinel, waits for it with
+ * agent-scope atomic loads; a wait past `limit` ticks of wall_clock64()
+ * (limit < 0: any wait) sets *ctl = 1.0 and the launch gives up (y then
+ * holds sentinels); a launch finding *ctl != 0 does nothing.  Values as
+ * bn_trsv_ell's, to the last bit. */
+#define BN_PART_BLOCK 256
+/* workgroups of bn_trsv_parts that fit on one CU at once (0: unknown) */
+int bn_trsv_parts_per_cu(char vt);
+void bn_trsv_parts(char vt, int np, const int *sp, const int *steps, const int *rows, const int *ja,
+                   const void *val, const void *dl, const void *b, void *y, const void *zbase, void *zsum,
+                   double *ctl, long long limit, hipStream_t stream);
+/* y[0..n) = the walk's sentinel (a signalling-NaN bit pattern) */
+void bn_walk_fill(char vt, int n, void *y, hipStream_t stream);
 /* out[q] = from[src[q]] (0 where src[q] < 0), and its FP32 rounding */
 void bn_ell_gather(long long n, const int *src, const double *from, double *out, float *outs, hipStream_t stream);
 

@@ -666,24 +666,34 @@ void dispatch_rowsolve(int lo, int cnt, const int *rows, const int *ia, const in
 // contribute zero, and the accumulators add as (0 + 1) + (2 + 3), so
 // every y_i is the same to the last bit.
 // ---------------------------------------------------------------------
+// the row sum of W slots whose y values are already gathered (shared by
+// the launch-per-level kernel and the one-launch walk, so both form the
+// same expression and round alike)
+template <typename T, int W>
+__device__ __forceinline__ T ell_sum(const int *j, const T *a, const T *yv, int i) {
+  T acc[4] = {0, 0, 0, 0};
+#pragma unroll
+  for (int s = 0; s < W; s++) {
+    // select the product, not the gathered value: y_i itself may be stale
+    const T p = a[s] * yv[s];
+    acc[s & 3] += (j[s] == i) ? T(0) : p;
+  }
+  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
 template <typename T, int W>
 __device__ __forceinline__ T ell_row(int k, int cnt, int i, const int *__restrict__ ja, const T *__restrict__ val,
                                      const T *y) {
-  T acc[4] = {0, 0, 0, 0};
   int j[W];
-  T a[W];
+  T a[W], yv[W];
 #pragma unroll
   for (int s = 0; s < W; s++) {
     j[s] = ja[s * cnt + k];
     a[s] = val[s * cnt + k];
   }
 #pragma unroll
-  for (int s = 0; s < W; s++) {
-    // select the product, not the gathered value: y_i itself may be stale
-    const T p = a[s] * y[j[s]];
-    acc[s & 3] += (j[s] == i) ? T(0) : p;
-  }
-  return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+  for (int s = 0; s < W; s++) yv[s] = y[j[s]];
+  return ell_sum<T, W>(j, a, yv, i);
 }
 
 template <typename T, int W>
@@ -699,6 +709,200 @@ __global__ __launch_bounds__(BLOCK) void k_trsv_ell(int cnt, const int *__restri
   const T v = (b[i] - s) * d;
   y[i] = v;
   if (zsum) zsum[i] = zbase[i] + v;
+}
+
+// ---------------------------------------------------------------------
+// A run of exact triangular levels l0 .. l1-1 of the ELL copy in ONE
+// launch (each level of the run has 1 <= w <= BN_ELL_MAX_W slots per row).
+// The rows are cut into np contiguous ranges ("parts"), one workgroup
+// each; a workgroup takes its part's rows of each level in turn (kb gives
+// their positions in the level's row list, which is ascending), with a
+// workgroup barrier between levels.  A row's predecessors of its own part
+// were written before that barrier; those of other parts may still be on
+// their way.  Before the launch y holds the sentinel walk_word<T>::none on
+// every row of the triangle, every y_i is written once, with an
+// agent-scope atomic store, and never holds anything else while the launch
+// runs (caches hold nothing older: the launch starts with them invalid).
+// So a predecessor's value is first read with a plain load (for its own
+// part from the CU's caches), and only if that still reads the sentinel
+// does the row wait for it with agent-scope atomic loads, which are
+// coherent across the XCDs' L2 caches.  The value is its own flag, so no
+// ordering between two locations is needed and no cache is written back
+// or invalidated as a whole.  With contiguous parts most predecessors of a
+// lattice are in the same part, and the dependency chain crosses parts
+// about np times instead of once per level.  No deadlock: a row waits only
+// for rows of earlier levels, each part walks the levels in order, and
+// the grid (np <= one workgroup per CU) is resident at once.  Every wait is
+// bounded: past `limit` ticks of the constant-rate clock a thread raises
+// *ctl, every other thread stops at its next check (still meeting every
+// barrier), and the host re-runs the solve with a launch per level.  The
+// sums are ell_sum<T, w>'s, those of the launch-per-level kernel, so y is
+// the same to the last bit.
+// ---------------------------------------------------------------------
+template <typename T> struct walk_word;
+template <> struct walk_word<double> {
+  typedef unsigned long long U;
+  static constexpr U none = 0x7ff4dead0000beefULL;  // signalling NaN
+};
+template <> struct walk_word<float> {
+  typedef unsigned int U;
+  static constexpr U none = 0x7fa0beefU;  // signalling NaN
+};
+// (arithmetic only ever produces quiet NaNs, so no computed y_i is the
+// sentinel)
+
+template <typename T>
+__device__ __forceinline__ typename walk_word<T>::U walk_load(const T *p) {
+  return __hip_atomic_load((typename walk_word<T>::U *)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool walk_stopped(const double *ctl) {
+  return __hip_atomic_load((unsigned long long *)ctl, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) != 0ULL;
+}
+
+// waits until the W values y_j of row i are computed (yv gets them), with
+// coherent loads; false when this thread or another gave up
+template <typename T, int W>
+__device__ __forceinline__ bool walk_wait(const int *j, int i, const T *y, double *ctl, long long limit, T *yv) {
+  typedef typename walk_word<T>::U U;
+  long long t0 = 0;
+  for (int spin = 0;; spin++) {
+    bool ready = true;
+#pragma unroll
+    for (int s = 0; s < W; s++) {
+      // y_i itself (the diagonal slot, padding) is never waited for
+      const U u = walk_load(y + j[s]);
+      yv[s] = __builtin_bit_cast(T, u);
+      ready &= (j[s] == i) | (u != walk_word<T>::none);
+    }
+    if (ready) return true;
+    // every 16th poll: has another thread given up, and is the limit past
+    // (limit < 0 gives up at the first check: the host forces the fallback)
+    if ((spin & 15) == 15) {
+      if (walk_stopped(ctl)) return false;
+      const long long t = (long long)wall_clock64();
+      if (spin == 15) t0 = t;
+      if (limit < 0 || (spin > 15 && t - t0 > limit)) {
+        __hip_atomic_store((unsigned long long *)ctl, 0x3ff0000000000000ULL /* 1.0 */, __ATOMIC_RELAXED,
+                           __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+      }
+    }
+    __builtin_amdgcn_s_sleep(1);
+  }
+}
+
+// the row's predecessors: plain loads first, the coherent wait only when
+// one of them still reads the sentinel
+template <typename T, int W>
+__device__ __forceinline__ bool part_row(const int *j, const T *a, int i, const T *y, double *ctl, long long limit,
+                                         T &sum) {
+  typedef typename walk_word<T>::U U;
+  T yv[W];
+  bool ready = true;
+#pragma unroll
+  for (int s = 0; s < W; s++) {
+    const U u = ((const U *)y)[j[s]];
+    yv[s] = __builtin_bit_cast(T, u);
+    ready &= (j[s] == i) | (u != walk_word<T>::none);
+  }
+  if (!ready && !walk_wait<T, W>(j, i, y, ctl, limit, yv)) return false;
+  sum = ell_sum<T, W>(j, a, yv, i);
+  return true;
+}
+
+// one row of a part's step: its index, 1/t_ii and slots (nothing computed
+// in the launch, so they can be loaded a step ahead)
+template <typename T>
+struct part_item {
+  int i;
+  T d;
+  int j[BN_ELL_MAX_W];
+  T a[BN_ELL_MAX_W];
+};
+
+// step descriptor S = {first entry of the level's row list, rows, level
+// rows * 16 + slots per row, first slot}: row k < S.y of the step is
+// rows[S.x + k], slot s of it at S.w + s * (S.z >> 4) + k
+template <typename T>
+__device__ __forceinline__ void part_fetch(part_item<T> &it, const int4 S, int k, const int *__restrict__ rows,
+                                           const int *__restrict__ ja, const T *__restrict__ val,
+                                           const T *__restrict__ dl) {
+  const int w = S.z & 15, cnt = S.z >> 4;
+  it.i = rows[S.x + k];
+  it.d = dl[S.x + k];
+#pragma unroll
+  for (int s = 0; s < BN_ELL_MAX_W; s++) {
+    if (s < w) {
+      it.j[s] = ja[S.w + s * cnt + k];
+      it.a[s] = val[S.w + s * cnt + k];
+    }
+  }
+}
+
+template <typename T>
+__device__ __forceinline__ bool part_solve(const part_item<T> &it, int w, const T *__restrict__ b, T *y,
+                                           const T *__restrict__ zbase, T *__restrict__ zsum, double *ctl,
+                                           long long limit) {
+  typedef typename walk_word<T>::U U;
+  const T bi = b[it.i];
+  T sum = T(0);
+  bool ok;
+  switch (w) {
+    case 1: ok = part_row<T, 1>(it.j, it.a, it.i, y, ctl, limit, sum); break;
+    case 2: ok = part_row<T, 2>(it.j, it.a, it.i, y, ctl, limit, sum); break;
+    case 3: ok = part_row<T, 3>(it.j, it.a, it.i, y, ctl, limit, sum); break;
+    case 4: ok = part_row<T, 4>(it.j, it.a, it.i, y, ctl, limit, sum); break;
+    case 5: ok = part_row<T, 5>(it.j, it.a, it.i, y, ctl, limit, sum); break;
+    case 6: ok = part_row<T, 6>(it.j, it.a, it.i, y, ctl, limit, sum); break;
+    case 7: ok = part_row<T, 7>(it.j, it.a, it.i, y, ctl, limit, sum); break;
+    default: ok = part_row<T, 8>(it.j, it.a, it.i, y, ctl, limit, sum); break;
+  }
+  if (!ok) return false;
+  const T v = (bi - sum) * it.d;
+  __hip_atomic_store((U *)(y + it.i), __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+  if (zsum) zsum[it.i] = zbase[it.i] + v;
+  return true;
+}
+
+// Part p's steps are steps[sp[p] .. sp[p+1]), its rows of each level of the
+// run in level order; the first BN_PART_BLOCK rows of the next step are
+// loaded while the current step waits for its predecessors.
+template <typename T>
+__global__ __launch_bounds__(BN_PART_BLOCK) void k_trsv_parts(const int *__restrict__ sp,
+                                                              const int4 *__restrict__ steps,
+                                                              const int *__restrict__ rows,
+                                                              const int *__restrict__ ja, const T *__restrict__ val,
+                                                              const T *__restrict__ dl, const T *__restrict__ b, T *y,
+                                                              const T *__restrict__ zbase, T *__restrict__ zsum,
+                                                              double *ctl, long long limit) {
+  const int p = blockIdx.x, tid = threadIdx.x;
+  const int t0 = sp[p], t1 = sp[p + 1];
+  if (t0 >= t1) return;
+  bool live = !walk_stopped(ctl);
+  int4 S = steps[t0];
+  part_item<T> cur, nxt;
+  part_fetch(cur, S, tid < S.y ? tid : 0, rows, ja, val, dl);
+  for (int t = t0; t < t1; t++) {
+    const int4 Sn = steps[t + 1 < t1 ? t + 1 : t];
+    if (t + 1 < t1) part_fetch(nxt, Sn, tid < Sn.y ? tid : 0, rows, ja, val, dl);
+    const int w = S.z & 15;
+    if (live && tid < S.y) live = part_solve(cur, w, b, y, zbase, zsum, ctl, limit);
+    for (int k = tid + BN_PART_BLOCK; live && k < S.y; k += BN_PART_BLOCK) {
+      part_item<T> more;
+      part_fetch(more, S, k, rows, ja, val, dl);
+      live = part_solve(more, w, b, y, zbase, zsum, ctl, limit);
+    }
+    __syncthreads();  // a failed thread still meets every barrier
+    S = Sn;
+    cur = nxt;
+  }
+}
+
+template <typename U>
+__global__ void k_fill_word(int n, U *__restrict__ x, U v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) x[i] = v;
 }
 
 __global__ void k_ell_gather(long long n, const int *__restrict__ src, const double *__restrict__ from,
@@ -1419,6 +1623,38 @@ void bn_trsv_ell(char vt, int w, int cnt, const int *rows, const int *ja, const 
   else
     trsv_ell_t<double>(w, cnt, rows, ja, (const double *)val, (const double *)dl, (const double *)b, (double *)y,
                        (const double *)zbase, (double *)zsum, stream);
+}
+
+void bn_trsv_parts(char vt, int np, const int *sp, const int *steps, const int *rows, const int *ja,
+                   const void *val, const void *dl, const void *b, void *y, const void *zbase, void *zsum,
+                   double *ctl, long long limit, hipStream_t stream) {
+  if (np <= 0) return;
+  const int4 *S = (const int4 *)steps;
+  if (vt == 's')
+    BN_LAUNCH(k_trsv_parts<float>, dim3(np), dim3(BN_PART_BLOCK), 0, stream, sp, S, rows, ja, (const float *)val,
+              (const float *)dl, (const float *)b, (float *)y, (const float *)zbase, (float *)zsum, ctl, limit);
+  else
+    BN_LAUNCH(k_trsv_parts<double>, dim3(np), dim3(BN_PART_BLOCK), 0, stream, sp, S, rows, ja,
+              (const double *)val, (const double *)dl, (const double *)b, (double *)y, (const double *)zbase,
+              (double *)zsum, ctl, limit);
+}
+
+int bn_trsv_parts_per_cu(char vt) {
+  int nb = 0;
+  const hipError_t e = vt == 's'
+                           ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trsv_parts<float>, BN_PART_BLOCK, 0)
+                           : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trsv_parts<double>, BN_PART_BLOCK, 0);
+  return e == hipSuccess ? nb : 0;
+}
+
+void bn_walk_fill(char vt, int n, void *y, hipStream_t stream) {
+  if (n <= 0) return;
+  if (vt == 's')
+    BN_LAUNCH(k_fill_word<unsigned int>, g1(n), dim3(BLOCK), 0, stream, n, (unsigned int *)y,
+              walk_word<float>::none);
+  else
+    BN_LAUNCH(k_fill_word<unsigned long long>, g1(n), dim3(BLOCK), 0, stream, n, (unsigned long long *)y,
+              walk_word<double>::none);
 }
 
 void bn_ell_gather(long long n, const int *src, const double *from, double *out, float *outs, hipStream_t stream) {

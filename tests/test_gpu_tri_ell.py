@@ -1,10 +1,14 @@
 """The exact IC(0) triangular solves (C_amg1/src/ichol.c:121-165) from the
-level-ordered ELL copies (one launch per level) against the same solves
-from the CSR rows with one lane per row:
-every row's sum is formed in the same order, so the preconditioner, every
-PCG iterate and the solution must agree to the last bit (FP64 and FP32,
-fine factor and the coarsest IC-PCG's factor), and the ELL copies must
-follow the factors through a value-only rebuild on the device."""
+level-ordered ELL copies against the same solves from the CSR rows with one
+lane per row: every row's sum is formed in the same order, so the
+preconditioner, every PCG iterate and the solution must agree to the last
+bit (FP64 and FP32, fine factor and the coarsest IC-PCG's factor), and the
+ELL copies must follow the factors through a value-only rebuild on the
+device.  Inside a PCG solve the ELL levels run as one launch per run of
+levels (bn_trsv_walk, rows waiting on their predecessors' values); outside
+it, and with BN_TRI_WALK=0, one launch per level.  Both must agree to the
+last bit, and a walk that gives up waiting must hand the solve back to the
+launch-per-level form with the same result."""
 import sys
 
 import numpy as np
@@ -26,9 +30,10 @@ def net():
     return make_network(52, 10, seed=2)
 
 
-def solver(monkeypatch, net, ell, **kw):
+def solver(monkeypatch, net, ell, walk=True, **kw):
     monkeypatch.setenv("BN_NO_AUTOTUNE", "1")
     monkeypatch.setenv("BN_TRI_ELL", "1" if ell else "0")
+    monkeypatch.setenv("BN_TRI_WALK", "1" if walk else "0")
     return AMGSolver(net.ia, net.ja, net.val, 0.25, 500, 20, 1, coarsening="pmis", **kw)
 
 
@@ -76,3 +81,35 @@ def test_ell_copies_follow_a_device_rebuild(monkeypatch, net):
     (xa, ia_), (xb, ib) = res
     print(f"after the rebuild: iterations {ia_} / {ib}", flush=True)
     assert ia_ == ib and np.array_equal(xa, xb)
+
+
+@pytest.mark.parametrize("mode", list(MODES))
+def test_walk_equals_a_launch_per_level(monkeypatch, net, mode):
+    case(f"exact IC(0): one launch per run of levels vs one per level, {mode}, n={net.n}")
+    out = []
+    for walk in (True, False):
+        s = solver(monkeypatch, net, True, walk=walk, **MODES[mode])
+        x, it = s.solve(net.b, tol=1e-10, maxiter=500)
+        x2, it2 = s.solve(net.b, tol=1e-10, maxiter=500)   # replayed step graphs
+        out.append((x, it, x2, it2))
+        s.free()
+    (xw, iw, xw2, iw2), (xl, il, xl2, il2) = out
+    print(f"iterations walk {iw}/{iw2}, per level {il}/{il2}", flush=True)
+    assert iw > 0 and iw == il == iw2 == il2
+    assert np.array_equal(xw, xl) and np.array_equal(xw2, xl2) and np.array_equal(xw, xw2)
+
+
+def test_walk_that_gives_up_hands_back_to_a_launch_per_level(monkeypatch, capfd, net):
+    case(f"exact IC(0) walk forced to give up at its first wait (BN_TRI_WALK_LIMIT=-1), n={net.n}")
+    ref = solver(monkeypatch, net, True, walk=False)
+    x0, i0 = ref.solve(net.b, tol=1e-10, maxiter=500)
+    ref.free()
+    monkeypatch.setenv("BN_TRI_WALK_LIMIT", "-1")
+    s = solver(monkeypatch, net, True, walk=True)
+    x1, i1 = s.solve(net.b, tol=1e-10, maxiter=500)
+    x2, i2 = s.solve(net.b, tol=1e-10, maxiter=500)   # the solver keeps the launch per level
+    s.free()
+    err = capfd.readouterr().err
+    print(f"iterations {i1}/{i2} vs {i0}", flush=True)
+    assert "timed out" in err
+    assert i1 == i2 == i0 and np.array_equal(x1, x0) and np.array_equal(x2, x0)
