@@ -111,6 +111,14 @@ typedef struct {
   int gs_nblk;          /* row blocks of Ac, colour by colour */
   int *gs_blk, *gs_blkq;/* device: their row bounds and entry offsets */
   int *gs_cblk;         /* host: first block of each colour (ncol + 1) */
+  /* one-launch sweep (bn_gs_walk): Ac's column words with bit 31 where the
+   * neighbour's colour precedes the row's, each part's rows of each colour
+   * (int2 per part and colour), parts, the sweep input's copy, and the
+   * lanes per row timed fastest (FP64, FP32; 0: a launch per colour) */
+  int *gs_jx, *gs_wsteps;
+  int gs_wnp;
+  void *gs_xold;
+  int gs_walk, gs_walk_s;
   int *perm_h;          /* host perm[new] = old */
   int *perm_d;
   void *b, *x, *r;      /* level work vectors in V-cycle precision */
@@ -173,7 +181,8 @@ struct bn_solver {
   int *tune;            /* tuned kernel choices of the last upload (6 ints
                            per operator), reused while the shapes match */
   int ntune;
-  int tune_gs[AMG_MAX_LEVELS][2]; /* the levels' sweep lanes (FP64, FP32) */
+  int tune_gs[AMG_MAX_LEVELS][4]; /* the levels' sweep lanes: colour launches
+                                     (FP64, FP32), one-launch sweep (FP64, FP32) */
   int vperm_state;      /* caller value order -> Asolve slots: 0 not yet
                            known, 1 the same order, 2 gathered by vperm_d */
   int *vperm_d;         /* device: Asolve slot k takes caller slot vperm_d[k] */

@@ -307,7 +307,7 @@ static float time_launches(struct bn_solver *s, const bn_tune_env *env, bn_dcsr 
 /* Lanes per row of a GS level's colour kernels: each option times whole
  * cold forward sweeps (all colours back to back, as the V-cycle runs
  * them), FP64 and FP32 */
-static void tune_gs(struct bn_solver *s, const bn_tune_env *env, bn_dlevel *d, void *x, void *b) {
+static void tune_gs(struct bn_solver *s, const bn_tune_env *env, bn_dlevel *d, void *x, void *b, float *best_ms) {
   enum { NOPT = 6 };
   const int opts[NOPT] = {1, 2, 4, 8, 16, BN_GS_BLOCKS};
   /* BN_GS_LANES pins the colour kernels' lanes per row (1, 2, 4, 8 or
@@ -329,6 +329,7 @@ static void tune_gs(struct bn_solver *s, const bn_tune_env *env, bn_dlevel *d, v
     for (int o = 0; o < NOPT; o++)
       if (opts[o] == l) {
         d->gs_lanes = d->gs_lanes_s = l;
+        best_ms[0] = best_ms[1] = -1.0f; /* pinned: not timed */
         return;
       }
   }
@@ -369,6 +370,85 @@ static void tune_gs(struct bn_solver *s, const bn_tune_env *env, bn_dlevel *d, v
               1e3 * best / BN_TUNE_REPS);
     if (p) d->gs_lanes_s = best_l;
     else d->gs_lanes = best_l;
+    best_ms[p] = best;
+  }
+}
+
+/* the one-launch sweep of a GS level (bn_gs_walk) with 1, 2, 4 or 8 lanes
+ * per row, timed like the colour launches (whole cold forward sweeps from a
+ * nonzero start, the input copy included); kept where it beats their best
+ * time.  BN_GS_WALK=0 keeps a launch per colour; BN_GS_WALK=1/2/4/8 pins
+ * the lanes of the one-launch sweep on every level that has one. */
+static void tune_gs_walk(struct bn_solver *s, const bn_tune_env *env, bn_dlevel *d, void *x, void *b,
+                         const float *best_ms) {
+  d->gs_walk = d->gs_walk_s = 0;
+  if (!d->gs_jx || !s->walk_ok) return;
+  const char *pin = getenv("BN_GS_WALK");
+  if (pin && pin[0]) {
+    const int l = atoi(pin);
+    if (l == 1 || l == 2 || l == 4 || l == 8) d->gs_walk = d->gs_walk_s = l;
+    return;
+  }
+  if (best_ms[0] < 0.0f) return; /* colour lanes pinned: keep the colour launches */
+  const int opts[4] = {1, 2, 4, 8};
+  double *ctl = s->scal + BN_S_WALKERR;
+  for (int p = 0; p < 2; p++) {
+    const char vt = p ? 's' : 'd';
+    const void *dinvc = vt == 's' ? (const void *)d->dinvc_s : (const void *)d->dinvc_d;
+    const void *val = vt == 's' ? (const void *)d->Ac.vs : (const void *)d->Ac.vd;
+    int best_l = 0;
+    float best = best_ms[p];
+    for (int o = 0; o < 4; o++) {
+      bn_fill('d', 1, ctl, 0.0, s->stream);
+      for (int r = -1; r < BN_TUNE_REPS; r++) {
+        if (env->flush) bn_dot_partial(env->nflush, env->flush, env->flush, NULL, NULL, s->part1, NULL, s->stream);
+        if (r >= 0) hipEventRecord(env->ev[2 * r], s->stream);
+        bn_gs_prep(vt, d->n, x, d->gs_xold, s->stream);
+        bn_gs_walk(vt, opts[o], d->gs_wnp, d->ncol, 0, d->gs_wsteps, d->Ac.ia, d->gs_jx, val, d->cmap_d, dinvc, b,
+                   x, d->gs_xold, 0, NULL, NULL, ctl, s->walk_limit, s->stream);
+        if (r >= 0) hipEventRecord(env->ev[2 * r + 1], s->stream);
+      }
+      hipEventSynchronize(env->ev[2 * BN_TUNE_REPS - 1]);
+      double flag = 0.0;
+      hipMemcpy(&flag, ctl, sizeof(double), hipMemcpyDeviceToHost);
+      float total = 0.0f;
+      for (int r = 0; r < BN_TUNE_REPS; r++) {
+        float ms = 0.0f;
+        hipEventElapsedTime(&ms, env->ev[2 * r], env->ev[2 * r + 1]);
+        total += ms;
+      }
+      const char *lg = getenv("BN_TUNE_LOG");
+      if (lg && lg[0] == '2')
+        fprintf(stderr, "  gs walk %c rows=%d colours=%d lanes=%d: %.1f us per cold sweep%s\n", vt, d->n, d->ncol,
+                opts[o], 1e3 * total / BN_TUNE_REPS, flag != 0.0 ? " (gave up)" : "");
+      if (flag != 0.0) {
+        bn_fill('d', 1, ctl, 0.0, s->stream);
+        best_l = 0;
+        break; /* a wait gave up: the colour launches serve */
+      }
+      if (total < best) best = total, best_l = opts[o];
+    }
+    const char *lg = getenv("BN_TUNE_LOG");
+    if (lg && (lg[0] == '1' || lg[0] == '2'))
+      fprintf(stderr, "tune gs walk %c rows=%d colours=%d: lanes=%d (%.1f us per cold sweep; colour launches %.1f)\n",
+              vt, d->n, d->ncol, best_l, 1e3 * best / BN_TUNE_REPS, 1e3 * best_ms[p] / BN_TUNE_REPS);
+    if (p) d->gs_walk_s = best_l;
+    else d->gs_walk = best_l;
+  }
+  hipStreamSynchronize(s->stream);
+}
+
+/* Without the autotuner the colour kernels run Ac's SpMV lanes and no
+ * sweep runs in one launch, unless BN_GS_LANES / BN_GS_WALK pin them (as
+ * with the tuner; one value for every level) */
+static void gs_pins(struct bn_solver *s) {
+  const char *pl = getenv("BN_GS_LANES"), *pw = getenv("BN_GS_WALK");
+  const int l = pl ? atoi(pl) : 0, w = pw ? atoi(pw) : 0;
+  for (int k = 0; k < s->nlev; k++) {
+    bn_dlevel *d = &s->lev[k];
+    if (!d->use_gs || d->block) continue;
+    if (l == 1 || l == 2 || l == 4 || l == 8 || l == 16 || l == BN_GS_BLOCKS) d->gs_lanes = d->gs_lanes_s = l;
+    d->gs_walk = d->gs_walk_s = (d->gs_jx && s->walk_ok && (w == 1 || w == 2 || w == 4 || w == 8)) ? w : 0;
   }
 }
 
@@ -511,9 +591,9 @@ static int tune_file_load(struct bn_solver *s, int nops, bn_dcsr *const *ops) {
   int nf = 0, nl = 0;
   int ok = fscanf(f, "%d %d", &nf, &nl) == 2 && nf == nops && nl == s->nlev;
   int *t = (int *)bn_xmalloc((size_t)(nops > 0 ? nops : 1) * BN_TUNE_FIELDS * sizeof(int));
-  int g[AMG_MAX_LEVELS][2];
+  int g[AMG_MAX_LEVELS][4];
   for (int k = 0; ok && k < nops * BN_TUNE_FIELDS; k++) ok = fscanf(f, "%d", &t[k]) == 1;
-  for (int l = 0; ok && l < nl; l++) ok = fscanf(f, "%d %d", &g[l][0], &g[l][1]) == 2;
+  for (int l = 0; ok && l < nl; l++) ok = fscanf(f, "%d %d %d %d", &g[l][0], &g[l][1], &g[l][2], &g[l][3]) == 4;
   fclose(f);
   for (int k = 0; ok && k < nops; k++)
     if (ops[k] && (t[BN_TUNE_FIELDS * k] != ops[k]->rows || t[BN_TUNE_FIELDS * k + 1] != ops[k]->nnz)) ok = 0;
@@ -524,7 +604,8 @@ static int tune_file_load(struct bn_solver *s, int nops, bn_dcsr *const *ops) {
   free(s->tune);
   s->tune = t;
   s->ntune = nops;
-  for (int l = 0; l < nl; l++) s->tune_gs[l][0] = g[l][0], s->tune_gs[l][1] = g[l][1];
+  for (int l = 0; l < nl; l++)
+    for (int q = 0; q < 4; q++) s->tune_gs[l][q] = g[l][q];
   return 1;
 }
 
@@ -538,7 +619,8 @@ static void tune_file_save(const struct bn_solver *s) {
     for (int q = 0; q < BN_TUNE_FIELDS; q++) fprintf(f, "%d ", s->tune[BN_TUNE_FIELDS * k + q]);
     fprintf(f, "\n");
   }
-  for (int l = 0; l < s->nlev; l++) fprintf(f, "%d %d\n", s->tune_gs[l][0], s->tune_gs[l][1]);
+  for (int l = 0; l < s->nlev; l++)
+    fprintf(f, "%d %d %d %d\n", s->tune_gs[l][0], s->tune_gs[l][1], s->tune_gs[l][2], s->tune_gs[l][3]);
   fclose(f);
 }
 
@@ -564,6 +646,8 @@ static void autotune_all(struct bn_solver *s) {
     for (int l = 0; l < s->nlev; l++) {
       s->lev[l].gs_lanes = s->tune_gs[l][0];
       s->lev[l].gs_lanes_s = s->tune_gs[l][1];
+      s->lev[l].gs_walk = s->lev[l].gs_jx ? s->tune_gs[l][2] : 0;
+      s->lev[l].gs_walk_s = s->lev[l].gs_jx ? s->tune_gs[l][3] : 0;
     }
   } else {
     void *x = s->x, *y = s->q;
@@ -583,9 +667,15 @@ static void autotune_all(struct bn_solver *s) {
       if (ops[k]) autotune_csr(s, &env, ops[k], x, y, flags[k] & 1, (flags[k] & 2) != 0);
     for (int l = 0; evok && l < s->nlev; l++) {
       bn_dlevel *d = &s->lev[l];
-      if (d->use_gs && !d->block && d->ncol > 0 && d->Ac.ia) tune_gs(s, &env, d, x, y);
+      if (d->use_gs && !d->block && d->ncol > 0 && d->Ac.ia) {
+        float best[2];
+        tune_gs(s, &env, d, x, y, best);
+        tune_gs_walk(s, &env, d, x, y, best);
+      }
       s->tune_gs[l][0] = d->gs_lanes;
       s->tune_gs[l][1] = d->gs_lanes_s;
+      s->tune_gs[l][2] = d->gs_walk;
+      s->tune_gs[l][3] = d->gs_walk_s;
     }
     hipStreamSynchronize(s->stream);
     for (int k = 0; k < 2 * BN_TUNE_REPS + 2; k++)
@@ -938,6 +1028,74 @@ static int upload_chunks(struct bn_solver *s, bn_dlevel *d, const dcsr_mat *Ap, 
   return 0;
 }
 
+/* Tables of the one-launch sweep (bn_gs_walk) of a GS level: the column
+ * words of the colour-ordered copy Ac with bit 31 set where the
+ * neighbour's colour precedes the row's, and each part's rows of each
+ * colour.  Part p holds level rows [p n / np, (p + 1) n / np); the rows of a
+ * colour are ascending in the level numbering, so its piece of a part is a
+ * contiguous range of Ac rows.  np is the most workgroups that fit on the
+ * device at once for every lanes choice and both precisions (all resident),
+ * at most one per 256 rows.  No tables (the colour launches serve) when
+ * nothing fits, with BN_GS_WALK=0, or when the walk is off. */
+static int gs_walk_tables(struct bn_solver *s, bn_dlevel *d, const dcsr_mat *Ac, const int *cperm) {
+  d->gs_jx = d->gs_wsteps = NULL;
+  d->gs_xold = NULL;
+  d->gs_wnp = 0;
+  const char *env = getenv("BN_GS_WALK");
+  if ((env && env[0] == '0') || !s->walk_ok || s->walk_blocks <= 0) return 0;
+  const int n = Ac->row, ncol = d->ncol;
+  if (n <= 0 || ncol <= 0 || Ac->nnz <= 0) return 0;
+  int fit = 2;
+  const int lanes[4] = {1, 2, 4, 8};
+  for (int k = 0; k < 4; k++) {
+    const int fd = bn_gs_walk_per_cu('d', lanes[k]), fs = bn_gs_walk_per_cu('s', lanes[k]);
+    if (fd < fit) fit = fd;
+    if (fs < fit) fit = fs;
+  }
+  if (fit <= 0) return 0;
+  int np = fit * s->walk_blocks;
+  if (np > (n + 255) / 256) np = (n + 255) / 256;
+  if (np < 1) np = 1;
+  int *colr = (int *)bn_xmalloc((size_t)n * sizeof(int));
+  for (int c = 0; c < ncol; c++)
+    for (int r = d->col_ptr[c]; r < d->col_ptr[c + 1]; r++) colr[cperm[r]] = c;
+  int *jx = (int *)bn_xmalloc((size_t)Ac->nnz * sizeof(int));
+#pragma omp parallel for schedule(static) if (Ac->nnz > 65536)
+  for (int r = 0; r < n; r++) {
+    const int c = colr[cperm[r]];
+    for (int q = Ac->ia[r]; q < Ac->ia[r + 1]; q++) {
+      const int j = Ac->ja[q];
+      jx[q] = colr[j] < c ? (int)((unsigned)j | 0x80000000u) : j;
+    }
+  }
+  free(colr);
+  int *st = (int *)bn_xmalloc((size_t)np * ncol * 2 * sizeof(int));
+  for (int c = 0; c < ncol; c++) {
+    const int lo = d->col_ptr[c], hi = d->col_ptr[c + 1];
+    int r = lo;
+    for (int p = 0; p < np; p++) {
+      const long long top = (long long)(p + 1) * n / np;
+      const int r0 = r;
+      while (r < hi && cperm[r] < top) r++;
+      st[2 * ((size_t)p * ncol + c)] = r0;
+      st[2 * ((size_t)p * ncol + c) + 1] = r - r0;
+    }
+  }
+  int rc = 0;
+  d->gs_jx = (int *)dmalloc(s, (size_t)Ac->nnz * sizeof(int));
+  d->gs_wsteps = (int *)dmalloc(s, (size_t)np * ncol * 2 * sizeof(int));
+  d->gs_xold = dmalloc(s, (size_t)n * sizeof(double));
+  if (!d->gs_jx || !d->gs_wsteps || !d->gs_xold ||
+      hipMemcpy(d->gs_jx, jx, (size_t)Ac->nnz * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+      hipMemcpy(d->gs_wsteps, st, (size_t)np * ncol * 2 * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)
+    rc = BN_ERR_HIP;
+  free(jx);
+  free(st);
+  if (rc) return rc;
+  d->gs_wnp = np;
+  return 0;
+}
+
 /* Colour-ordered row copy of a GS level: row r of Ac is the off-diagonal
  * part of row cperm[r] of Ap (columns unchanged; the sweep divides by the
  * diagonal through 1/a_ii, so a row's diagonal entry is never read), plus
@@ -992,6 +1150,7 @@ static int upload_colour_rows(struct bn_solver *s, bn_dlevel *d, const dcsr_mat 
     CK(hipMemcpy(d->gs_blkq, bk, ((size_t)nb + 1) * sizeof(int), hipMemcpyHostToDevice));
     free(bk);
   }
+  if (!rc) rc = gs_walk_tables(s, d, &Ac, cperm);
   bn_csr_free(&Ac);
   if (rc) return rc;
   d->cmap_d = (int *)dmalloc(s, (size_t)(n > 0 ? n : 1) * sizeof(int));
@@ -1236,6 +1395,7 @@ int bn_device_upload(struct bn_solver *s) {
   if (!rc) {
     const char *env = getenv("BN_NO_AUTOTUNE");
     if (!(env && env[0] && env[0] != '0')) autotune_all(s);
+    else gs_pins(s);
     if (apply_mode() == 2) pin_blocks(s);
     CK(hipDeviceSynchronize());
     s->dev_ready = 1;

@@ -166,6 +166,20 @@ static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int bac
     return;
   }
   const int gs = d->use_gs && s->opt.smoother == SMOOTHER_CF_GS;
+  const int walk = vt == 's' ? d->gs_walk_s : d->gs_walk;
+  if (gs && s->walk_on && walk > 0 && d->gs_jx) {
+    /* each sweep in one launch (bn_gs_walk): the input copied aside, x
+     * filled with the sentinel */
+    const void *dinvc = vecv(d->dinvc_d, d->dinvc_s, vt);
+    for (int sw = 0; sw < nu; sw++) {
+      const int last = (sw == nu - 1), zero = x_zero && sw == 0;
+      bn_gs_prep(vt, n, x, zero ? NULL : d->gs_xold, s->stream);
+      bn_gs_walk(vt, walk, d->gs_wnp, d->ncol, backward, d->gs_wsteps, d->Ac.ia, d->gs_jx, opv(&d->Ac, vt),
+                 d->cmap_d, dinvc, b, x, d->gs_xold, zero, last ? zbase : NULL, last ? zsum : NULL,
+                 s->scal + BN_S_WALKERR, s->walk_limit, s->stream);
+    }
+    return;
+  }
   if (gs) {
     /* from zero, the first colour's kernel also zeroes the other rows
      * (unless the first sweep is also the last and must write zsum) */

@@ -225,6 +225,26 @@ int bn_trsv_parts_per_cu(char vt);
 void bn_trsv_parts(char vt, int np, const int *sp, const int *steps, const int *rows, const int *ja,
                    const void *val, const void *dl, const void *b, void *y, const void *zbase, void *zsum,
                    double *ctl, long long limit, hipStream_t stream);
+/* One multicolour Gauss-Seidel sweep of a level in one launch of np
+ * workgroups of BN_GSW_BLOCK threads (np no more than fit on the device at
+ * once).  Part p's rows of colour c are rows steps[2*(p*ncol+c)] .. + its
+ * count (int2) of the colour-ordered copy (ia, jx, val; row r updates
+ * level row map[r] with 1/a_ii dinvc[r]); jx is the copy's column array
+ * with bit 31 set where the neighbour's colour precedes the row's.
+ * Colours run in reverse when backward.  x must hold the sentinel
+ * (bn_gs_prep) and xold the sweep's input (zero: the input is zero, xold
+ * unused); the sweep writes x (and zsum = zbase + x when zsum != NULL).
+ * Rows of `lanes` lanes (1, 2, 4 or 8) form the colour launches' sums with
+ * the same lanes, to the last bit.  Waits, limit and *ctl as
+ * bn_trsv_parts. */
+#define BN_GSW_BLOCK 1024
+/* workgroups of bn_gs_walk that fit on one CU at once (0: unknown) */
+int bn_gs_walk_per_cu(char vt, int lanes);
+void bn_gs_walk(char vt, int lanes, int np, int ncol, int backward, const int *steps, const int *ia, const int *jx,
+                const void *val, const int *map, const void *dinvc, const void *b, void *x, void *xold, int zero,
+                const void *zbase, void *zsum, double *ctl, long long limit, hipStream_t stream);
+/* xold = x (xold may be NULL), then x = the sentinel */
+void bn_gs_prep(char vt, int n, void *x, void *xold, hipStream_t stream);
 /* y[0..n) = the walk's sentinel (a signalling-NaN bit pattern) */
 void bn_walk_fill(char vt, int n, void *y, hipStream_t stream);
 /* out[q] = from[src[q]] (0 where src[q] < 0), and its FP32 rounding */

@@ -81,10 +81,13 @@ __device__ __forceinline__ void clear_tail(double *partial) {
 // dependent load rounds.  R = 8 for two lanes per row (a 7-entry fine
 // row then finishes in one round), 4 otherwise (one lane: the extra
 // registers and masked loads of R = 8 measured slower).  SKIP_DIAG drops
-// the entry whose column equals `self`.
-template <typename T, int LANES, bool SKIP_DIAG, int R = (LANES == 2 ? 8 : 4)>
-__device__ __forceinline__ T row_dot(int q, const int e, const int *__restrict__ ja,
-                                     const T *__restrict__ val, const T *__restrict__ x, int self) {
+// the entry whose column equals `self`.  `gather(c, j)` returns x at
+// stored column word c and sets j to the column it names (the plain form:
+// x[c], j = c), so a caller can keep flags in the column words or read x
+// from more than one place with the same sums.
+template <typename T, int LANES, bool SKIP_DIAG, int R = (LANES == 2 ? 8 : 4), typename G>
+__device__ __forceinline__ T row_dot_g(int q, const int e, const int *__restrict__ ja,
+                                       const T *__restrict__ val, G gather, int self) {
   T acc[4] = {0, 0, 0, 0};
   for (; q + (R - 1) * LANES < e; q += R * LANES) {
     int j[R];
@@ -98,8 +101,10 @@ __device__ __forceinline__ T row_dot(int q, const int e, const int *__restrict__
     for (int u = 0; u < R; u++) {
       // select the product, not the gathered value: x[self] may be stale
       // (a triangular solve's own unknown) and is never used
-      const T p = a[u] * x[j[u]];
-      acc[u & 3] += (SKIP_DIAG && j[u] == self) ? T(0) : p;
+      int jd;
+      const T xv = gather(j[u], jd);
+      const T p = a[u] * xv;
+      acc[u & 3] += (SKIP_DIAG && jd == self) ? T(0) : p;
     }
   }
   if (q < e) {
@@ -121,11 +126,25 @@ __device__ __forceinline__ T row_dot(int q, const int e, const int *__restrict__
       // selected away - on the product, since a load feeding a select
       // becomes a branch around the load
       const T hm = q + u * LANES < e ? T(1) : T(0);
-      const T p = (a[u] * hm) * x[j[u]];
-      acc[u & 3] += (SKIP_DIAG && j[u] == self) ? T(0) : p;
+      int jd;
+      const T xv = gather(j[u], jd);
+      const T p = (a[u] * hm) * xv;
+      acc[u & 3] += (SKIP_DIAG && jd == self) ? T(0) : p;
     }
   }
   return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
+template <typename T, int LANES, bool SKIP_DIAG, int R = (LANES == 2 ? 8 : 4)>
+__device__ __forceinline__ T row_dot(int q, const int e, const int *__restrict__ ja,
+                                     const T *__restrict__ val, const T *__restrict__ x, int self) {
+  return row_dot_g<T, LANES, SKIP_DIAG, R>(
+      q, e, ja, val,
+      [x](int c, int &j) {
+        j = c;
+        return x[c];
+      },
+      self);
 }
 
 // Block-wide sum over one long row: each thread keeps 16 strides of
@@ -899,6 +918,118 @@ __global__ __launch_bounds__(BN_PART_BLOCK) void k_trsv_parts(const int *__restr
   }
 }
 
+// ---------------------------------------------------------------------
+// One whole multicolour Gauss-Seidel sweep of a level in ONE launch (the
+// colour launches' k_rowsolve<T, LANES, RS_MAP> rows, same sums).  The
+// level's rows are cut into contiguous parts (level numbering), one
+// workgroup each; a part takes its rows of each colour in sweep order
+// (backward: colours in reverse), with a workgroup barrier between
+// colours.  Before the launch x holds the sentinel on every row and xold
+// the sweep's input (ZERO: the input is zero and xold is not read).  A row
+// reads a neighbour of an earlier colour of the sweep from x (its new
+// value, waited for while it still holds the sentinel) and one of a later
+// colour from xold, which nothing writes during the sweep - exactly the
+// values the colour launches read, without the launch between colours.
+// The column words of the colour-ordered copy carry bit 31 when the
+// neighbour's colour precedes the row's (jx, forward order).  Waits,
+// bounds, the give-up flag and the deadlock argument are k_trsv_parts's.
+// ---------------------------------------------------------------------
+template <typename T, int LANES, bool ZERO, bool COHERENT>
+__device__ __forceinline__ T gs_walk_sum(int r, int i, const int *__restrict__ ia, const int *__restrict__ jx,
+                                         const T *__restrict__ val, const T *x, const T *__restrict__ xold,
+                                         int backward, int sub, bool &bad) {
+  typedef typename walk_word<T>::U U;
+  bool b0 = false;
+  const T s = row_dot_g<T, LANES, true>(
+      ia[r] + sub, ia[r + 1], jx, val,
+      [&](int c, int &j) -> T {
+        j = c & 0x7fffffff;
+        const bool early = (c < 0) != (backward != 0);
+        U u;
+        if (ZERO) {
+          // a later colour's input is zero: read x (the sentinel there
+          // reads as "not early", so it never counts)
+          u = COHERENT ? walk_load(x + j) : ((const U *)x)[j];
+          u = early ? u : U(0);
+        } else {
+          const T *src = early ? x : xold;
+          u = COHERENT ? walk_load(src + j) : ((const U *)src)[j];
+        }
+        b0 |= early & (u == walk_word<T>::none);
+        return __builtin_bit_cast(T, u);
+      },
+      i);
+  bad = b0;
+  return s;
+}
+
+template <typename T, int LANES, bool ZERO>
+__global__ __launch_bounds__(BN_GSW_BLOCK) void k_gs_walk(int ncol, int backward, const int2 *__restrict__ steps,
+                                                          const int *__restrict__ ia, const int *__restrict__ jx,
+                                                          const T *__restrict__ val, const int *__restrict__ map,
+                                                          const T *__restrict__ dinvc, const T *__restrict__ b, T *x,
+                                                          const T *__restrict__ xold, const T *__restrict__ zbase,
+                                                          T *__restrict__ zsum, double *ctl, long long limit) {
+  typedef typename walk_word<T>::U U;
+  constexpr int RPP = BN_GSW_BLOCK / LANES;
+  const int p = blockIdx.x, sub = threadIdx.x % LANES, slot = threadIdx.x / LANES;
+  bool live = !walk_stopped(ctl);
+  for (int t = 0; t < ncol; t++) {
+    const int c = backward ? ncol - 1 - t : t;
+    const int2 S = steps[p * ncol + c];  // {first row of the colour-ordered copy, rows}
+    if (S.y == 0) continue;              // the same for the whole workgroup
+    for (int k = slot; live && k < S.y; k += RPP) {
+      const int r = S.x + k;
+      const int i = map[r];
+      const T bi = b[i], dr = dinvc[r];
+      bool bad;
+      T s = gs_walk_sum<T, LANES, ZERO, false>(r, i, ia, jx, val, x, xold, backward, sub, bad);
+      if (LANES > 1) s = group_sum<LANES>(s);
+      int nbad = group_sum<LANES>(bad ? 1 : 0);
+      long long t0 = 0;
+      for (int spin = 0; nbad; spin++) {
+        // an earlier colour's neighbour was not there yet: the whole row
+        // again with coherent loads, until none reads the sentinel
+        // every 16th retry: has another thread given up, and is the limit
+        // past (limit < 0 gives up at the first retry: the host forces the
+        // fallback)
+        if (limit < 0 || (spin & 15) == 15) {
+          const long long tc = (long long)wall_clock64();
+          if (spin == 15) t0 = tc;
+          if (walk_stopped(ctl) || limit < 0 || (spin > 15 && tc - t0 > limit)) {
+            if (!walk_stopped(ctl))
+              __hip_atomic_store((unsigned long long *)ctl, 0x3ff0000000000000ULL /* 1.0 */, __ATOMIC_RELAXED,
+                                 __HIP_MEMORY_SCOPE_AGENT);
+            live = false;
+            break;
+          }
+        }
+        __builtin_amdgcn_s_sleep(1);
+        s = gs_walk_sum<T, LANES, ZERO, true>(r, i, ia, jx, val, x, xold, backward, sub, bad);
+        if (LANES > 1) s = group_sum<LANES>(s);
+        nbad = group_sum<LANES>(bad ? 1 : 0);
+      }
+      if (!live) break;
+      if (sub == 0) {
+        const T v = (bi - s) * dr;
+        __hip_atomic_store((U *)(x + i), __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (zsum) zsum[i] = zbase[i] + v;
+      }
+    }
+    __syncthreads();  // a failed thread still meets every barrier
+  }
+}
+
+// before a sweep: xold = x (unless the input is zero), x = the sentinel
+template <typename T>
+__global__ void k_gs_prep(int n, T *__restrict__ x, T *__restrict__ xold) {
+  typedef typename walk_word<T>::U U;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  if (xold) xold[i] = x[i];
+  ((U *)x)[i] = walk_word<T>::none;
+}
+
 template <typename U>
 __global__ void k_fill_word(int n, U *__restrict__ x, U v) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -1645,6 +1776,67 @@ int bn_trsv_parts_per_cu(char vt) {
                            ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trsv_parts<float>, BN_PART_BLOCK, 0)
                            : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_trsv_parts<double>, BN_PART_BLOCK, 0);
   return e == hipSuccess ? nb : 0;
+}
+
+int bn_gs_walk_per_cu(char vt, int lanes) {
+  int nb = 0, best = 1 << 30;
+  for (int z = 0; z < 2; z++) {
+    hipError_t e = hipErrorInvalidValue;
+#define BN_OCC(T, L)                                                                                         \
+  e = z ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gs_walk<T, L, true>, BN_GSW_BLOCK, 0)        \
+        : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_gs_walk<T, L, false>, BN_GSW_BLOCK, 0)
+    if (vt == 's') {
+      switch (lanes) {
+        case 1: BN_OCC(float, 1); break;
+        case 2: BN_OCC(float, 2); break;
+        case 4: BN_OCC(float, 4); break;
+        default: BN_OCC(float, 8); break;
+      }
+    } else {
+      switch (lanes) {
+        case 1: BN_OCC(double, 1); break;
+        case 2: BN_OCC(double, 2); break;
+        case 4: BN_OCC(double, 4); break;
+        default: BN_OCC(double, 8); break;
+      }
+    }
+#undef BN_OCC
+    if (e != hipSuccess) return 0;
+    if (nb < best) best = nb;
+  }
+  return best;
+}
+
+void bn_gs_walk(char vt, int lanes, int np, int ncol, int backward, const int *steps, const int *ia, const int *jx,
+                const void *val, const int *map, const void *dinvc, const void *b, void *x, void *xold, int zero,
+                const void *zbase, void *zsum, double *ctl, long long limit, hipStream_t stream) {
+  if (np <= 0 || ncol <= 0) return;
+  const int2 *S = (const int2 *)steps;
+#define BN_GW(T, L, Z)                                                                                            \
+  BN_LAUNCH((k_gs_walk<T, L, Z>), dim3(np), dim3(BN_GSW_BLOCK), 0, stream, ncol, backward, S, ia, jx, (const T *)val, \
+            map, (const T *)dinvc, (const T *)b, (T *)x, (const T *)xold, (const T *)zbase, (T *)zsum, ctl, limit)
+#define BN_GWL(T, Z)              \
+  switch (lanes) {                \
+    case 1: BN_GW(T, 1, Z); break; \
+    case 2: BN_GW(T, 2, Z); break; \
+    case 4: BN_GW(T, 4, Z); break; \
+    default: BN_GW(T, 8, Z); break; \
+  }
+  if (vt == 's') {
+    if (zero) { BN_GWL(float, true) } else { BN_GWL(float, false) }
+  } else {
+    if (zero) { BN_GWL(double, true) } else { BN_GWL(double, false) }
+  }
+#undef BN_GWL
+#undef BN_GW
+}
+
+void bn_gs_prep(char vt, int n, void *x, void *xold, hipStream_t stream) {
+  if (n <= 0) return;
+  if (vt == 's')
+    BN_LAUNCH(k_gs_prep<float>, g1(n), dim3(BLOCK), 0, stream, n, (float *)x, (float *)xold);
+  else
+    BN_LAUNCH(k_gs_prep<double>, g1(n), dim3(BLOCK), 0, stream, n, (double *)x, (double *)xold);
 }
 
 void bn_walk_fill(char vt, int n, void *y, hipStream_t stream) {
