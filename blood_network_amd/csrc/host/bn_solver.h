@@ -85,6 +85,14 @@ typedef struct {
   int nf, nb;           /* levels of the forward / backward sweep */
   int *rows_f, *rows_b; /* device: rows grouped by level */
   int *ptr_f, *ptr_b;   /* host: level boundaries (NULL: not built) */
+  /* one-launch sweep (bn_sweep_parts; NULL jx: a launch per level): the
+   * operator's column words with bit 31 where the forward sweep visits the
+   * neighbour earlier, parts, each part's steps (device: offsets np + 1 and
+   * int2 steps, per direction), the sweep input's copy */
+  int *jx;
+  int np;
+  int *sp_f, *sp_b, *st_f, *st_b;
+  void *xold;
 } bn_sweep;
 
 typedef struct {

@@ -928,6 +928,100 @@ void bn_sweep_free_host(bn_sweep *w) {
   memset(w, 0, sizeof(*w));
 }
 
+/* one direction's steps of the one-launch sweep: part p holds rows
+ * [p n / np, (p + 1) n / np); its piece of each dependency level is
+ * contiguous in the level's ascending row list */
+static int sweep_steps(struct bn_solver *s, int n, int np, int nl, const int *ptr, const int *rows, int **sp_d,
+                       int **st_d) {
+  long long cnt = 0;
+  int *kb = (int *)bn_xmalloc((size_t)(nl > 0 ? nl : 1) * (size_t)(np + 1) * sizeof(int));
+  for (int l = 0; l < nl; l++) {
+    int k = 0;
+    const int c = ptr[l + 1] - ptr[l];
+    for (int p = 0; p <= np; p++) {
+      const long long lo = (long long)p * n / np;
+      while (k < c && rows[ptr[l] + k] < lo) k++;
+      kb[(size_t)l * (np + 1) + p] = p == np ? c : k;
+    }
+    for (int p = 0; p < np; p++) cnt += kb[(size_t)l * (np + 1) + p + 1] > kb[(size_t)l * (np + 1) + p];
+  }
+  int *sp = (int *)bn_xmalloc((size_t)(np + 1) * sizeof(int));
+  int *st = (int *)bn_xmalloc((size_t)(cnt > 0 ? cnt : 1) * 2 * sizeof(int));
+  long long at = 0;
+  for (int p = 0; p < np; p++) {
+    sp[p] = (int)at;
+    for (int l = 0; l < nl; l++) {
+      const int k0 = kb[(size_t)l * (np + 1) + p], k1 = kb[(size_t)l * (np + 1) + p + 1];
+      if (k1 == k0) continue;
+      st[2 * at] = ptr[l] + k0;
+      st[2 * at + 1] = k1 - k0;
+      at++;
+    }
+  }
+  sp[np] = (int)at;
+  free(kb);
+  *sp_d = (int *)dmalloc(s, (size_t)(np + 1) * sizeof(int));
+  *st_d = (int *)dmalloc(s, (size_t)(cnt > 0 ? cnt : 1) * 2 * sizeof(int));
+  int rc = (!*sp_d || !*st_d) ? -1 : 0;
+  if (!rc && (hipMemcpy(*sp_d, sp, (size_t)(np + 1) * sizeof(int), hipMemcpyHostToDevice) != hipSuccess ||
+              (cnt > 0 && hipMemcpy(*st_d, st, (size_t)cnt * 2 * sizeof(int), hipMemcpyHostToDevice) != hipSuccess)))
+    rc = -1;
+  free(sp);
+  free(st);
+  return rc;
+}
+
+/* Tables of the one-launch exact sweep (bn_sweep_parts).  Parts: as many as
+ * fit on the device at once (at most one per row); BN_SWEEP_PARTS
+ * overrides.  At the 1.07M-unknown network's cf_order sweeps a solve took
+ * 3.8 s with 1 part, 2.5 s with 8, 1.85 s with 64, 1.66 s with 512, against
+ * 1.96 s for a launch per level.  None (a launch per level) with
+ * BN_SWEEP_WALK=0, when the walk is off or nothing fits; a failure here
+ * only keeps the launches. */
+static int sweep_walk_tables(struct bn_solver *s, const dcsr_mat *A, const int *order, const int *rf, const int *rb,
+                             bn_sweep *w) {
+  w->jx = NULL;
+  const char *env = getenv("BN_SWEEP_WALK");
+  if ((env && env[0] == '0') || !s->walk_ok || s->walk_blocks <= 0) return 0;
+  const int n = A->row, nnz = A->nnz;
+  if (n <= 0 || nnz <= 0) return 0;
+  int fit = bn_sweep_parts_per_cu('d');
+  const int fs = bn_sweep_parts_per_cu('s');
+  if (fs < fit) fit = fs;
+  if (fit > 2) fit = 2;
+  if (fit <= 0) return 0;
+  int np = fit * s->walk_blocks;
+  const char *penv = getenv("BN_SWEEP_PARTS");
+  if (penv && atoi(penv) > 0) np = atoi(penv);
+  if (np > fit * s->walk_blocks) np = fit * s->walk_blocks;
+  if (np > n) np = n;
+  if (np < 1) np = 1;
+  int *pos = (int *)bn_xmalloc((size_t)n * sizeof(int));
+  for (int k = 0; k < n; k++) pos[order ? order[k] : k] = k;
+  int *jx = (int *)bn_xmalloc((size_t)nnz * sizeof(int));
+#pragma omp parallel for schedule(static) if (nnz > 65536)
+  for (int i = 0; i < n; i++)
+    for (int q = A->ia[i]; q < A->ia[i + 1]; q++) {
+      const int j = A->ja[q];
+      jx[q] = (j != i && pos[j] < pos[i]) ? (int)((unsigned)j | 0x80000000u) : j;
+    }
+  free(pos);
+  int *d = (int *)dmalloc(s, (size_t)nnz * sizeof(int));
+  void *xo = dmalloc(s, (size_t)n * sizeof(double));
+  int rc = (!d || !xo || hipMemcpy(d, jx, (size_t)nnz * sizeof(int), hipMemcpyHostToDevice) != hipSuccess) ? -1 : 0;
+  free(jx);
+  if (!rc) rc = sweep_steps(s, n, np, w->nf, w->ptr_f, rf, &w->sp_f, &w->st_f);
+  if (!rc) rc = sweep_steps(s, n, np, w->nb, w->ptr_b, rb, &w->sp_b, &w->st_b);
+  if (rc) {
+    (void)hipGetLastError();
+    return 0; /* the launches per level serve */
+  }
+  w->np = np;
+  w->xold = xo;
+  w->jx = d;
+  return 0;
+}
+
 int bn_sweep_upload(struct bn_solver *s, const dcsr_mat *A, const int *order, bn_sweep *w) {
   const int n = A->row;
   int *rf = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
@@ -954,9 +1048,9 @@ int bn_sweep_upload(struct bn_solver *s, const dcsr_mat *A, const int *order, bn
     bn_set_error("copy of a sweep schedule to the device failed");
     rc = BN_ERR_HIP;
   }
-  free(rf);
-  free(rb);
   if (rc) {
+    free(rf);
+    free(rb);
     free(pf);
     free(pb);
     return rc;
@@ -967,7 +1061,10 @@ int bn_sweep_upload(struct bn_solver *s, const dcsr_mat *A, const int *order, bn
   w->rows_b = db;
   w->ptr_f = pf;
   w->ptr_b = pb;
-  return 0;
+  rc = sweep_walk_tables(s, A, order, rf, rb, w);
+  free(rf);
+  free(rb);
+  return rc;
 }
 
 /* Chunks of the block smoother: at most max_rows rows each; at least 256

@@ -117,10 +117,21 @@ static void smooth_exact(struct bn_solver *s, int lev, int kind, const void *b, 
   const int *rows = backward ? w->rows_b : w->rows_f;
   const int *ptr = backward ? w->ptr_b : w->ptr_f;
   const void *dinv = vecv(d->dinv_d, d->dinv_s, vt);
-  for (int sw = 0; sw < nu; sw++)
-    for (int l = 0; l < nl; l++)
-      bn_trsv_level(vt, ptr[l + 1] - ptr[l], rows + ptr[l], d->A.ia, d->A.ja, opv(&d->A, vt), dinv, b, x,
-                    bn_lanes(&d->A, vt), NULL, NULL, s->stream);
+  if (s->walk_on && w->jx) {
+    /* each sweep in one launch (bn_sweep_parts): the input copied aside, x
+     * filled with the sentinel */
+    for (int sw = 0; sw < nu; sw++) {
+      bn_gs_prep(vt, d->n, x, w->xold, s->stream);
+      bn_sweep_parts(vt, bn_lanes(&d->A, vt), w->np, backward, backward ? w->sp_b : w->sp_f,
+                     backward ? w->st_b : w->st_f, rows, d->A.ia, w->jx, opv(&d->A, vt), dinv, b, x, w->xold,
+                     s->scal + BN_S_WALKERR, s->walk_limit, s->stream);
+    }
+  } else {
+    for (int sw = 0; sw < nu; sw++)
+      for (int l = 0; l < nl; l++)
+        bn_trsv_level(vt, ptr[l + 1] - ptr[l], rows + ptr[l], d->A.ia, d->A.ja, opv(&d->A, vt), dinv, b, x,
+                      bn_lanes(&d->A, vt), NULL, NULL, s->stream);
+  }
   if (zsum) {
     bn_copy(vt, d->n, zbase, zsum, s->stream);
     bn_axpy1(vt, d->n, x, zsum, s->stream);

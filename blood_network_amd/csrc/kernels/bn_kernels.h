@@ -243,6 +243,20 @@ int bn_gs_walk_per_cu(char vt, int lanes);
 void bn_gs_walk(char vt, int lanes, int np, int ncol, int backward, const int *steps, const int *ia, const int *jx,
                 const void *val, const int *map, const void *dinvc, const void *b, void *x, void *xold, int zero,
                 const void *zbase, void *zsum, double *ctl, long long limit, hipStream_t stream);
+/* An exact-order Gauss-Seidel sweep in one launch of np workgroups of
+ * BN_PART_BLOCK threads (np no more than fit at once): part p's steps are
+ * steps[sp[p] .. sp[p+1]) (int2 {first entry of the dependency level's row
+ * list `rows`, rows}); row i of (ia, jx, val) updates x_i with dinv[i]; jx
+ * is the column array with bit 31 set where the forward sweep visits the
+ * neighbour earlier (the backward sweep: where it visits it later).  x must
+ * hold the sentinel and xold the sweep's input (bn_gs_prep).  Rows of
+ * `lanes` lanes (1..64; other values as 64, like bn_trsv_level) form
+ * bn_trsv_level's sums, to the last bit.  Waits, limit and *ctl as
+ * bn_trsv_parts. */
+int bn_sweep_parts_per_cu(char vt);
+void bn_sweep_parts(char vt, int lanes, int np, int backward, const int *sp, const int *steps, const int *rows,
+                    const int *ia, const int *jx, const void *val, const void *dinv, const void *b, void *x,
+                    const void *xold, double *ctl, long long limit, hipStream_t stream);
 /* xold = x (xold may be NULL), then x = the sentinel */
 void bn_gs_prep(char vt, int n, void *x, void *xold, hipStream_t stream);
 /* y[0..n) = the walk's sentinel (a signalling-NaN bit pattern) */

@@ -944,7 +944,9 @@ __device__ __forceinline__ T gs_walk_sum(int r, int i, const int *__restrict__ i
       ia[r] + sub, ia[r + 1], jx, val,
       [&](int c, int &j) -> T {
         j = c & 0x7fffffff;
-        const bool early = (c < 0) != (backward != 0);
+        // the row's own entry (an operator with its diagonal) is never
+        // waited for: its product is dropped
+        const bool early = ((c < 0) != (backward != 0)) & (j != i);
         U u;
         if (ZERO) {
           // a later colour's input is zero: read x (the sentinel there
@@ -963,6 +965,49 @@ __device__ __forceinline__ T gs_walk_sum(int r, int i, const int *__restrict__ i
   return s;
 }
 
+// One row of a sweep in a walk: the sum from plain loads, then, while an
+// earlier row's value still reads the sentinel, the whole row again with
+// coherent loads; writes x_i (and zsum).  false when this thread or another
+// gave up (the row's lanes agree).
+template <typename T, int LANES, bool ZERO>
+__device__ __forceinline__ bool walk_row_solve(int r, int i, const int *__restrict__ ia, const int *__restrict__ jx,
+                                               const T *__restrict__ val, const T *__restrict__ dinv,
+                                               const T *__restrict__ b, T *x, const T *__restrict__ xold,
+                                               const T *__restrict__ zbase, T *__restrict__ zsum, int backward,
+                                               int sub, double *ctl, long long limit) {
+  typedef typename walk_word<T>::U U;
+  const T bi = b[i], dr = dinv[r];
+  bool bad;
+  T s = gs_walk_sum<T, LANES, ZERO, false>(r, i, ia, jx, val, x, xold, backward, sub, bad);
+  if (LANES > 1) s = group_sum<LANES>(s);
+  int nbad = group_sum<LANES>(bad ? 1 : 0);
+  long long t0 = 0;
+  for (int spin = 0; nbad; spin++) {
+    // every 16th retry: has another thread given up, and is the limit past
+    // (limit < 0 gives up at the first retry: the host forces the fallback)
+    if (limit < 0 || (spin & 15) == 15) {
+      const long long tc = (long long)wall_clock64();
+      if (spin == 15) t0 = tc;
+      if (walk_stopped(ctl) || limit < 0 || (spin > 15 && tc - t0 > limit)) {
+        if (!walk_stopped(ctl))
+          __hip_atomic_store((unsigned long long *)ctl, 0x3ff0000000000000ULL /* 1.0 */, __ATOMIC_RELAXED,
+                             __HIP_MEMORY_SCOPE_AGENT);
+        return false;
+      }
+    }
+    __builtin_amdgcn_s_sleep(1);
+    s = gs_walk_sum<T, LANES, ZERO, true>(r, i, ia, jx, val, x, xold, backward, sub, bad);
+    if (LANES > 1) s = group_sum<LANES>(s);
+    nbad = group_sum<LANES>(bad ? 1 : 0);
+  }
+  if (sub == 0) {
+    const T v = (bi - s) * dr;
+    __hip_atomic_store((U *)(x + i), __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (zsum) zsum[i] = zbase[i] + v;
+  }
+  return true;
+}
+
 template <typename T, int LANES, bool ZERO>
 __global__ __launch_bounds__(BN_GSW_BLOCK) void k_gs_walk(int ncol, int backward, const int2 *__restrict__ steps,
                                                           const int *__restrict__ ia, const int *__restrict__ jx,
@@ -970,7 +1015,6 @@ __global__ __launch_bounds__(BN_GSW_BLOCK) void k_gs_walk(int ncol, int backward
                                                           const T *__restrict__ dinvc, const T *__restrict__ b, T *x,
                                                           const T *__restrict__ xold, const T *__restrict__ zbase,
                                                           T *__restrict__ zsum, double *ctl, long long limit) {
-  typedef typename walk_word<T>::U U;
   constexpr int RPP = BN_GSW_BLOCK / LANES;
   const int p = blockIdx.x, sub = threadIdx.x % LANES, slot = threadIdx.x / LANES;
   bool live = !walk_stopped(ctl);
@@ -980,41 +1024,45 @@ __global__ __launch_bounds__(BN_GSW_BLOCK) void k_gs_walk(int ncol, int backward
     if (S.y == 0) continue;              // the same for the whole workgroup
     for (int k = slot; live && k < S.y; k += RPP) {
       const int r = S.x + k;
-      const int i = map[r];
-      const T bi = b[i], dr = dinvc[r];
-      bool bad;
-      T s = gs_walk_sum<T, LANES, ZERO, false>(r, i, ia, jx, val, x, xold, backward, sub, bad);
-      if (LANES > 1) s = group_sum<LANES>(s);
-      int nbad = group_sum<LANES>(bad ? 1 : 0);
-      long long t0 = 0;
-      for (int spin = 0; nbad; spin++) {
-        // an earlier colour's neighbour was not there yet: the whole row
-        // again with coherent loads, until none reads the sentinel
-        // every 16th retry: has another thread given up, and is the limit
-        // past (limit < 0 gives up at the first retry: the host forces the
-        // fallback)
-        if (limit < 0 || (spin & 15) == 15) {
-          const long long tc = (long long)wall_clock64();
-          if (spin == 15) t0 = tc;
-          if (walk_stopped(ctl) || limit < 0 || (spin > 15 && tc - t0 > limit)) {
-            if (!walk_stopped(ctl))
-              __hip_atomic_store((unsigned long long *)ctl, 0x3ff0000000000000ULL /* 1.0 */, __ATOMIC_RELAXED,
-                                 __HIP_MEMORY_SCOPE_AGENT);
-            live = false;
-            break;
-          }
-        }
-        __builtin_amdgcn_s_sleep(1);
-        s = gs_walk_sum<T, LANES, ZERO, true>(r, i, ia, jx, val, x, xold, backward, sub, bad);
-        if (LANES > 1) s = group_sum<LANES>(s);
-        nbad = group_sum<LANES>(bad ? 1 : 0);
-      }
-      if (!live) break;
-      if (sub == 0) {
-        const T v = (bi - s) * dr;
-        __hip_atomic_store((U *)(x + i), __builtin_bit_cast(U, v), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        if (zsum) zsum[i] = zbase[i] + v;
-      }
+      live = walk_row_solve<T, LANES, ZERO>(r, map[r], ia, jx, val, dinvc, b, x, xold, zbase, zsum, backward, sub,
+                                            ctl, limit);
+    }
+    __syncthreads();  // a failed thread still meets every barrier
+  }
+}
+
+// ---------------------------------------------------------------------
+// An exact-order Gauss-Seidel sweep (the sequential sweep's rows grouped by
+// dependency level) in ONE launch: the level operator's rows are cut into
+// contiguous parts, one workgroup each, and a part walks its rows of each
+// dependency level in turn (steps[sp[p] .. sp[p+1]): {first entry of the
+// level's row list, rows}), with a barrier between levels.  A row reads a
+// neighbour the sequential sweep visits earlier from x (its new value,
+// waited for while it holds the sentinel) and one it visits later from
+// xold, the sweep's input: bit 31 of the column word (jx) marks the
+// neighbours visited earlier in the forward order.  The sums are
+// k_rowsolve<T, LANES, RS_LIST>'s, to the last bit.
+// ---------------------------------------------------------------------
+template <typename T, int LANES>
+__global__ __launch_bounds__(BN_PART_BLOCK) void k_sweep_parts(int backward, const int *__restrict__ sp,
+                                                               const int2 *__restrict__ steps,
+                                                               const int *__restrict__ rows,
+                                                               const int *__restrict__ ia, const int *__restrict__ jx,
+                                                               const T *__restrict__ val, const T *__restrict__ dinv,
+                                                               const T *__restrict__ b, T *x,
+                                                               const T *__restrict__ xold, double *ctl,
+                                                               long long limit) {
+  constexpr int RPP = BN_PART_BLOCK / LANES;
+  const int p = blockIdx.x, sub = threadIdx.x % LANES, slot = threadIdx.x / LANES;
+  const int t0 = sp[p], t1 = sp[p + 1];
+  if (t0 >= t1) return;
+  bool live = !walk_stopped(ctl);
+  for (int t = t0; t < t1; t++) {
+    const int2 S = steps[t];
+    for (int k = slot; live && k < S.y; k += RPP) {
+      const int i = rows[S.x + k];
+      live = walk_row_solve<T, LANES, false>(i, i, ia, jx, val, dinv, b, x, xold, nullptr, nullptr, backward, sub,
+                                             ctl, limit);
     }
     __syncthreads();  // a failed thread still meets every barrier
   }
@@ -1829,6 +1877,54 @@ void bn_gs_walk(char vt, int lanes, int np, int ncol, int backward, const int *s
   }
 #undef BN_GWL
 #undef BN_GW
+}
+
+int bn_sweep_parts_per_cu(char vt) {
+  int best = 1 << 30;
+  for (int k = 0; k < 7; k++) {
+    int nb = 0;
+    hipError_t e = hipErrorInvalidValue;
+#define BN_OCC(T, L) e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, k_sweep_parts<T, L>, BN_PART_BLOCK, 0)
+#define BN_OCCL(T)                 \
+  switch (k) {                     \
+    case 0: BN_OCC(T, 1); break;   \
+    case 1: BN_OCC(T, 2); break;   \
+    case 2: BN_OCC(T, 4); break;   \
+    case 3: BN_OCC(T, 8); break;   \
+    case 4: BN_OCC(T, 16); break;  \
+    case 5: BN_OCC(T, 32); break;  \
+    default: BN_OCC(T, 64); break; \
+  }
+    if (vt == 's') { BN_OCCL(float) } else { BN_OCCL(double) }
+#undef BN_OCCL
+#undef BN_OCC
+    if (e != hipSuccess) return 0;
+    if (nb < best) best = nb;
+  }
+  return best;
+}
+
+void bn_sweep_parts(char vt, int lanes, int np, int backward, const int *sp, const int *steps, const int *rows,
+                    const int *ia, const int *jx, const void *val, const void *dinv, const void *b, void *x,
+                    const void *xold, double *ctl, long long limit, hipStream_t stream) {
+  if (np <= 0) return;
+  const int2 *S = (const int2 *)steps;
+#define BN_SP(T, L)                                                                                                \
+  BN_LAUNCH((k_sweep_parts<T, L>), dim3(np), dim3(BN_PART_BLOCK), 0, stream, backward, sp, S, rows, ia, jx,         \
+            (const T *)val, (const T *)dinv, (const T *)b, (T *)x, (const T *)xold, ctl, limit)
+#define BN_SPL(T)                    \
+  switch (lanes) {                   \
+    case 1: BN_SP(T, 1); break;      \
+    case 2: BN_SP(T, 2); break;      \
+    case 4: BN_SP(T, 4); break;      \
+    case 8: BN_SP(T, 8); break;      \
+    case 16: BN_SP(T, 16); break;    \
+    case 32: BN_SP(T, 32); break;    \
+    default: BN_SP(T, 64); break;    \
+  }
+  if (vt == 's') { BN_SPL(float) } else { BN_SPL(double) }
+#undef BN_SPL
+#undef BN_SP
 }
 
 void bn_gs_prep(char vt, int n, void *x, void *xold, hipStream_t stream) {

@@ -84,7 +84,7 @@ def test_ell_copies_follow_a_device_rebuild(monkeypatch, net):
 
 
 @pytest.mark.parametrize("mode", list(MODES))
-def test_walk_equals_a_launch_per_level(monkeypatch, net, mode):
+def test_walk_equals_a_launch_per_level(monkeypatch, capfd, net, mode):
     case(f"exact IC(0): one launch per run of levels vs one per level, {mode}, n={net.n}")
     out = []
     for walk in (True, False):
@@ -95,6 +95,7 @@ def test_walk_equals_a_launch_per_level(monkeypatch, net, mode):
         s.free()
     (xw, iw, xw2, iw2), (xl, il, xl2, il2) = out
     print(f"iterations walk {iw}/{iw2}, per level {il}/{il2}", flush=True)
+    assert "timed out" not in capfd.readouterr().err  # the walk itself ran, no hand-back
     assert iw > 0 and iw == il == iw2 == il2
     assert np.array_equal(xw, xl) and np.array_equal(xw2, xl2) and np.array_equal(xw, xw2)
 
