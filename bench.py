@@ -410,6 +410,44 @@ def networks_bench(args, count=64):
     print(json.dumps(out), flush=True)
 
 
+def plain_headline_bench(args):
+    """The plain FP64 GPU path alone on the headline network (4.36M
+    unknowns, HMIS): amg_solver_create's choices (multicolour C-then-F GS on
+    levels of <= 32 colours, exact IC(0) solved one launch per triangle,
+    dense coarsest inverse, fixed-beta CG; C_amg1/src/amg_pcg.c:12-101),
+    stopping at ||r||/||b|| < rtol.  A step is one solve from x0 = 0 with
+    every array resident in HBM; prints its own JSON line (profiling runs
+    of the drop-in path use it)."""
+    import numpy as np
+
+    from blood_network_amd import AMGSolver, WORKLOADS, make_network
+    from blood_network_amd import _lib as L
+
+    Ledge, depth, ntrees, coarsening = WORKLOADS[args.workload]
+    net = make_network(Ledge, depth, ntrees)
+    plain = AMGSolver(net.ia, net.ja, net.val, coarsening=coarsening, stop="relres")
+    d_b = L.DeviceArray(net.b)
+    d_x = L.DeviceArray(np.zeros(net.n))
+    zeros = np.zeros(net.n)
+    for _ in range(args.warmup):
+        d_x.upload(zeros)
+        plain.solve_device(d_b.ptr, d_x.ptr, tol=args.rtol, maxiter=1000)
+    ms = []
+    for k in range(args.steps):
+        d_x.upload(zeros)
+        it = plain.solve_device(d_b.ptr, d_x.ptr, tol=args.rtol, maxiter=1000)
+        ms.append(plain.report.solve_ms)
+        log(f"[plain] step {k}: {it} iterations, {ms[-1]:.2f} ms")
+    plain.free()
+    print(json.dumps({
+        "metric": "plain FP64 PCG solve time to 1e-8 rel. residual, 4M-node net (drop-in path)",
+        "value": round(float(np.median(ms)), 3), "unit": "ms", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "higher_is_better": False, "dtype": "fp64",
+        "data": "synthetic (deterministic generator, seed 0)",
+        "config": {"workload": args.workload, "n": net.n, "nnz": net.nnz, "coarsening": coarsening,
+                   "rtol": args.rtol, "iterations": it}}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -436,6 +474,9 @@ def main():
     ap.add_argument("--networks", action="store_true",
                     help="measure 64 uneven networks (lattice edge U{12..40}, tree depth U{6..10}) solved "
                          "in one segmented solve vs a loop of single solves (prints its own JSON line)")
+    ap.add_argument("--plain", action="store_true",
+                    help="time only the plain FP64 path (the drop-in's choices) on the headline network "
+                         "(prints its own JSON line)")
     ap.add_argument("--no-graph", action="store_true",
                     help="enqueue each PCG step kernel by kernel instead of replaying the captured "
                          "hipGraph (for kernel-tracing profilers)")
@@ -464,6 +505,8 @@ def main():
         return descent_bench(args)
     if args.workload == "lattice100_trees15":
         return plain_bench(args)
+    if args.plain:
+        return plain_headline_bench(args)
 
     Ledge, depth, ntrees, coarsening = WORKLOADS[args.workload]
     t0 = time.perf_counter()
