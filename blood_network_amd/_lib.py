@@ -208,7 +208,8 @@ def _load():
         "bn_chunk_order": (C.c_int, [C.POINTER(CSR), IP, C.c_int, C.c_int, IP, C.POINTER(IP), C.POINTER(IP),
                                      C.POINTER(IP)]),
         "bn_csr_apply_seg": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, C.c_int, C.c_int, C.c_int,
-                                     VP, VP, VP, C.c_int, C.c_int, VP, VP]),
+                                     VP, VP, VP, C.c_int, VP, VP, VP]),
+        "bn_seg_reset": (None, [C.c_int, VP, C.c_int, VP, VP]),
         "bn_csr_apply_blocks": (None, [C.c_char, C.c_int, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
         "bn_csr_resid_blocks_scaled": (None, [C.c_char, C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
         "bn_row_blocks": (C.c_int, [IP, C.c_int, C.c_int, C.c_int, IP]),

@@ -31,7 +31,9 @@ typedef struct {
   int *seg_row;    /* device: segment -> long row index                */
   int *seg_ptr_h;  /* host copy of seg_ptr                             */
   int nmulti_all;  /* long rows with more than one segment (a prefix)  */
-  double *seg_part;/* device: partial sums of multi-segment rows       */
+  double *seg_part;/* device: partial sums of multi-segment rows (the  */
+                   /* sentinel between launches)                       */
+  int *seg_cnt;    /* device: finished segments per multi-segment row  */
   int nblk;        /* row blocks for the adaptive kernel               */
   int *blk;        /* device: nblk + 1 row boundaries                  */
   int *blkq;       /* device: their entry offsets, blkq[k] = ia[blk[k]] */
@@ -200,6 +202,9 @@ struct bn_solver {
    * units (the most parts), wait limit in wall_clock64() ticks */
   int walk_ok, walk_on, walk_blocks;
   long long walk_limit;
+  int jac_fuse;         /* Jacobi-smoothed levels fuse their steps into the
+                           launches around them (vcycle.c; BN_FUSE_JACOBI=0
+                           keeps separate launches) */
   int coarse_dense;     /* coarsest solve: 1 dense inverse, 0 IC-PCG */
   double *cinv_d;       /* dense inverse of the coarsest operator */
   float *cinv_s;
@@ -280,26 +285,37 @@ static inline void bn_dcsr_apply(const bn_dcsr *m, char vt, int op, int lanes, c
                                  const void *b, hipStream_t st) {
   const int nl = bn_nlong(m, lanes);
   const int nseg = nl > 0 ? m->seg_ptr_h[nl] : 0;
-  const int nmulti = nl < m->nmulti_all ? nl : m->nmulti_all;
   bn_csr_apply_seg(vt, op, m->rows, m->ia, m->ja, vt == 's' ? (const void *)m->vs : (const void *)m->vd, x, y, b,
-                   lanes, bn_long_len(lanes), nl, m->long_rows, m->seg_ptr, m->seg_row, nseg, nmulti, m->seg_part,
-                   st);
+                   lanes, bn_long_len(lanes), nl, m->long_rows, m->seg_ptr, m->seg_row, nseg, m->seg_part,
+                   m->seg_cnt, st);
 }
 /* y = op(M, x [, b]) with M's tuned kernel: row blocks, or lanes per row
- * with the long-row bin */
-static inline void bn_op_apply(const bn_dcsr *m, char vt, int op, const void *x, void *y, const void *b,
-                               hipStream_t st) {
+ * with the long-row bin; with a Jacobi step fused in (xz, w, om as
+ * bn_csr_apply_seg_jac; op BN_OP_JAC) */
+static inline void bn_op_apply_jac(const bn_dcsr *m, char vt, int op, const void *x, void *y, const void *b,
+                                   void *xz, const void *w, double om, hipStream_t st) {
+  const void *val = vt == 's' ? (const void *)m->vs : (const void *)m->vd;
   if (vt == 's' ? m->adapt_s : m->adapt) {
-    bn_csr_apply_blocks(vt, op, m->nblk, m->blk, m->blkq, m->ia, m->ja, vt == 's' ? (const void *)m->vs : (const void *)m->vd,
-                        x, y, b, st);
+    bn_csr_apply_blocks_jac(vt, op, m->nblk, m->blk, m->blkq, m->ia, m->ja, val, x, y, b, xz, w, om, st);
     return;
   }
-  bn_dcsr_apply(m, vt, op, bn_spmv_lanes(m, vt), x, y, b, st);
+  const int lanes = bn_spmv_lanes(m, vt);
+  const int nl = bn_nlong(m, lanes);
+  const int nseg = nl > 0 ? m->seg_ptr_h[nl] : 0;
+  bn_csr_apply_seg_jac(vt, op, m->rows, m->ia, m->ja, val, x, y, b, lanes, bn_long_len(lanes), nl, m->long_rows,
+                       m->seg_ptr, m->seg_row, nseg, m->seg_part, m->seg_cnt, xz, w, om, st);
+}
+static inline void bn_op_apply(const bn_dcsr *m, char vt, int op, const void *x, void *y, const void *b,
+                               hipStream_t st) {
+  bn_op_apply_jac(m, vt, op, x, y, b, NULL, NULL, 0.0, st);
 }
 
 /* census.c: the V-cycle's and the outer PCG's bandwidth kernels, launched
  * on the solver's stream (recorded while a kernel census runs) */
 void bn_launch_apply(struct bn_solver *s, const bn_dcsr *M, char vt, int op, const void *x, void *y, const void *b);
+/* the same with a Jacobi step fused in (bn_op_apply_jac) */
+void bn_launch_apply_jac(struct bn_solver *s, const bn_dcsr *M, char vt, int op, const void *x, void *y,
+                         const void *b, void *xz, const void *w, double om);
 void bn_launch_resid_scaled(struct bn_solver *s, const bn_dcsr *M, char vt, const void *x, void *y, const void *b,
                             const void *dsc, void *ys);
 void bn_launch_gs(struct bn_solver *s, const bn_dlevel *d, char vt, int c, int lanes, const void *b, void *x,

@@ -33,6 +33,7 @@ typedef struct {
   char vt;
   const void *m;       /* bn_dcsr (K_APPLY, K_RESID_SCALED, K_JACOBI, K_SPMV) or bn_dlevel (K_GS) */
   const void *v[7];    /* the launch's vector arguments */
+  double om;           /* K_APPLY: the fused Jacobi step's damping */
 } ckey;
 
 typedef struct {
@@ -89,8 +90,10 @@ static void shape_describe(struct bn_solver *s, cshape *sh) {
        * the scaled residual dsc read and ys written */
       double b = (double)M->nnz * (4.0 + v) + ((double)M->rows + 1.0) * 4.0 + (double)M->cols * v +
                  (double)M->rows * v;
-      if (k->op == BN_OP_RESID || k->op == BN_OP_ADD) b += (double)M->rows * v;
+      if (k->op == BN_OP_RESID || k->op == BN_OP_ADD || k->op == BN_OP_JAC) b += (double)M->rows * v;
       if (k->kind == K_RESID_SCALED) b += 2.0 * M->rows * v;
+      /* a fused Jacobi step: its weights read, and for SET the step written */
+      if (k->kind == K_APPLY && k->v[4]) b += (double)M->rows * v * (k->op == BN_OP_SET ? 2.0 : 1.0);
       sh->bytes = b;
       break;
     }
@@ -179,15 +182,21 @@ static ckey mk(int kind, char vt, int op, const void *m) {
 /* ---------------------------------------------------------------------
  * launch wrappers
  * ------------------------------------------------------------------- */
-void bn_launch_apply(struct bn_solver *s, const bn_dcsr *M, char vt, int op, const void *x, void *y, const void *b) {
+void bn_launch_apply_jac(struct bn_solver *s, const bn_dcsr *M, char vt, int op, const void *x, void *y,
+                         const void *b, void *xz, const void *w, double om) {
   int ev = -1;
   if (s->census) {
     ckey k = mk(K_APPLY, vt, op, M);
-    k.v[0] = x, k.v[1] = y, k.v[2] = b;
+    k.v[0] = x, k.v[1] = y, k.v[2] = b, k.v[3] = xz, k.v[4] = w;
+    k.om = om;
     ev = census_begin(s, &k);
   }
-  bn_op_apply(M, vt, op, x, y, b, s->stream);
+  bn_op_apply_jac(M, vt, op, x, y, b, xz, w, om, s->stream);
   census_end(s, ev);
+}
+
+void bn_launch_apply(struct bn_solver *s, const bn_dcsr *M, char vt, int op, const void *x, void *y, const void *b) {
+  bn_launch_apply_jac(s, M, vt, op, x, y, b, NULL, NULL, 0.0);
 }
 
 void bn_launch_resid_scaled(struct bn_solver *s, const bn_dcsr *M, char vt, const void *x, void *y, const void *b,
@@ -259,7 +268,8 @@ void bn_launch_spmv(struct bn_solver *s, const bn_dcsr *A, int op, const void *x
 static void relaunch(struct bn_solver *s, const ckey *k) {
   switch (k->kind) {
     case K_APPLY:
-      bn_launch_apply(s, (const bn_dcsr *)k->m, k->vt, k->op, k->v[0], (void *)k->v[1], k->v[2]);
+      bn_launch_apply_jac(s, (const bn_dcsr *)k->m, k->vt, k->op, k->v[0], (void *)k->v[1], k->v[2],
+                          (void *)k->v[3], k->v[4], k->om);
       break;
     case K_RESID_SCALED:
       bn_launch_resid_scaled(s, (const bn_dcsr *)k->m, k->vt, k->v[0], (void *)k->v[1], k->v[2], k->v[3],

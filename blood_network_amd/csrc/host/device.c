@@ -103,6 +103,7 @@ static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
   d->long_rows = NULL;
   d->seg_ptr = d->seg_row = d->seg_ptr_h = NULL;
   d->seg_part = NULL;
+  d->seg_cnt = NULL;
   d->nmulti_all = 0;
   int nl = 0;
   for (int i = 0; i < m->row; i++) nl += (m->ia[i + 1] - m->ia[i] > bn_long_len(1));
@@ -135,7 +136,8 @@ static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
     d->seg_ptr = (int *)dmalloc(s, ((size_t)nl + 1) * sizeof(int));
     d->seg_row = (int *)dmalloc(s, (size_t)sp[nl] * sizeof(int));
     d->seg_part = (double *)dmalloc(s, (size_t)(sp[d->nmulti_all] > 0 ? sp[d->nmulti_all] : 1) * sizeof(double));
-    if (!d->long_rows || !d->seg_ptr || !d->seg_row || !d->seg_part) {
+    d->seg_cnt = (int *)dmalloc(s, (size_t)(d->nmulti_all > 0 ? d->nmulti_all : 1) * sizeof(int));
+    if (!d->long_rows || !d->seg_ptr || !d->seg_row || !d->seg_part || !d->seg_cnt) {
       free(lr);
       free(sp);
       free(sr);
@@ -144,6 +146,7 @@ static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
     CK(hipMemcpy(d->long_rows, lr, (size_t)nl * sizeof(int), hipMemcpyHostToDevice));
     CK(hipMemcpy(d->seg_ptr, sp, ((size_t)nl + 1) * sizeof(int), hipMemcpyHostToDevice));
     CK(hipMemcpy(d->seg_row, sr, (size_t)sp[nl] * sizeof(int), hipMemcpyHostToDevice));
+    bn_seg_reset(sp[d->nmulti_all], d->seg_part, d->nmulti_all, d->seg_cnt, s->stream);
     free(lr);
     free(sr);
     d->seg_ptr_h = sp; /* kept for the per-lanes segment counts */
@@ -1300,6 +1303,10 @@ int bn_device_upload(struct bn_solver *s) {
      * first wait, which exercises the re-run with a launch per level */
     const char *lenv = getenv("BN_TRI_WALK_LIMIT");
     if (lenv && lenv[0]) s->walk_limit = atoll(lenv);
+  }
+  {
+    const char *fenv = getenv("BN_FUSE_JACOBI");
+    s->jac_fuse = !(fenv && fenv[0] == '0');
   }
   s->vt = (s->opt.precision == BN_PREC_FP32) ? 's' : 'd';
   const amg_data *amg = s->amg;

@@ -1,8 +1,9 @@
 /*
  * schedule.c - host analysis that turns sequential sweeps into parallel
  * schedules for the device:
- *   - greedy multicolouring of the C set and of the F set of a level
- *     (C-then-F multicolour Gauss-Seidel), or of all points (plain GS);
+ *   - multicolouring of the C set and of the F set of a level (C-then-F
+ *     multicolour Gauss-Seidel), or of all points (plain GS): two colours
+ *     by breadth-first parity where the couplings allow, greedy otherwise;
  *   - level sets of a triangular factor (dependency depth per row) for
  *     level-scheduled triangular solves;
  *   - level sets of a sequential Gauss-Seidel sweep in a given visiting
@@ -45,6 +46,57 @@ static int greedy_class(const dcsr_mat *A, const int *cls, int want, int *col) {
   return ncol;
 }
 
+/* Two colours (one when no two members couple) by breadth-first parity
+ * when the couplings among the nodes with cls[i] == want have no odd
+ * cycle: components are searched from their lowest node, neighbours in
+ * row order.  A lattice with trees hung on it is such a graph, and its C
+ * and F sets are too, where the greedy pass spends extra colours on the
+ * rows it meets late.  Returns the colour count with col[i] set for the
+ * class members, or 0 (col untouched for them) when there is an odd
+ * cycle. */
+static int bipartite_class(const dcsr_mat *A, const int *cls, int want, int *col) {
+  const int n = A->row;
+  int *q = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  int *par = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
+  for (int i = 0; i < n; i++) par[i] = -1;
+  int ok = 1, ncol = 0;
+  for (int s = 0; s < n && ok; s++) {
+    if ((cls && cls[s] != want) || par[s] >= 0) continue;
+    int h = 0, t = 0;
+    q[t++] = s;
+    par[s] = 0;
+    if (ncol < 1) ncol = 1;
+    while (h < t && ok) {
+      const int i = q[h++];
+      for (int k = A->ia[i]; k < A->ia[i + 1]; k++) {
+        const int j = A->ja[k];
+        if (j == i || (cls && cls[j] != want)) continue;
+        if (par[j] < 0) {
+          par[j] = par[i] ^ 1;
+          ncol = 2;
+          q[t++] = j;
+        } else if (par[j] == par[i]) {
+          ok = 0;
+          break;
+        }
+      }
+    }
+  }
+  if (ok)
+    for (int i = 0; i < n; i++)
+      if (!cls || cls[i] == want) col[i] = par[i];
+  free(q);
+  free(par);
+  return ok ? ncol : 0;
+}
+
+/* the class's colours: two by parity when its couplings allow, else the
+ * greedy pass */
+static int colour_class(const dcsr_mat *A, const int *cls, int want, int *col) {
+  const int nc = bipartite_class(A, cls, want, col);
+  return nc > 0 ? nc : greedy_class(A, cls, want, col);
+}
+
 int bn_colour_order(const dcsr_mat *A, const int *mis, int split, int *perm, int **col_ptr) {
   const int n = A->row;
   int *col = (int *)bn_xmalloc((size_t)(n > 0 ? n : 1) * sizeof(int));
@@ -61,11 +113,11 @@ int bn_colour_order(const dcsr_mat *A, const int *mis, int split, int *perm, int
    * its own class): the F class as a task beside the C class */
   if (nclass == 2) {
 #pragma omp task shared(ncol_cls, col) firstprivate(A, cls) if (n > 65536)
-    ncol_cls[1] = greedy_class(A, cls, 1, col);
-    ncol_cls[0] = greedy_class(A, cls, 0, col);
+    ncol_cls[1] = colour_class(A, cls, 1, col);
+    ncol_cls[0] = colour_class(A, cls, 0, col);
 #pragma omp taskwait
   } else {
-    ncol_cls[0] = greedy_class(A, cls, 0, col);
+    ncol_cls[0] = colour_class(A, cls, 0, col);
   }
   const int ncol = ncol_cls[0] + ncol_cls[1];
   /* global colour id: C colours first, then F colours */

@@ -235,6 +235,45 @@ static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int bac
 /* x = A_c^{-1} b on the coarsest level */
 static void coarse_solve(struct bn_solver *s, const void *b, void *x);
 
+/* Level l is smoothed by smooth()'s Jacobi steps, and those steps are
+ * fused into the launches around them (jac_fuse): the first step from zero
+ * into the restriction that forms the level's b (its SET writes x = om w b
+ * beside b), each further step into one launch (BN_OP_JAC: the residual and
+ * the update that RESID + bn_jacobi_update form, into the other of x and
+ * the level's r, so the steps alternate and the last lands in x).  The
+ * values are those of the separate launches, to the last bit. */
+static int jacobi_fused(const struct bn_solver *s, int l) {
+  if (!s->jac_fuse || l <= 0 || l >= s->nlev - 1 || bn_vc_exact_kind(s) >= 0) return 0;
+  const bn_dlevel *d = &s->lev[l];
+  return !d->block && !(d->use_gs && s->opt.smoother == SMOOTHER_CF_GS);
+}
+
+static void jacobi_weights(const struct bn_solver *s, const bn_dlevel *d, const void **w, double *om) {
+  const int jac = (s->opt.smoother == SMOOTHER_JACOBI);
+  const char vt = s->vt;
+  if (jac) *w = vt == 's' ? (const void *)d->dinv_s : (const void *)d->dinv_d;
+  else *w = vt == 's' ? (const void *)d->l1inv_s : (const void *)d->l1inv_d;
+  *om = jac ? s->opt.omega : 1.0;
+}
+
+/* k fused Jacobi steps on level l starting from cur (x or the level's r),
+ * alternating between the two; returns where the last step wrote */
+static void *jacobi_steps(struct bn_solver *s, int l, const void *b, void *cur, void *x, int k) {
+  bn_dlevel *d = &s->lev[l];
+  const void *w;
+  double om;
+  jacobi_weights(s, d, &w, &om);
+  for (int i = 0; i < k; i++) {
+    void *out = cur == x ? d->r : x;
+    bn_launch_apply_jac(s, &d->A, s->vt, BN_OP_JAC, cur, out, b, NULL, w, om);
+    cur = out;
+  }
+  return cur;
+}
+
+/* x_zero: 1 x is zero on entry; 2 (a fused Jacobi level only) the first
+ * Jacobi step from zero is already in x, or in the level's r when the
+ * number of further steps is odd */
 void bn_vc_backslash(struct bn_solver *s, int lev, const void *b, void *x, int x_zero) {
   bn_dlevel *d = &s->lev[lev];
   if (lev == s->nlev - 1) {
@@ -242,10 +281,31 @@ void bn_vc_backslash(struct bn_solver *s, int lev, const void *b, void *x, int x
     return;
   }
   bn_dlevel *c = &s->lev[lev + 1];
-  smooth(s, lev, b, x, 0, x_zero, NULL, NULL);
+  const int nu = s->opt.nu > 0 ? s->opt.nu : 1;
+  if (x_zero && jacobi_fused(s, lev)) {
+    void *first = ((nu - 1) & 1) ? d->r : x;
+    if (x_zero != 2) {
+      const void *w;
+      double om;
+      jacobi_weights(s, d, &w, &om);
+      bn_jacobi_zero(s->vt, d->n, b, w, om, first, s->stream);
+    }
+    jacobi_steps(s, lev, b, first, x, nu - 1);
+  } else {
+    smooth(s, lev, b, x, 0, x_zero, NULL, NULL);
+  }
   apply(s, &d->A, BN_OP_RESID, x, d->r, b);
-  apply(s, &d->R, BN_OP_SET, d->r, c->b, NULL);
-  bn_vc_backslash(s, lev + 1, c->b, c->x, 1);
+  if (jacobi_fused(s, lev + 1)) {
+    /* the child's first Jacobi step from zero rides on its restriction */
+    const void *w;
+    double om;
+    jacobi_weights(s, c, &w, &om);
+    bn_launch_apply_jac(s, &d->R, s->vt, BN_OP_SET, d->r, c->b, NULL, ((nu - 1) & 1) ? c->r : c->x, w, om);
+    bn_vc_backslash(s, lev + 1, c->b, c->x, 2);
+  } else {
+    apply(s, &d->R, BN_OP_SET, d->r, c->b, NULL);
+    bn_vc_backslash(s, lev + 1, c->b, c->x, 1);
+  }
   apply(s, &d->P, BN_OP_ADD, c->x, x, NULL);
 }
 
@@ -268,6 +328,14 @@ static void fwdslash_rec(struct bn_solver *s, int lev, const void *b, void *x, i
     apply(s, &d->R, BN_OP_SET, d->r, c->b, NULL);
   }
   fwdslash_rec(s, lev + 1, c->b, c->x, 1, NULL, NULL);
+  if (x_zero && !zsum && jacobi_fused(s, lev)) {
+    /* interpolate into the buffer the nu fused steps start from */
+    const int nu = s->opt.nu > 0 ? s->opt.nu : 1;
+    void *first = (nu & 1) ? d->r : x;
+    apply(s, &d->P, BN_OP_SET, c->x, first, NULL);
+    jacobi_steps(s, lev, b, first, x, nu);
+    return;
+  }
   if (d->block && x_zero) {
     /* interpolate straight into the buffer the first sweep reads */
     void *in = block_input(d, s->opt.nu > 0 ? s->opt.nu : 1, x);

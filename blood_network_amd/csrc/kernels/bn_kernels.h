@@ -77,7 +77,8 @@ void bn_time_next_launch(hipEvent_t start, hipEvent_t stop);
 enum {
   BN_OP_SET = 0,    /* y = A x                 */
   BN_OP_RESID = 1,  /* y = b - A x             */
-  BN_OP_ADD = 2     /* y = y + A x             */
+  BN_OP_ADD = 2,    /* y = y + A x             */
+  BN_OP_JAC = 3     /* y = x + om w (b - A x): one Jacobi step (the *_jac entry points) */
 };
 
 /* y = op(A, x [, b]); lanes = threads per row (1..64, power of two).
@@ -98,16 +99,31 @@ void bn_csr_apply_binned(char vt, int op, int n, const int *ia, const int *ja,
 
 /* Same split, long rows cut into segments of BN_SEG_NNZ entries, one
  * block per segment: long row t = long_rows[t] owns segments
- * seg_ptr[t] .. seg_ptr[t+1]-1 (seg_row[s] = t); the first nmulti long
- * rows have more than one segment and their partial sums go through
- * part[0 .. seg_ptr[nmulti]) before a fixed-order finish.  nseg =
- * seg_ptr[nlong]. */
+ * seg_ptr[t] .. seg_ptr[t+1]-1 (seg_row[s] = t); nseg = seg_ptr[nlong].
+ * The rows with more than one segment (a prefix of long_rows) pass their
+ * partial sums through part[seg_ptr[t] ..] and count their segments in
+ * cnt[t]; the last segment of a row to finish adds the partials in
+ * segment order.  part's slots must hold the walk's sentinel and cnt
+ * zeros (bn_seg_reset) before the first call; every call leaves them so.
+ * Calls that share part and cnt must not overlap. */
 #define BN_SEG_NNZ 1024
 void bn_csr_apply_seg(char vt, int op, int n, const int *ia, const int *ja,
                       const void *val, const void *x, void *y, const void *b,
                       int lanes, int long_len, int nlong, const int *long_rows,
-                      const int *seg_ptr, const int *seg_row, int nseg, int nmulti,
-                      double *part, hipStream_t stream);
+                      const int *seg_ptr, const int *seg_row, int nseg,
+                      double *part, int *cnt, hipStream_t stream);
+/* The same with a Jacobi step fused in (op BN_OP_SET with xz != NULL: also
+ * xz_r = om y_r w_r, bn_jacobi_zero's x for right-hand side y; op
+ * BN_OP_JAC: y_r = x_r + om (b_r - (A x)_r) w_r, what BN_OP_RESID into r
+ * followed by bn_jacobi_update(r, w, om, x) leave in x, to the last bit;
+ * y must not be x). */
+void bn_csr_apply_seg_jac(char vt, int op, int n, const int *ia, const int *ja,
+                          const void *val, const void *x, void *y, const void *b,
+                          int lanes, int long_len, int nlong, const int *long_rows,
+                          const int *seg_ptr, const int *seg_row, int nseg,
+                          double *part, int *cnt, void *xz, const void *w, double om, hipStream_t stream);
+/* part[0..nslot) = the sentinel, cnt[0..nmulti) = 0 */
+void bn_seg_reset(int nslot, double *part, int nmulti, int *cnt, hipStream_t stream);
 
 /* Row-block split for bn_csr_apply_blocks: consecutive rows grouped so a
  * block holds at most BN_ROWBLOCK_NNZ entries and BN_ROWBLOCK_ROWS rows; a
@@ -126,6 +142,11 @@ void bn_csr_apply_blocks(char vt, int op, int nblk, const int *blocks, const int
                          const int *ia, const int *ja, const void *val,
                          const void *x, void *y, const void *b,
                          hipStream_t stream);
+/* The same with a Jacobi step fused in, as bn_csr_apply_seg_jac. */
+void bn_csr_apply_blocks_jac(char vt, int op, int nblk, const int *blocks, const int *blkq,
+                             const int *ia, const int *ja, const void *val,
+                             const void *x, void *y, const void *b, void *xz, const void *w, double om,
+                             hipStream_t stream);
 /* y = b - A x by the row-block kernel, also writing ys_r = y_r dsc_r */
 void bn_csr_resid_blocks_scaled(char vt, int nblk, const int *blocks, const int *blkq, const int *ia,
                                 const int *ja, const void *val, const void *x, void *y, const void *b,

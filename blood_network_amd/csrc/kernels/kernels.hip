@@ -67,6 +67,19 @@ __device__ __forceinline__ double block_sum(double v) {
   return t;
 }
 
+// The sentinel of values not written yet (the one-launch walks' y, the
+// long-row segments' partial sums): a signalling-NaN bit pattern, which
+// arithmetic never produces (it only ever produces quiet NaNs).
+template <typename T> struct walk_word;
+template <> struct walk_word<double> {
+  typedef unsigned long long U;
+  static constexpr U none = 0x7ff4dead0000beefULL;
+};
+template <> struct walk_word<float> {
+  typedef unsigned int U;
+  static constexpr U none = 0x7fa0beefU;
+};
+
 // Partial-sum buffers always hold BN_RED_BLOCKS slots; the finishing
 // kernel sums all of them, so logical block 0 zeroes the slots past the
 // grid (instead of a memset node, which is unsafe inside captured graphs).
@@ -150,9 +163,10 @@ __device__ __forceinline__ T row_dot(int q, const int e, const int *__restrict__
 // Block-wide sum over one long row: each thread keeps 16 strides of
 // loads in flight per round, so a row of tens of thousands of entries
 // takes a few memory round trips instead of one per 4 strides.
-template <typename T>
-__device__ __forceinline__ T row_dot_deep(int q, const int e, const int *__restrict__ ja,
-                                          const T *__restrict__ val, const T *__restrict__ x) {
+// `gather(c)` returns x at column c.
+template <typename T, typename G>
+__device__ __forceinline__ T row_dot_deep_g(int q, const int e, const int *__restrict__ ja,
+                                            const T *__restrict__ val, G gather) {
   constexpr int U = 16;
   T acc[4] = {0, 0, 0, 0};
   for (; q + (U - 1) * BLOCK < e; q += U * BLOCK) {
@@ -164,7 +178,7 @@ __device__ __forceinline__ T row_dot_deep(int q, const int e, const int *__restr
       a[u] = val[q + u * BLOCK];
     }
 #pragma unroll
-    for (int u = 0; u < U; u++) acc[u & 3] += a[u] * x[j[u]];
+    for (int u = 0; u < U; u++) acc[u & 3] += a[u] * gather(j[u]);
   }
   if (q < e) {
     // at most U - 1 strides left: one round, padding slots re-read the
@@ -180,10 +194,16 @@ __device__ __forceinline__ T row_dot_deep(int q, const int e, const int *__restr
 #pragma unroll
     for (int u = 0; u < U - 1; u++) {
       const T hm = q + u * BLOCK < e ? T(1) : T(0);
-      acc[u & 3] += (a[u] * hm) * x[j[u]];
+      acc[u & 3] += (a[u] * hm) * gather(j[u]);
     }
   }
   return (acc[0] + acc[1]) + (acc[2] + acc[3]);
+}
+
+template <typename T>
+__device__ __forceinline__ T row_dot_deep(int q, const int e, const int *__restrict__ ja,
+                                          const T *__restrict__ val, const T *__restrict__ x) {
+  return row_dot_deep_g<T>(q, e, ja, val, [x](int c) { return x[c]; });
 }
 
 // Grid for one-row-block-per-block kernels: rounded up to a multiple of
@@ -202,14 +222,48 @@ inline int grid_for(long long nrb) {
 }
 
 // ---------------------------------------------------------------------
-// CSR apply: y = A x | y = b - A x | y += A x, optional fused w.y dot
+// The Jacobi work a CSR launch can do on the rows it finishes, so that a
+// Jacobi-smoothed level needs no launch of its own for it:
+//  * SET with xz: also xz_r = om y_r w_r, the first Jacobi step from zero
+//    on the result (k_jacobi_zero's product), written beside y;
+//  * JAC: y_r = x_r + om (b_r - (A x)_r) w_r, one Jacobi step - the
+//    residual rounded as the RESID launch stores it, then
+//    k_jacobi_update's update - into y, which must not be x.
+// ---------------------------------------------------------------------
+template <typename T>
+struct jacobi_epi {
+  T *xz;
+  const T *w;
+  T om;
+};
+
+template <typename T, int OP>
+__device__ __forceinline__ T csr_finish(T s, T base, const T *__restrict__ x, int row, const jacobi_epi<T> &e) {
+  if (OP == BN_OP_SET) return s;
+  if (OP == BN_OP_RESID) return base - s;
+  if (OP == BN_OP_JAC) {
+    const T r = base - s;
+    return x[row] + e.om * r * e.w[row];
+  }
+  return base + s;
+}
+
+template <typename T, int OP>
+__device__ __forceinline__ void csr_finish_store(T *__restrict__ y, int row, T out, const jacobi_epi<T> &e) {
+  y[row] = out;
+  if (OP == BN_OP_SET && e.xz) e.xz[row] = e.om * out * e.w[row];
+}
+
+// ---------------------------------------------------------------------
+// CSR apply: y = A x | y = b - A x | y += A x | a Jacobi step (JAC),
+// optional fused w.y dot
 // ---------------------------------------------------------------------
 template <typename T, int LANES, int OP, bool DOT, int R = (LANES == 2 ? 8 : 4)>
 __global__ __launch_bounds__(BLOCK) void k_csr(int n, int chunk, const int *__restrict__ ia,
                                                const int *__restrict__ ja, const T *__restrict__ val,
                                                const T *__restrict__ x, T *__restrict__ y,
                                                const T *__restrict__ b, const T *__restrict__ w,
-                                               double *__restrict__ partial, int long_len) {
+                                               double *__restrict__ partial, int long_len, jacobi_epi<T> e) {
   constexpr int RPB = BLOCK / LANES;
   const int lb = logical_block();
   const int sub = threadIdx.x % LANES;
@@ -224,15 +278,12 @@ __global__ __launch_bounds__(BLOCK) void k_csr(int n, int chunk, const int *__re
       const int q0 = ia[row], q1 = ia[row + 1];
       if (q1 - q0 > long_len) continue; /* handled by k_csr_long */
       // b / y ahead of the row's gathers: one memory round trip fewer
-      const T base = OP == BN_OP_RESID ? b[row] : OP == BN_OP_ADD ? y[row] : T(0);
+      const T base = (OP == BN_OP_RESID || OP == BN_OP_JAC) ? b[row] : OP == BN_OP_ADD ? y[row] : T(0);
       T s = row_dot<T, LANES, false, R>(q0 + sub, q1, ja, val, x, row);
       if (LANES > 1) s = group_sum<LANES>(s);
       if (sub == 0) {
-        T out;
-        if (OP == BN_OP_SET) out = s;
-        else if (OP == BN_OP_RESID) out = base - s;
-        else out = base + s;
-        y[row] = out;
+        const T out = csr_finish<T, OP>(s, base, x, row, e);
+        csr_finish_store<T, OP>(y, row, out, e);
         if (DOT) acc += (double)w[row] * (double)out;
       }
     }
@@ -246,11 +297,12 @@ __global__ __launch_bounds__(BLOCK) void k_csr(int n, int chunk, const int *__re
 
 template <typename T, int LANES, int R = (LANES == 2 ? 8 : 4)>
 void launch_csr(int op, int n, const int *ia, const int *ja, const T *val, const T *x, T *y,
-                const T *b, const T *w, double *partial, int long_len, hipStream_t st) {
+                const T *b, const T *w, double *partial, int long_len, jacobi_epi<T> e, hipStream_t st) {
   constexpr int RPB = BLOCK / LANES;
   const long long nrb = ((long long)n + RPB - 1) / RPB;
 #define BN_CSR_LAUNCH(OPV, DOTV, G, CH) \
-  BN_LAUNCH((k_csr<T, LANES, OPV, DOTV, R>), dim3(G), dim3(BLOCK), 0, st, n, CH, ia, ja, val, x, y, b, w, partial, long_len)
+  BN_LAUNCH((k_csr<T, LANES, OPV, DOTV, R>), dim3(G), dim3(BLOCK), 0, st, n, CH, ia, ja, val, x, y, b, w, partial, \
+            long_len, e)
   // fused dot: a capped grid so the partials fit BN_RED_BLOCKS slots.
   // Past that many row blocks each block would walk several of them in
   // turn, which measured slower than the one-row-block-per-block launch
@@ -267,6 +319,7 @@ void launch_csr(int op, int n, const int *ia, const int *ja, const T *val, const
     const int g = grid_rows(nrb);
     if (op == BN_OP_SET) BN_CSR_LAUNCH(BN_OP_SET, false, g, 1);
     else if (op == BN_OP_RESID) BN_CSR_LAUNCH(BN_OP_RESID, false, g, 1);
+    else if (op == BN_OP_JAC) BN_CSR_LAUNCH(BN_OP_JAC, false, g, 1);
     else BN_CSR_LAUNCH(BN_OP_ADD, false, g, 1);
     if (split) bn_dot_partial(n, (const double *)w, (const double *)y, NULL, NULL, partial, NULL, st);
   }
@@ -276,39 +329,40 @@ void launch_csr(int op, int n, const int *ia, const int *ja, const T *val, const
 // lanes = 1..64, optionally with a BN_LANES_DEEP* bit
 template <typename T, int LANES>
 void launch_csr_rounds(int deep, int op, int n, const int *ia, const int *ja, const T *val, const T *x, T *y,
-                       const T *b, const T *w, double *partial, int long_len, hipStream_t st) {
+                       const T *b, const T *w, double *partial, int long_len, jacobi_epi<T> e, hipStream_t st) {
   if (deep == BN_LANES_DEEP16)
-    launch_csr<T, LANES, 16>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st);
+    launch_csr<T, LANES, 16>(op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st);
   else if (deep == BN_LANES_DEEP8)
-    launch_csr<T, LANES, 8>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st);
+    launch_csr<T, LANES, 8>(op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st);
   else
-    launch_csr<T, LANES>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st);
+    launch_csr<T, LANES>(op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st);
 }
 
 template <typename T>
 void dispatch_csr(int op, int n, const int *ia, const int *ja, const T *val, const T *x, T *y,
-                  const T *b, int lanes_in, const T *w, double *partial, int long_len, hipStream_t st) {
+                  const T *b, int lanes_in, const T *w, double *partial, int long_len, hipStream_t st,
+                  jacobi_epi<T> e = jacobi_epi<T>{nullptr, nullptr, T(0)}) {
   const int deep = lanes_in & (BN_LANES_DEEP8 | BN_LANES_DEEP16);
   const int lanes = lanes_in & BN_LANES_MASK;
   if (deep) {
     switch (lanes) {
-      case 1: launch_csr_rounds<T, 1>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, st); return;
-      case 2: launch_csr_rounds<T, 2>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, st); return;
-      case 4: launch_csr_rounds<T, 4>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, st); return;
-      case 8: launch_csr_rounds<T, 8>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, st); return;
-      case 16: launch_csr_rounds<T, 16>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, st); return;
-      case 32: launch_csr_rounds<T, 32>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, st); return;
-      default: launch_csr_rounds<T, 64>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, st); return;
+      case 1: launch_csr_rounds<T, 1>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st); return;
+      case 2: launch_csr_rounds<T, 2>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st); return;
+      case 4: launch_csr_rounds<T, 4>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st); return;
+      case 8: launch_csr_rounds<T, 8>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st); return;
+      case 16: launch_csr_rounds<T, 16>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st); return;
+      case 32: launch_csr_rounds<T, 32>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st); return;
+      default: launch_csr_rounds<T, 64>(deep, op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st); return;
     }
   }
   switch (lanes) {
-    case 1: launch_csr<T, 1>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
-    case 2: launch_csr<T, 2>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
-    case 4: launch_csr<T, 4>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
-    case 8: launch_csr<T, 8>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
-    case 16: launch_csr<T, 16>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
-    case 32: launch_csr<T, 32>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
-    default: launch_csr<T, 64>(op, n, ia, ja, val, x, y, b, w, partial, long_len, st); break;
+    case 1: launch_csr<T, 1>(op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st); break;
+    case 2: launch_csr<T, 2>(op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st); break;
+    case 4: launch_csr<T, 4>(op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st); break;
+    case 8: launch_csr<T, 8>(op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st); break;
+    case 16: launch_csr<T, 16>(op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st); break;
+    case 32: launch_csr<T, 32>(op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st); break;
+    default: launch_csr<T, 64>(op, n, ia, ja, val, x, y, b, w, partial, long_len, e, st); break;
   }
 }
 
@@ -332,14 +386,23 @@ __global__ __launch_bounds__(BLOCK) void k_csr_long(const int *__restrict__ rows
 // Segmented long-row bin: long row t (rows[t], longest first) is cut
 // into segments of BN_SEG_NNZ entries, segments seg_ptr[t] ..
 // seg_ptr[t+1]-1, one block each.  A one-segment row is finished by its
-// block; a longer row's block writes its partial sum and k_seg_finish adds
-// the partials in segment order (deterministic, no atomics).
+// block.  A longer row's blocks publish their partial sums (agent-scope
+// stores into part, which holds the sentinel between launches) and count
+// themselves in cnt[t]; the block that completes the count adds the
+// partials in segment order - the fixed order, whichever block it is -
+// finishes the row, and puts back the sentinel and the zero count.  A
+// partial is read with agent-scope loads until it is no longer the
+// sentinel: its block stored it before counting itself, so the wait ends
+// at once, and it is bounded (BN_SEG_SPINS polls) in any case.
+#define BN_SEG_SPINS 4096
 template <typename T, int OP>
 __global__ __launch_bounds__(BLOCK) void k_csr_seg(const int *__restrict__ seg_row, const int *__restrict__ seg_ptr,
                                                    const int *__restrict__ rows, const int *__restrict__ ia,
                                                    const int *__restrict__ ja, const T *__restrict__ val,
                                                    const T *__restrict__ x, T *__restrict__ y,
-                                                   const T *__restrict__ b, double *__restrict__ part) {
+                                                   const T *__restrict__ b, double *part, int *cnt,
+                                                   jacobi_epi<T> e) {
+  typedef unsigned long long U;
   const int sg = blockIdx.x;
   const int t = seg_row[sg];
   const int row = rows[t];
@@ -347,45 +410,56 @@ __global__ __launch_bounds__(BLOCK) void k_csr_seg(const int *__restrict__ seg_r
   const int beg = ia[row] + (sg - s0) * BN_SEG_NNZ;
   const int end = min(beg + BN_SEG_NNZ, ia[row + 1]);
   const T v = row_dot<T, BLOCK, false>(beg + threadIdx.x, end, ja, val, x, row);
-  const double tsum = block_sum((double)v);
-  if (threadIdx.x != 0) return;
+  double tsum = block_sum((double)v);
   if (s1 - s0 > 1) {
-    part[sg] = tsum;
-    return;
+    __shared__ int last;
+    __shared__ double ps[BLOCK];
+    if (threadIdx.x == 0) {
+      __hip_atomic_store((U *)(part + sg), __builtin_bit_cast(U, tsum), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");  // the store completes before the count
+      last = __hip_atomic_fetch_add(cnt + t, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == s1 - s0 - 1;
+    }
+    __syncthreads();
+    if (!last) return;
+    // the row's partials, BLOCK at a time loaded together, added in order
+    tsum = 0.0;
+    for (int k0 = s0; k0 < s1; k0 += BLOCK) {
+      const int k = k0 + threadIdx.x;
+      if (k < s1) {
+        U u = __hip_atomic_load((U *)(part + k), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        for (int spin = 0; u == walk_word<double>::none && spin < BN_SEG_SPINS; spin++) {
+          __builtin_amdgcn_s_sleep(1);
+          u = __hip_atomic_load((U *)(part + k), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+        ps[threadIdx.x] = __builtin_bit_cast(double, u);
+        __hip_atomic_store((U *)(part + k), walk_word<double>::none, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      }
+      __syncthreads();
+      if (threadIdx.x == 0)
+        for (int q = 0; q < BLOCK && k0 + q < s1; q++) tsum += ps[q];
+      __syncthreads();
+    }
+    if (threadIdx.x == 0) __hip_atomic_store(cnt + t, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
   }
-  if (OP == BN_OP_SET) y[row] = (T)tsum;
-  else if (OP == BN_OP_RESID) y[row] = b[row] - (T)tsum;
-  else y[row] = y[row] + (T)tsum;
-}
-
-template <typename T, int OP>
-__global__ void k_seg_finish(int nmulti, const int *__restrict__ seg_ptr, const int *__restrict__ rows,
-                             const double *__restrict__ part, T *__restrict__ y, const T *__restrict__ b) {
-  const int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nmulti) return;
-  double acc = 0.0;
-  for (int k = seg_ptr[t]; k < seg_ptr[t + 1]; k++) acc += part[k];
-  const int row = rows[t];
-  if (OP == BN_OP_SET) y[row] = (T)acc;
-  else if (OP == BN_OP_RESID) y[row] = b[row] - (T)acc;
-  else y[row] = y[row] + (T)acc;
+  if (threadIdx.x != 0) return;
+  const T base = (OP == BN_OP_RESID || OP == BN_OP_JAC) ? b[row] : OP == BN_OP_ADD ? y[row] : T(0);
+  csr_finish_store<T, OP>(y, row, csr_finish<T, OP>((T)tsum, base, x, row, e), e);
 }
 
 template <typename T>
-void launch_seg(int op, int nlong, int nmulti, const int *rows, const int *seg_ptr, const int *seg_row, int nseg,
-                double *part, const int *ia, const int *ja, const T *val, const T *x, T *y, const T *b,
-                hipStream_t st) {
+void launch_seg(int op, int nlong, const int *rows, const int *seg_ptr, const int *seg_row, int nseg,
+                double *part, int *cnt, const int *ia, const int *ja, const T *val, const T *x, T *y, const T *b,
+                jacobi_epi<T> e, hipStream_t st) {
   if (nlong <= 0 || nseg <= 0) return;
-  const dim3 gf((unsigned)((nmulti + BLOCK - 1) / BLOCK));
 #define BN_SEG(OPV)                                                                                          \
   BN_LAUNCH((k_csr_seg<T, OPV>), dim3(nseg), dim3(BLOCK), 0, st, seg_row, seg_ptr, rows, ia, ja, val, x, \
-                     y, b, part);                                                                             \
-  if (nmulti > 0)                                                                                            \
-    BN_LAUNCH((k_seg_finish<T, OPV>), gf, dim3(BLOCK), 0, st, nmulti, seg_ptr, rows, part, y, b)
+                     y, b, part, cnt, e)
   if (op == BN_OP_SET) {
     BN_SEG(BN_OP_SET);
   } else if (op == BN_OP_RESID) {
     BN_SEG(BN_OP_RESID);
+  } else if (op == BN_OP_JAC) {
+    BN_SEG(BN_OP_JAC);
   } else {
     BN_SEG(BN_OP_ADD);
   }
@@ -411,46 +485,25 @@ void launch_long(int op, int nlong, const int *rows, const int *ia, const int *j
 // fully coalesced segment, stages val * x[ja] in LDS, then sums each row
 // with tpr = pow2 <= 256 / rows threads.  Fixed summation order per row.
 // ---------------------------------------------------------------------
-// SCALED (RESID only): also write ys_r = y_r dsc_r, the first Jacobi step
-// from zero of a triangle whose right-hand side is this residual (the
-// product k_diag_scale would form)
-//
-// Latency: a block's work is three dependent memory round trips.  (1) The
-// block's row bounds and, from blkq (the entry offsets ia[blocks[k]],
-// stored beside them), its entry bounds; (2) the entries' columns and
-// values, and - issued with them - the bounds of the row this thread
-// reduces and its b (or y) and dsc; (3) the x gathers.  Without blkq the
-// entry bounds come from ia, one round trip more.  The reduction reads
-// LDS only.
-template <typename T, int OP, bool SCALED = false>
-__global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__restrict__ blocks,
-                                                      const int *__restrict__ blkq,
-                                                      const int *__restrict__ ia, const int *__restrict__ ja,
-                                                      const T *__restrict__ val, const T *__restrict__ x,
-                                                      T *__restrict__ y, const T *__restrict__ b,
-                                                      const T *__restrict__ dsc = nullptr,
-                                                      T *__restrict__ ys = nullptr) {
+// The block's work over an access policy `io`: io.gx(j) gathers x_j,
+// io.base(r) is the value the row's sum is combined with (b_r for RESID,
+// y_r for ADD), io.dsc(r) the SCALED factor and io.out(r, s, base, dsc)
+// forms and stores the result.  prod: BN_ROWBLOCK_NNZ + 1 words of LDS.
+template <typename T, int OP, typename IO>
+__device__ __forceinline__ void csr_block(int bid, const int *__restrict__ blocks, const int *__restrict__ blkq,
+                                          const int *__restrict__ ia, const int *__restrict__ ja,
+                                          const T *__restrict__ val, const IO &io, T *prod) {
   constexpr int PER = BN_ROWBLOCK_NNZ / BLOCK;
-  __shared__ T prod[BN_ROWBLOCK_NNZ + 1];  // + a spare word padding slots write
-  const int bid = logical_block();
-  if (bid >= nblk) return;
   const int r0 = blocks[bid], r1 = blocks[bid + 1];
   const int q0 = blkq ? blkq[bid] : ia[r0];
   const int q1 = blkq ? blkq[bid + 1] : ia[r1];
   const int cnt = q1 - q0;
   if (cnt > BN_ROWBLOCK_NNZ) {
     // vector mode: one long row (r1 == r0 + 1)
-    const T base = OP == BN_OP_RESID ? b[r0] : OP == BN_OP_ADD ? y[r0] : T(0);
-    const T s = row_dot_deep<T>(q0 + threadIdx.x, q1, ja, val, x);
+    const T base = io.base(r0);
+    const T s = row_dot_deep_g<T>(q0 + threadIdx.x, q1, ja, val, [&io](int c) { return io.gx(c); });
     const double t = block_sum((double)s);
-    if (threadIdx.x == 0) {
-      if (OP == BN_OP_SET) y[r0] = (T)t;
-      else if (OP == BN_OP_RESID) {
-        const T out = base - (T)t;
-        y[r0] = out;
-        if (SCALED) ys[r0] = out * dsc[r0];
-      } else y[r0] = base + (T)t;
-    }
+    if (threadIdx.x == 0) io.out(r0, (T)t, base, io.dsc(r0));
     return;
   }
   // tpr = largest power of two <= 256 / rows, at most one wave; with at
@@ -465,8 +518,8 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__res
   const int rf = r0 + (threadIdx.x >> lg);
   const int rc = rf < r1 ? rf : r1 - 1;
   const int e0 = ia[rc] - q0, e1 = ia[rc + 1] - q0;
-  const T base0 = OP == BN_OP_RESID ? b[rc] : OP == BN_OP_ADD ? y[rc] : T(0);
-  const T dsc0 = SCALED ? dsc[rc] : T(0);
+  const T base0 = io.base(rc);
+  const T dsc0 = io.dsc(rc);
   if (cnt > 0) {
     // every slot loads (past the block's entries: its last entry, with a
     // zero coefficient), so all loads and gathers are in flight together
@@ -485,7 +538,7 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__res
       // read) keeps every gather out of a branch
       const int k = threadIdx.x + u * BLOCK;
       const T hm = k < cnt ? T(1) : T(0);
-      prod[k < cnt ? k : BN_ROWBLOCK_NNZ] = (aa[u] * hm) * x[jj[u]];
+      prod[k < cnt ? k : BN_ROWBLOCK_NNZ] = (aa[u] * hm) * io.gx(jj[u]);
     }
   }
   __syncthreads();
@@ -501,34 +554,84 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__res
     if (r < r1)
       for (int k = ks + sub; k < ke; k += tpr) s += prod[k];
     for (int o = tpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (sub == 0 && r < r1) {
-      const T base = first ? base0 : OP == BN_OP_RESID ? b[r] : OP == BN_OP_ADD ? y[r] : T(0);
-      if (OP == BN_OP_SET) y[r] = s;
-      else if (OP == BN_OP_RESID) {
-        const T out = base - s;
-        y[r] = out;
-        if (SCALED) ys[r] = out * (first ? dsc0 : dsc[r]);
-      } else y[r] = base + s;
+    if (sub == 0 && r < r1) io.out(r, s, first ? base0 : io.base(r), first ? dsc0 : io.dsc(r));
+  }
+}
+
+// plain loads and stores: y = A x | y = b - A x | y += A x, and with
+// SCALED (RESID only) also ys_r = y_r dsc_r, the first Jacobi step from
+// zero of a triangle whose right-hand side is this residual (the product
+// k_diag_scale would form)
+template <typename T, int OP, bool SCALED>
+struct blocks_io {
+  const T *__restrict__ x;
+  T *__restrict__ y;
+  const T *__restrict__ b;
+  const T *__restrict__ dsc_;
+  T *__restrict__ ys;
+  jacobi_epi<T> e;
+  __device__ __forceinline__ T gx(int j) const { return x[j]; }
+  __device__ __forceinline__ T base(int r) const {
+    return (OP == BN_OP_RESID || OP == BN_OP_JAC) ? b[r] : OP == BN_OP_ADD ? y[r] : T(0);
+  }
+  __device__ __forceinline__ T dsc(int r) const { return SCALED ? dsc_[r] : T(0); }
+  __device__ __forceinline__ void out(int r, T s, T base, T d) const {
+    if (OP == BN_OP_SET) {
+      csr_finish_store<T, OP>(y, r, s, e);
+    } else if (OP == BN_OP_RESID) {
+      const T o = base - s;
+      y[r] = o;
+      if (SCALED) ys[r] = o * d;
+    } else if (OP == BN_OP_JAC) {
+      y[r] = csr_finish<T, OP>(s, base, x, r, e);
+    } else {
+      y[r] = base + s;
     }
   }
+};
+
+// Latency: a block's work is three dependent memory round trips.  (1) The
+// block's row bounds and, from blkq (the entry offsets ia[blocks[k]],
+// stored beside them), its entry bounds; (2) the entries' columns and
+// values, and - issued with them - the bounds of the row this thread
+// reduces and its b (or y) and dsc; (3) the x gathers.  Without blkq the
+// entry bounds come from ia, one round trip more.  The reduction reads
+// LDS only.
+template <typename T, int OP, bool SCALED = false>
+__global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__restrict__ blocks,
+                                                      const int *__restrict__ blkq,
+                                                      const int *__restrict__ ia, const int *__restrict__ ja,
+                                                      const T *__restrict__ val, const T *__restrict__ x,
+                                                      T *__restrict__ y, const T *__restrict__ b,
+                                                      const T *__restrict__ dsc = nullptr,
+                                                      T *__restrict__ ys = nullptr,
+                                                      jacobi_epi<T> e = jacobi_epi<T>{nullptr, nullptr, T(0)}) {
+  __shared__ T prod[BN_ROWBLOCK_NNZ + 1];  // + a spare word padding slots write
+  const int bid = logical_block();
+  if (bid >= nblk) return;
+  const blocks_io<T, OP, SCALED> io{x, y, b, dsc, ys, e};
+  csr_block<T, OP>(bid, blocks, blkq, ia, ja, val, io, prod);
 }
 
 template <typename T>
 void launch_blocks(int op, int nblk, const int *blocks, const int *blkq, const int *ia, const int *ja, const T *val,
-                   const T *x, T *y, const T *b, hipStream_t st) {
+                   const T *x, T *y, const T *b, jacobi_epi<T> e, hipStream_t st) {
   if (nblk <= 0) return;
   const int g = grid_rows(nblk);
   const T *no_dsc = nullptr;
   T *no_ys = nullptr;
   if (op == BN_OP_SET)
     BN_LAUNCH((k_csr_blocks<T, BN_OP_SET>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
-              no_dsc, no_ys);
+              no_dsc, no_ys, e);
   else if (op == BN_OP_RESID)
     BN_LAUNCH((k_csr_blocks<T, BN_OP_RESID>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
-              no_dsc, no_ys);
+              no_dsc, no_ys, e);
+  else if (op == BN_OP_JAC)
+    BN_LAUNCH((k_csr_blocks<T, BN_OP_JAC>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
+              no_dsc, no_ys, e);
   else
     BN_LAUNCH((k_csr_blocks<T, BN_OP_ADD>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
-              no_dsc, no_ys);
+              no_dsc, no_ys, e);
 }
 
 // ---------------------------------------------------------------------
@@ -758,17 +861,6 @@ __global__ __launch_bounds__(BLOCK) void k_trsv_ell(int cnt, const int *__restri
 // sums are ell_sum<T, w>'s, those of the launch-per-level kernel, so y is
 // the same to the last bit.
 // ---------------------------------------------------------------------
-template <typename T> struct walk_word;
-template <> struct walk_word<double> {
-  typedef unsigned long long U;
-  static constexpr U none = 0x7ff4dead0000beefULL;  // signalling NaN
-};
-template <> struct walk_word<float> {
-  typedef unsigned int U;
-  static constexpr U none = 0x7fa0beefU;  // signalling NaN
-};
-// (arithmetic only ever produces quiet NaNs, so no computed y_i is the
-// sentinel)
 
 template <typename T>
 __device__ __forceinline__ typename walk_word<T>::U walk_load(const T *p) {
@@ -1601,21 +1693,27 @@ void bn_csr_resid_blocks_scaled(char vt, int nblk, const int *blocks, const int 
   if (vt == 's')
     BN_LAUNCH((k_csr_blocks<float, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, stream, nblk, blocks, blkq,
                        ia, ja, (const float *)val, (const float *)x, (float *)y, (const float *)b, (const float *)dsc,
-                       (float *)ys);
+                       (float *)ys, jacobi_epi<float>{nullptr, nullptr, 0.0f});
   else
     BN_LAUNCH((k_csr_blocks<double, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, stream, nblk, blocks, blkq,
                        ia, ja, (const double *)val, (const double *)x, (double *)y, (const double *)b,
-                       (const double *)dsc, (double *)ys);
+                       (const double *)dsc, (double *)ys, jacobi_epi<double>{nullptr, nullptr, 0.0});
+}
+
+void bn_csr_apply_blocks_jac(char vt, int op, int nblk, const int *blocks, const int *blkq, const int *ia,
+                             const int *ja, const void *val, const void *x, void *y, const void *b, void *xz,
+                             const void *w, double om, hipStream_t stream) {
+  if (vt == 's')
+    launch_blocks<float>(op, nblk, blocks, blkq, ia, ja, (const float *)val, (const float *)x, (float *)y,
+                         (const float *)b, jacobi_epi<float>{(float *)xz, (const float *)w, (float)om}, stream);
+  else
+    launch_blocks<double>(op, nblk, blocks, blkq, ia, ja, (const double *)val, (const double *)x, (double *)y,
+                          (const double *)b, jacobi_epi<double>{(double *)xz, (const double *)w, om}, stream);
 }
 
 void bn_csr_apply_blocks(char vt, int op, int nblk, const int *blocks, const int *blkq, const int *ia, const int *ja,
                          const void *val, const void *x, void *y, const void *b, hipStream_t stream) {
-  if (vt == 's')
-    launch_blocks<float>(op, nblk, blocks, blkq, ia, ja, (const float *)val, (const float *)x, (float *)y,
-                         (const float *)b, stream);
-  else
-    launch_blocks<double>(op, nblk, blocks, blkq, ia, ja, (const double *)val, (const double *)x, (double *)y,
-                          (const double *)b, stream);
+  bn_csr_apply_blocks_jac(vt, op, nblk, blocks, blkq, ia, ja, val, x, y, b, NULL, NULL, 0.0, stream);
 }
 
 void bn_block_gs(char vt, int nchunk, int max_rows, const int *chunk, const int *ccol, const int *cbnd,
@@ -1633,23 +1731,40 @@ void bn_block_gs(char vt, int nchunk, int max_rows, const int *chunk, const int 
                               backward, lanes, (const double *)zbase, (double *)zsum, stream);
 }
 
-void bn_csr_apply_seg(char vt, int op, int n, const int *ia, const int *ja, const void *val, const void *x,
-                      void *y, const void *b, int lanes, int long_len, int nlong, const int *long_rows,
-                      const int *seg_ptr, const int *seg_row, int nseg, int nmulti, double *part,
-                      hipStream_t stream) {
+void bn_csr_apply_seg_jac(char vt, int op, int n, const int *ia, const int *ja, const void *val, const void *x,
+                          void *y, const void *b, int lanes, int long_len, int nlong, const int *long_rows,
+                          const int *seg_ptr, const int *seg_row, int nseg, double *part, int *cnt, void *xz,
+                          const void *w, double om, hipStream_t stream) {
   if (n <= 0) return;
   if (nlong <= 0) long_len = 0x7fffffff;
   if (vt == 's') {
+    const jacobi_epi<float> e{(float *)xz, (const float *)w, (float)om};
     dispatch_csr<float>(op, n, ia, ja, (const float *)val, (const float *)x, (float *)y, (const float *)b,
-                        lanes, nullptr, nullptr, long_len, stream);
-    launch_seg<float>(op, nlong, nmulti, long_rows, seg_ptr, seg_row, nseg, part, ia, ja, (const float *)val,
-                      (const float *)x, (float *)y, (const float *)b, stream);
+                        lanes, nullptr, nullptr, long_len, stream, e);
+    launch_seg<float>(op, nlong, long_rows, seg_ptr, seg_row, nseg, part, cnt, ia, ja, (const float *)val,
+                      (const float *)x, (float *)y, (const float *)b, e, stream);
   } else {
+    const jacobi_epi<double> e{(double *)xz, (const double *)w, om};
     dispatch_csr<double>(op, n, ia, ja, (const double *)val, (const double *)x, (double *)y,
-                         (const double *)b, lanes, nullptr, nullptr, long_len, stream);
-    launch_seg<double>(op, nlong, nmulti, long_rows, seg_ptr, seg_row, nseg, part, ia, ja, (const double *)val,
-                       (const double *)x, (double *)y, (const double *)b, stream);
+                         (const double *)b, lanes, nullptr, nullptr, long_len, stream, e);
+    launch_seg<double>(op, nlong, long_rows, seg_ptr, seg_row, nseg, part, cnt, ia, ja, (const double *)val,
+                       (const double *)x, (double *)y, (const double *)b, e, stream);
   }
+}
+
+void bn_csr_apply_seg(char vt, int op, int n, const int *ia, const int *ja, const void *val, const void *x,
+                      void *y, const void *b, int lanes, int long_len, int nlong, const int *long_rows,
+                      const int *seg_ptr, const int *seg_row, int nseg, double *part, int *cnt,
+                      hipStream_t stream) {
+  bn_csr_apply_seg_jac(vt, op, n, ia, ja, val, x, y, b, lanes, long_len, nlong, long_rows, seg_ptr, seg_row, nseg,
+                       part, cnt, NULL, NULL, 0.0, stream);
+}
+
+void bn_seg_reset(int nslot, double *part, int nmulti, int *cnt, hipStream_t stream) {
+  if (nslot > 0)
+    BN_LAUNCH(k_fill_word<unsigned long long>, g1(nslot), dim3(BLOCK), 0, stream, nslot,
+              (unsigned long long *)part, walk_word<double>::none);
+  if (nmulti > 0) BN_LAUNCH(k_fill<int>, g1(nmulti), dim3(BLOCK), 0, stream, nmulti, cnt, 0);
 }
 
 void bn_gs_color(char vt, int lo, int hi, const int *ia, const int *ja, const void *val,

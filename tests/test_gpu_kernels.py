@@ -136,6 +136,10 @@ def test_row_block_kernel_against_oracle(oracle, vt):
         assert np.array_equal(dys.numpy().view(bits), (g2 * dsc).astype(dt).view(bits)), name
 
 
+# the sentinel of values not written yet (a signalling-NaN bit pattern)
+SENTINEL64 = np.uint64(0x7ff4dead0000beef)
+
+
 def seg_bin(ia, long_len, seg=1024):
     """Long-row bin with segments: rows longer than long_len, longest first
     (ties by index), cut into `seg`-entry segments."""
@@ -176,15 +180,22 @@ def test_segmented_long_rows_against_oracle(oracle, vt):
             one = np.zeros(1, np.int32)
             drows = dev(rows if len(rows) else one)
             dsp, dsr = dev(seg_ptr), dev(seg_row if len(seg_row) else one)
-            dpart = dev(np.zeros(max(1, int(seg_ptr[nmulti]))))
+            # the multi-segment rows' partials pass through part (the
+            # sentinel between calls) and are counted in cnt (zero between
+            # calls): every call must leave both as it found them
+            nslot = int(seg_ptr[nmulti])
+            dpart = dev(np.zeros(max(1, nslot)))
+            dcnt = dev(np.ones(max(1, nmulti), np.int32))
+            lib.bn_seg_reset(nslot, dpart.ptr, nmulti, dcnt.ptr, None)
             for op, ref, init in ((0, ref_set, None), (1, ref_res, None), (2, ref_add, y0)):
                 dy = dev((init if init is not None else np.zeros(n)).astype(dt))
                 lib.bn_csr_apply_seg(vt.encode(), op, n, dia.ptr, dja.ptr, dv.ptr, dx.ptr, dy.ptr, db.ptr, lanes,
                                      16 * (lanes & 0xFF), len(rows), drows.ptr, dsp.ptr, dsr.ptr, int(seg_ptr[-1]),
-                                     nmulti,
-                                     dpart.ptr, None)
+                                     dpart.ptr, dcnt.ptr, None)
                 got = dy.numpy().astype(np.float64)
                 assert relerr(got, ref) < tol, (name, lanes, op)
+                assert np.all(dpart.numpy()[:nslot].view(np.uint64) == SENTINEL64), (name, lanes, op)
+                assert np.all(dcnt.numpy()[:nmulti] == 0), (name, lanes, op)
 
 
 def test_restriction_gather_equals_scatter(oracle):

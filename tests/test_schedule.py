@@ -97,6 +97,79 @@ def test_colour_order_is_a_proper_colouring():
         assert np.all(np.diff(seg) > 0)
 
 
+def colour_of(ia, ja, val, mis, split):
+    n = len(ia) - 1
+    m = L.CSR(n, n, len(ja), L.ip(ia), L.ip(ja), L.dp(val))
+    perm = np.zeros(n, np.int32)
+    cp = C.POINTER(C.c_int)()
+    ncol = lib.bn_colour_order(C.byref(m), L.ip(mis), split, L.ip(perm), C.byref(cp))
+    col_ptr = np.ctypeslib.as_array(cp, (ncol + 1,)).copy()
+    colour = np.empty(n, np.int64)
+    for c in range(ncol):
+        colour[perm[col_ptr[c]:col_ptr[c + 1]]] = c
+    A = sp.csr_matrix((val, ja, ia), shape=(n, n)).tocoo()
+    off = A.row != A.col
+    assert not np.any(colour[A.row[off]] == colour[A.col[off]])   # a proper colouring
+    return ncol, colour
+
+
+def grid(k, diag=False):
+    n = k * k
+    rows, cols = [], []
+    for i in range(k):
+        for j in range(k):
+            u = i * k + j
+            for di, dj in ((0, 1), (1, 0)) + (((1, 1),) if diag else ()):
+                if i + di < k and j + dj < k:
+                    v = (i + di) * k + j + dj
+                    rows += [u, v]
+                    cols += [v, u]
+    A = sp.csr_matrix((-np.ones(len(rows)), (rows, cols)), shape=(n, n)) + 4.0 * sp.identity(n)
+    A = A.tocsr()
+    A.sort_indices()
+    return A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data.astype(np.float64)
+
+
+def test_bipartite_couplings_take_two_colours():
+    # a grid has no odd cycle: two colours by parity (the greedy pass in row
+    # order also finds two here; the parity colouring is the checkerboard)
+    ia, ja, val = grid(9)
+    n = len(ia) - 1
+    ncol, colour = colour_of(ia, ja, val, np.zeros(n, np.int32), 0)
+    i, j = np.divmod(np.arange(n), 9)
+    assert ncol == 2 and np.array_equal(colour, (i + j) % 2)
+    # the C set of the checkerboard couples to nothing of its own: one
+    # colour per class
+    mis = ((i + j) % 2 == 0).astype(np.int32)
+    ncol, colour = colour_of(ia, ja, val, mis, 1)
+    assert ncol == 2 and np.array_equal(colour, 1 - mis)
+
+
+def test_crown_graph_takes_two_colours_where_greedy_takes_three():
+    # a_i couples to b_j for i != j, rows ordered a1 b1 a2 b2 a3 b3: the
+    # greedy pass in row order needs three colours, the graph is bipartite
+    # and the parity colouring takes two (the a's and the b's)
+    k = 3
+    edges = [(2 * i, 2 * j + 1) for i in range(k) for j in range(k) if i != j]
+    n = 2 * k
+    rows = [a for a, b in edges] + [b for a, b in edges]
+    cols = [b for a, b in edges] + [a for a, b in edges]
+    A = (sp.csr_matrix((-np.ones(len(rows)), (rows, cols)), shape=(n, n)) + 4.0 * sp.identity(n)).tocsr()
+    A.sort_indices()
+    ia, ja, val = A.indptr.astype(np.int32), A.indices.astype(np.int32), A.data
+    ncol, colour = colour_of(ia, ja, val, np.zeros(n, np.int32), 0)
+    assert ncol == 2 and np.array_equal(colour, np.arange(n) % 2)
+
+
+def test_odd_cycle_falls_back_to_the_greedy_colouring():
+    # diagonals make triangles: no two-colouring exists, the greedy pass
+    # colours (properly, checked in colour_of) with three or more
+    ia, ja, val = grid(7, diag=True)
+    n = len(ia) - 1
+    ncol, _ = colour_of(ia, ja, val, np.zeros(n, np.int32), 0)
+    assert ncol >= 3
+
+
 def test_level_sets_respect_dependencies():
     net = make_network(10, 5, seed=6)
     A = sp.csr_matrix((net.val, net.ja, net.ia), shape=(net.n, net.n))
