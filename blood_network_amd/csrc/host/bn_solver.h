@@ -293,29 +293,32 @@ static inline void bn_dcsr_apply(const bn_dcsr *m, char vt, int op, int lanes, c
  * with the long-row bin; with a Jacobi step fused in (xz, w, om as
  * bn_csr_apply_seg_jac; op BN_OP_JAC) */
 static inline void bn_op_apply_jac(const bn_dcsr *m, char vt, int op, const void *x, void *y, const void *b,
-                                   void *xz, const void *w, double om, hipStream_t st) {
+                                   void *xz, const void *w, double om, const unsigned char *colour, int c,
+                                   hipStream_t st) {
   const void *val = vt == 's' ? (const void *)m->vs : (const void *)m->vd;
   if (vt == 's' ? m->adapt_s : m->adapt) {
-    bn_csr_apply_blocks_jac(vt, op, m->nblk, m->blk, m->blkq, m->ia, m->ja, val, x, y, b, xz, w, om, st);
+    bn_csr_apply_blocks_jac(vt, op, m->nblk, m->blk, m->blkq, m->ia, m->ja, val, x, y, b, xz, w, om, colour, c,
+                            st);
     return;
   }
   const int lanes = bn_spmv_lanes(m, vt);
   const int nl = bn_nlong(m, lanes);
   const int nseg = nl > 0 ? m->seg_ptr_h[nl] : 0;
   bn_csr_apply_seg_jac(vt, op, m->rows, m->ia, m->ja, val, x, y, b, lanes, bn_long_len(lanes), nl, m->long_rows,
-                       m->seg_ptr, m->seg_row, nseg, m->seg_part, m->seg_cnt, xz, w, om, st);
+                       m->seg_ptr, m->seg_row, nseg, m->seg_part, m->seg_cnt, xz, w, om, colour, c, st);
 }
 static inline void bn_op_apply(const bn_dcsr *m, char vt, int op, const void *x, void *y, const void *b,
                                hipStream_t st) {
-  bn_op_apply_jac(m, vt, op, x, y, b, NULL, NULL, 0.0, st);
+  bn_op_apply_jac(m, vt, op, x, y, b, NULL, NULL, 0.0, NULL, 0, st);
 }
 
 /* census.c: the V-cycle's and the outer PCG's bandwidth kernels, launched
  * on the solver's stream (recorded while a kernel census runs) */
 void bn_launch_apply(struct bn_solver *s, const bn_dcsr *M, char vt, int op, const void *x, void *y, const void *b);
-/* the same with a Jacobi step fused in (bn_op_apply_jac) */
+/* the same with a smoother's first step or a Jacobi step fused in
+ * (bn_op_apply_jac) */
 void bn_launch_apply_jac(struct bn_solver *s, const bn_dcsr *M, char vt, int op, const void *x, void *y,
-                         const void *b, void *xz, const void *w, double om);
+                         const void *b, void *xz, const void *w, double om, const unsigned char *colour, int c);
 void bn_launch_resid_scaled(struct bn_solver *s, const bn_dcsr *M, char vt, const void *x, void *y, const void *b,
                             const void *dsc, void *ys);
 void bn_launch_gs(struct bn_solver *s, const bn_dlevel *d, char vt, int c, int lanes, const void *b, void *x,

@@ -34,6 +34,7 @@ typedef struct {
   const void *m;       /* bn_dcsr (K_APPLY, K_RESID_SCALED, K_JACOBI, K_SPMV) or bn_dlevel (K_GS) */
   const void *v[7];    /* the launch's vector arguments */
   double om;           /* K_APPLY: the fused Jacobi step's damping */
+  const unsigned char *colour; /* K_APPLY: the fused first colour's colour array and colour */
 } ckey;
 
 typedef struct {
@@ -94,6 +95,7 @@ static void shape_describe(struct bn_solver *s, cshape *sh) {
       if (k->kind == K_RESID_SCALED) b += 2.0 * M->rows * v;
       /* a fused Jacobi step: its weights read, and for SET the step written */
       if (k->kind == K_APPLY && k->v[4]) b += (double)M->rows * v * (k->op == BN_OP_SET ? 2.0 : 1.0);
+      if (k->kind == K_APPLY && k->colour) b += (double)M->rows; /* the colour bytes */
       sh->bytes = b;
       break;
     }
@@ -183,20 +185,22 @@ static ckey mk(int kind, char vt, int op, const void *m) {
  * launch wrappers
  * ------------------------------------------------------------------- */
 void bn_launch_apply_jac(struct bn_solver *s, const bn_dcsr *M, char vt, int op, const void *x, void *y,
-                         const void *b, void *xz, const void *w, double om) {
+                         const void *b, void *xz, const void *w, double om, const unsigned char *colour, int c) {
   int ev = -1;
   if (s->census) {
     ckey k = mk(K_APPLY, vt, op, M);
     k.v[0] = x, k.v[1] = y, k.v[2] = b, k.v[3] = xz, k.v[4] = w;
     k.om = om;
+    k.colour = colour;
+    k.c = c;
     ev = census_begin(s, &k);
   }
-  bn_op_apply_jac(M, vt, op, x, y, b, xz, w, om, s->stream);
+  bn_op_apply_jac(M, vt, op, x, y, b, xz, w, om, colour, c, s->stream);
   census_end(s, ev);
 }
 
 void bn_launch_apply(struct bn_solver *s, const bn_dcsr *M, char vt, int op, const void *x, void *y, const void *b) {
-  bn_launch_apply_jac(s, M, vt, op, x, y, b, NULL, NULL, 0.0);
+  bn_launch_apply_jac(s, M, vt, op, x, y, b, NULL, NULL, 0.0, NULL, 0);
 }
 
 void bn_launch_resid_scaled(struct bn_solver *s, const bn_dcsr *M, char vt, const void *x, void *y, const void *b,
@@ -269,7 +273,7 @@ static void relaunch(struct bn_solver *s, const ckey *k) {
   switch (k->kind) {
     case K_APPLY:
       bn_launch_apply_jac(s, (const bn_dcsr *)k->m, k->vt, k->op, k->v[0], (void *)k->v[1], k->v[2],
-                          (void *)k->v[3], k->v[4], k->om);
+                          (void *)k->v[3], k->v[4], k->om, k->colour, k->c);
       break;
     case K_RESID_SCALED:
       bn_launch_resid_scaled(s, (const bn_dcsr *)k->m, k->vt, k->v[0], (void *)k->v[1], k->v[2], k->v[3],

@@ -153,7 +153,8 @@ void bn_gs_colour(struct bn_solver *s, const bn_dlevel *d, char vt, int c, int l
 }
 
 /* nu smoothing sweeps on level `lev`; backward runs colours in reverse.
- * x_zero: x is zero on entry (first colour then needs no row sums).
+ * x_zero: x is zero on entry (first colour then needs no row sums); 2:
+ * and the first colour's values are already in x (gs_first_fused).
  * zsum != NULL: the final sweep also writes zsum = zbase + x. */
 static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int backward, int x_zero,
                    const void *zbase, void *zsum) {
@@ -203,6 +204,7 @@ static void smooth(struct bn_solver *s, int lev, const void *b, void *x, int bac
         const int lo = d->col_ptr[c], hi = d->col_ptr[c + 1];
         const void *dinvc = vecv(d->dinvc_d, d->dinvc_s, vt);
         if (sw == 0 && t == 0 && fused_zero) {
+          if (x_zero == 2) continue; /* written by the restriction that formed b */
           if (d->colour_d)
             bn_gs_color_zero_colour(vt, n, c, d->colour_d, vecv(d->dinv_d, d->dinv_s, vt), b, x, s->stream);
           else
@@ -248,6 +250,17 @@ static int jacobi_fused(const struct bn_solver *s, int l) {
   return !d->block && !(d->use_gs && s->opt.smoother == SMOOTHER_CF_GS);
 }
 
+/* Level l's forward multicolour sweep from zero starts with the kernel
+ * that writes b_i / a_ii on colour 0 and zero elsewhere (smooth()'s
+ * fused_zero), and that too rides on the restriction forming b. */
+static int gs_first_fused(const struct bn_solver *s, int l) {
+  if (!s->jac_fuse || l <= 0 || l >= s->nlev - 1 || bn_vc_exact_kind(s) >= 0) return 0;
+  const bn_dlevel *d = &s->lev[l];
+  if (d->block || !(d->use_gs && s->opt.smoother == SMOOTHER_CF_GS) || d->ncol <= 0 || !d->colour_d) return 0;
+  const int walk = s->vt == 's' ? d->gs_walk_s : d->gs_walk;
+  return !(s->walk_on && walk > 0 && d->gs_jx); /* the one-launch sweep starts from the sentinel */
+}
+
 static void jacobi_weights(const struct bn_solver *s, const bn_dlevel *d, const void **w, double *om) {
   const int jac = (s->opt.smoother == SMOOTHER_JACOBI);
   const char vt = s->vt;
@@ -265,15 +278,16 @@ static void *jacobi_steps(struct bn_solver *s, int l, const void *b, void *cur, 
   jacobi_weights(s, d, &w, &om);
   for (int i = 0; i < k; i++) {
     void *out = cur == x ? d->r : x;
-    bn_launch_apply_jac(s, &d->A, s->vt, BN_OP_JAC, cur, out, b, NULL, w, om);
+    bn_launch_apply_jac(s, &d->A, s->vt, BN_OP_JAC, cur, out, b, NULL, w, om, NULL, 0);
     cur = out;
   }
   return cur;
 }
 
-/* x_zero: 1 x is zero on entry; 2 (a fused Jacobi level only) the first
- * Jacobi step from zero is already in x, or in the level's r when the
- * number of further steps is odd */
+/* x_zero: 1 x is zero on entry; 2 the smoother's first step from zero
+ * was written by the restriction that formed b: on a fused Jacobi level
+ * into x, or into the level's r when the number of further steps is odd,
+ * on a Gauss-Seidel level the first colour into x (gs_first_fused) */
 void bn_vc_backslash(struct bn_solver *s, int lev, const void *b, void *x, int x_zero) {
   bn_dlevel *d = &s->lev[lev];
   if (lev == s->nlev - 1) {
@@ -300,7 +314,14 @@ void bn_vc_backslash(struct bn_solver *s, int lev, const void *b, void *x, int x
     const void *w;
     double om;
     jacobi_weights(s, c, &w, &om);
-    bn_launch_apply_jac(s, &d->R, s->vt, BN_OP_SET, d->r, c->b, NULL, ((nu - 1) & 1) ? c->r : c->x, w, om);
+    bn_launch_apply_jac(s, &d->R, s->vt, BN_OP_SET, d->r, c->b, NULL, ((nu - 1) & 1) ? c->r : c->x, w, om, NULL,
+                        0);
+    bn_vc_backslash(s, lev + 1, c->b, c->x, 2);
+  } else if (gs_first_fused(s, lev + 1)) {
+    /* and so does the first colour of a Gauss-Seidel child's sweep from
+     * zero (b_i / a_ii on colour 0, zero elsewhere) */
+    bn_launch_apply_jac(s, &d->R, s->vt, BN_OP_SET, d->r, c->b, NULL, c->x, vecv(c->dinv_d, c->dinv_s, s->vt), 1.0,
+                        c->colour_d, 0);
     bn_vc_backslash(s, lev + 1, c->b, c->x, 2);
   } else {
     apply(s, &d->R, BN_OP_SET, d->r, c->b, NULL);

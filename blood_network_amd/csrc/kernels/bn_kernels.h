@@ -112,16 +112,19 @@ void bn_csr_apply_seg(char vt, int op, int n, const int *ia, const int *ja,
                       int lanes, int long_len, int nlong, const int *long_rows,
                       const int *seg_ptr, const int *seg_row, int nseg,
                       double *part, int *cnt, hipStream_t stream);
-/* The same with a Jacobi step fused in (op BN_OP_SET with xz != NULL: also
- * xz_r = om y_r w_r, bn_jacobi_zero's x for right-hand side y; op
- * BN_OP_JAC: y_r = x_r + om (b_r - (A x)_r) w_r, what BN_OP_RESID into r
- * followed by bn_jacobi_update(r, w, om, x) leave in x, to the last bit;
- * y must not be x). */
+/* The same with a smoother's first step fused in (op BN_OP_SET with xz !=
+ * NULL: also xz_r = om y_r w_r, bn_jacobi_zero's x for right-hand side y,
+ * and with colour != NULL that only where colour_r == c and zero elsewhere,
+ * bn_gs_color_zero_colour's x when om = 1 and w = 1/a_ii), or a Jacobi step
+ * (op BN_OP_JAC: y_r = x_r + om (b_r - (A x)_r) w_r, what BN_OP_RESID into
+ * r followed by bn_jacobi_update(r, w, om, x) leave in x; y must not be x),
+ * to the last bit. */
 void bn_csr_apply_seg_jac(char vt, int op, int n, const int *ia, const int *ja,
                           const void *val, const void *x, void *y, const void *b,
                           int lanes, int long_len, int nlong, const int *long_rows,
                           const int *seg_ptr, const int *seg_row, int nseg,
-                          double *part, int *cnt, void *xz, const void *w, double om, hipStream_t stream);
+                          double *part, int *cnt, void *xz, const void *w, double om,
+                          const unsigned char *colour, int c, hipStream_t stream);
 /* part[0..nslot) = the sentinel, cnt[0..nmulti) = 0 */
 void bn_seg_reset(int nslot, double *part, int nmulti, int *cnt, hipStream_t stream);
 
@@ -146,7 +149,7 @@ void bn_csr_apply_blocks(char vt, int op, int nblk, const int *blocks, const int
 void bn_csr_apply_blocks_jac(char vt, int op, int nblk, const int *blocks, const int *blkq,
                              const int *ia, const int *ja, const void *val,
                              const void *x, void *y, const void *b, void *xz, const void *w, double om,
-                             hipStream_t stream);
+                             const unsigned char *colour, int c, hipStream_t stream);
 /* y = b - A x by the row-block kernel, also writing ys_r = y_r dsc_r */
 void bn_csr_resid_blocks_scaled(char vt, int nblk, const int *blocks, const int *blkq, const int *ia,
                                 const int *ja, const void *val, const void *x, void *y, const void *b,

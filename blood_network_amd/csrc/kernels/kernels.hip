@@ -225,7 +225,10 @@ inline int grid_for(long long nrb) {
 // The Jacobi work a CSR launch can do on the rows it finishes, so that a
 // Jacobi-smoothed level needs no launch of its own for it:
 //  * SET with xz: also xz_r = om y_r w_r, the first Jacobi step from zero
-//    on the result (k_jacobi_zero's product), written beside y;
+//    on the result (k_jacobi_zero's product), written beside y; with a
+//    colour array only on the rows of colour c and zero elsewhere (om = 1,
+//    w = 1/a_ii: k_diag_scale_colour's first Gauss-Seidel colour from
+//    zero);
 //  * JAC: y_r = x_r + om (b_r - (A x)_r) w_r, one Jacobi step - the
 //    residual rounded as the RESID launch stores it, then
 //    k_jacobi_update's update - into y, which must not be x.
@@ -235,6 +238,8 @@ struct jacobi_epi {
   T *xz;
   const T *w;
   T om;
+  const unsigned char *colour;  // SET: xz_r only where colour_r == c, zero elsewhere
+  int c;
 };
 
 template <typename T, int OP>
@@ -251,7 +256,10 @@ __device__ __forceinline__ T csr_finish(T s, T base, const T *__restrict__ x, in
 template <typename T, int OP>
 __device__ __forceinline__ void csr_finish_store(T *__restrict__ y, int row, T out, const jacobi_epi<T> &e) {
   y[row] = out;
-  if (OP == BN_OP_SET && e.xz) e.xz[row] = e.om * out * e.w[row];
+  if (OP == BN_OP_SET && e.xz) {
+    const T v = e.om * out * e.w[row];
+    e.xz[row] = !e.colour || e.colour[row] == e.c ? v : T(0);
+  }
 }
 
 // ---------------------------------------------------------------------
@@ -341,7 +349,7 @@ void launch_csr_rounds(int deep, int op, int n, const int *ia, const int *ja, co
 template <typename T>
 void dispatch_csr(int op, int n, const int *ia, const int *ja, const T *val, const T *x, T *y,
                   const T *b, int lanes_in, const T *w, double *partial, int long_len, hipStream_t st,
-                  jacobi_epi<T> e = jacobi_epi<T>{nullptr, nullptr, T(0)}) {
+                  jacobi_epi<T> e = jacobi_epi<T>{nullptr, nullptr, T(0), nullptr, 0}) {
   const int deep = lanes_in & (BN_LANES_DEEP8 | BN_LANES_DEEP16);
   const int lanes = lanes_in & BN_LANES_MASK;
   if (deep) {
@@ -605,7 +613,7 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__res
                                                       T *__restrict__ y, const T *__restrict__ b,
                                                       const T *__restrict__ dsc = nullptr,
                                                       T *__restrict__ ys = nullptr,
-                                                      jacobi_epi<T> e = jacobi_epi<T>{nullptr, nullptr, T(0)}) {
+                                                      jacobi_epi<T> e = jacobi_epi<T>{nullptr, nullptr, T(0), nullptr, 0}) {
   __shared__ T prod[BN_ROWBLOCK_NNZ + 1];  // + a spare word padding slots write
   const int bid = logical_block();
   if (bid >= nblk) return;
@@ -1693,27 +1701,29 @@ void bn_csr_resid_blocks_scaled(char vt, int nblk, const int *blocks, const int 
   if (vt == 's')
     BN_LAUNCH((k_csr_blocks<float, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, stream, nblk, blocks, blkq,
                        ia, ja, (const float *)val, (const float *)x, (float *)y, (const float *)b, (const float *)dsc,
-                       (float *)ys, jacobi_epi<float>{nullptr, nullptr, 0.0f});
+                       (float *)ys, jacobi_epi<float>{nullptr, nullptr, 0.0f, nullptr, 0});
   else
     BN_LAUNCH((k_csr_blocks<double, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, stream, nblk, blocks, blkq,
                        ia, ja, (const double *)val, (const double *)x, (double *)y, (const double *)b,
-                       (const double *)dsc, (double *)ys, jacobi_epi<double>{nullptr, nullptr, 0.0});
+                       (const double *)dsc, (double *)ys, jacobi_epi<double>{nullptr, nullptr, 0.0, nullptr, 0});
 }
 
 void bn_csr_apply_blocks_jac(char vt, int op, int nblk, const int *blocks, const int *blkq, const int *ia,
                              const int *ja, const void *val, const void *x, void *y, const void *b, void *xz,
-                             const void *w, double om, hipStream_t stream) {
+                             const void *w, double om, const unsigned char *colour, int c, hipStream_t stream) {
   if (vt == 's')
     launch_blocks<float>(op, nblk, blocks, blkq, ia, ja, (const float *)val, (const float *)x, (float *)y,
-                         (const float *)b, jacobi_epi<float>{(float *)xz, (const float *)w, (float)om}, stream);
+                         (const float *)b, jacobi_epi<float>{(float *)xz, (const float *)w, (float)om, colour, c},
+                         stream);
   else
     launch_blocks<double>(op, nblk, blocks, blkq, ia, ja, (const double *)val, (const double *)x, (double *)y,
-                          (const double *)b, jacobi_epi<double>{(double *)xz, (const double *)w, om}, stream);
+                          (const double *)b, jacobi_epi<double>{(double *)xz, (const double *)w, om, colour, c},
+                          stream);
 }
 
 void bn_csr_apply_blocks(char vt, int op, int nblk, const int *blocks, const int *blkq, const int *ia, const int *ja,
                          const void *val, const void *x, void *y, const void *b, hipStream_t stream) {
-  bn_csr_apply_blocks_jac(vt, op, nblk, blocks, blkq, ia, ja, val, x, y, b, NULL, NULL, 0.0, stream);
+  bn_csr_apply_blocks_jac(vt, op, nblk, blocks, blkq, ia, ja, val, x, y, b, NULL, NULL, 0.0, NULL, 0, stream);
 }
 
 void bn_block_gs(char vt, int nchunk, int max_rows, const int *chunk, const int *ccol, const int *cbnd,
@@ -1734,17 +1744,17 @@ void bn_block_gs(char vt, int nchunk, int max_rows, const int *chunk, const int 
 void bn_csr_apply_seg_jac(char vt, int op, int n, const int *ia, const int *ja, const void *val, const void *x,
                           void *y, const void *b, int lanes, int long_len, int nlong, const int *long_rows,
                           const int *seg_ptr, const int *seg_row, int nseg, double *part, int *cnt, void *xz,
-                          const void *w, double om, hipStream_t stream) {
+                          const void *w, double om, const unsigned char *colour, int c, hipStream_t stream) {
   if (n <= 0) return;
   if (nlong <= 0) long_len = 0x7fffffff;
   if (vt == 's') {
-    const jacobi_epi<float> e{(float *)xz, (const float *)w, (float)om};
+    const jacobi_epi<float> e{(float *)xz, (const float *)w, (float)om, colour, c};
     dispatch_csr<float>(op, n, ia, ja, (const float *)val, (const float *)x, (float *)y, (const float *)b,
                         lanes, nullptr, nullptr, long_len, stream, e);
     launch_seg<float>(op, nlong, long_rows, seg_ptr, seg_row, nseg, part, cnt, ia, ja, (const float *)val,
                       (const float *)x, (float *)y, (const float *)b, e, stream);
   } else {
-    const jacobi_epi<double> e{(double *)xz, (const double *)w, om};
+    const jacobi_epi<double> e{(double *)xz, (const double *)w, om, colour, c};
     dispatch_csr<double>(op, n, ia, ja, (const double *)val, (const double *)x, (double *)y,
                          (const double *)b, lanes, nullptr, nullptr, long_len, stream, e);
     launch_seg<double>(op, nlong, long_rows, seg_ptr, seg_row, nseg, part, cnt, ia, ja, (const double *)val,
@@ -1757,7 +1767,7 @@ void bn_csr_apply_seg(char vt, int op, int n, const int *ia, const int *ja, cons
                       const int *seg_ptr, const int *seg_row, int nseg, double *part, int *cnt,
                       hipStream_t stream) {
   bn_csr_apply_seg_jac(vt, op, n, ia, ja, val, x, y, b, lanes, long_len, nlong, long_rows, seg_ptr, seg_row, nseg,
-                       part, cnt, NULL, NULL, 0.0, stream);
+                       part, cnt, NULL, NULL, 0.0, NULL, 0, stream);
 }
 
 void bn_seg_reset(int nslot, double *part, int nmulti, int *cnt, hipStream_t stream) {
