@@ -117,6 +117,8 @@ static void release_all(struct bn_solver *s) {
   bn_device_release(s);
   if (s->host_scal) hipHostFree(s->host_scal);
   s->host_scal = NULL;
+  if (s->host_step) hipHostFree(s->host_step);
+  s->host_step = NULL;
   if (s->ev0) hipEventDestroy(s->ev0);
   if (s->ev1) hipEventDestroy(s->ev1);
   s->ev0 = s->ev1 = NULL;

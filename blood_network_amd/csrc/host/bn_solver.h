@@ -202,6 +202,9 @@ struct bn_solver {
    * units (the most parts), wait limit in wall_clock64() ticks */
   int walk_ok, walk_on, walk_blocks;
   long long walk_limit;
+  int u1_first;         /* the next preconditioner's level-0 sweep from zero
+                           finds its first colour in u1 (written by the
+                           outer x/r update) */
   int jac_fuse;         /* Jacobi-smoothed levels fuse their steps into the
                            launches around them (vcycle.c; BN_FUSE_JACOBI=0
                            keeps separate launches) */
@@ -218,6 +221,8 @@ struct bn_solver {
   double *part1, *part2, *part3;
   double *cpart1, *cpart2;  /* partials of the coarsest IC-PCG */
   double *host_scal;    /* pinned mirror of scal */
+  double *host_step;    /* coherent host memory the step's last kernel
+                           copies the scalars into (NULL: a copy per step) */
   double *hbuf;         /* host staging (n doubles, pinned) */
   hipEvent_t ev0, ev1;
   /* captured PCG steps, one per mode (never re-captured in place) */
@@ -375,6 +380,9 @@ void bn_vc_precond(struct bn_solver *s, const double *g, double *z);
 void bn_vc_vcycle_precond(struct bn_solver *s, const double *g, double *z);
 /* FP32 V-cycle: FP32 input in s->g_in, FP32 result left in s->zout */
 void bn_vc_precond_s(struct bn_solver *s);
+/* level 0's sweep from zero starts with the first-colour kernel that the
+ * outer x/r update can write (bn_pcg_update_xr_s0, then u1_first = 1) */
+int bn_vc_first_colour_fused(const struct bn_solver *s);
 void bn_vc_fine_ic(struct bn_solver *s, const void *b, void *x);
 
 /* pcg.c */

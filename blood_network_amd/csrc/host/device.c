@@ -1493,6 +1493,9 @@ int bn_device_upload(struct bn_solver *s) {
     if (!rc) rc = alloc_vec(s, (void **)&s->cpart1, BN_RED_BLOCKS);
     if (!rc) rc = alloc_vec(s, (void **)&s->cpart2, BN_RED_BLOCKS);
     if (!rc && !s->host_scal) CK(hipHostMalloc((void **)&s->host_scal, BN_S_COUNT * 2 * sizeof(double), 0));
+    if (!rc && !s->host_step &&
+        hipHostMalloc((void **)&s->host_step, BN_S_COUNT * sizeof(double), hipHostMallocCoherent) != hipSuccess)
+      s->host_step = NULL; /* the step's scalars are then copied */
   }
   for (int l = 0; l < nlev; l++) free(inv[l]);
   tph[4] = now_ms();

@@ -17,6 +17,7 @@
  */
 #include <math.h>
 #include <stdio.h>
+#include <string.h>
 #include <stdlib.h>
 #include <time.h>
 
@@ -47,7 +48,15 @@ static void enqueue_step(struct bn_solver *s) {
   bn_launch_spmv(s, A, BN_OP_SET, s->p, s->q, NULL, s->p, s->part1);
   bn_scalar_finish(1, s->part1, NULL, NULL, sc, 0, st);
   if (f32) {
-    bn_pcg_update_xr_s(n, s->x, s->p, s->r, s->q, sc, s->part2, (float *)s->g_in, st);
+    const bn_dlevel *d0 = &s->lev[0];
+    if (bn_vc_first_colour_fused(s)) {
+      /* the preconditioner's first colour rides on the update */
+      bn_pcg_update_xr_s0(n, s->x, s->p, s->r, s->q, sc, s->part2, (float *)s->g_in, d0->colour_d, 0, d0->dinv_s,
+                          (float *)s->u1, st);
+      s->u1_first = 1;
+    } else {
+      bn_pcg_update_xr_s(n, s->x, s->p, s->r, s->q, sc, s->part2, (float *)s->g_in, st);
+    }
     bn_vc_precond_s(s);
     bn_dot_rz_s(n, s->r, (const float *)s->zout, flex ? s->q : NULL, s->part1, s->part3, st);
   } else {
@@ -55,7 +64,11 @@ static void enqueue_step(struct bn_solver *s) {
     bn_vc_precond(s, s->r, s->z);
     bn_dot_partial(n, s->r, s->z, flex ? s->z : NULL, flex ? s->q : NULL, s->part1, s->part3, st);
   }
-  bn_scalar_finish(flex ? 4 : 2, s->part1, s->part2, flex ? s->part3 : NULL, sc, 0, st);
+  /* the scalars also go straight to the host (no copy after the step) */
+  if (s->host_step)
+    bn_scalar_finish_host(flex ? 4 : 2, s->part1, s->part2, flex ? s->part3 : NULL, sc, s->host_step, st);
+  else
+    bn_scalar_finish(flex ? 4 : 2, s->part1, s->part2, flex ? s->part3 : NULL, sc, 0, st);
 }
 
 static int mode_key(const struct bn_solver *s) {
@@ -170,11 +183,12 @@ static int pcg_body(struct bn_solver *s, const double *d_b, double *d_x, double 
     for (it = 1; it <= maxiter; it++) {
       if (gexec) hipGraphLaunch(gexec, st);
       else enqueue_step(s);
-      hipMemcpyAsync(hs, sc, BN_S_COUNT * sizeof(double), hipMemcpyDeviceToHost, st);
+      if (!s->host_step) hipMemcpyAsync(hs, sc, BN_S_COUNT * sizeof(double), hipMemcpyDeviceToHost, st);
       if (hipStreamSynchronize(st) != hipSuccess) {
         bn_set_error("device failure in PCG step %d", it);
         return BN_ERR_HIP;
       }
+      if (s->host_step) memcpy(hs, s->host_step, BN_S_COUNT * sizeof(double));
       if (hs[BN_S_WALKERR] != 0.0) return walk_gave_up(s, d_b, d_x, tol, maxiter, print_level, info);
       rel = hs[BN_S_REL];
       rnorm = sqrt(hs[BN_S_RR]);

@@ -254,12 +254,14 @@ static int jacobi_fused(const struct bn_solver *s, int l) {
  * that writes b_i / a_ii on colour 0 and zero elsewhere (smooth()'s
  * fused_zero), and that too rides on the restriction forming b. */
 static int gs_first_fused(const struct bn_solver *s, int l) {
-  if (!s->jac_fuse || l <= 0 || l >= s->nlev - 1 || bn_vc_exact_kind(s) >= 0) return 0;
+  if (!s->jac_fuse || l < 0 || l >= s->nlev - 1 || bn_vc_exact_kind(s) >= 0) return 0;
   const bn_dlevel *d = &s->lev[l];
   if (d->block || !(d->use_gs && s->opt.smoother == SMOOTHER_CF_GS) || d->ncol <= 0 || !d->colour_d) return 0;
   const int walk = s->vt == 's' ? d->gs_walk_s : d->gs_walk;
   return !(s->walk_on && walk > 0 && d->gs_jx); /* the one-launch sweep starts from the sentinel */
 }
+
+int bn_vc_first_colour_fused(const struct bn_solver *s) { return gs_first_fused(s, 0); }
 
 static void jacobi_weights(const struct bn_solver *s, const bn_dlevel *d, const void **w, double *om) {
   const int jac = (s->opt.smoother == SMOOTHER_JACOBI);
@@ -545,11 +547,19 @@ static int coarse_icpcg(struct bn_solver *s, const double *b, double *x) {
  * preconditioners (input g and output z are FP64, level-0 numbering)
  * ------------------------------------------------------------------- */
 /* gv, zv in the V-cycle's precision (FP32 copies when vt = 's') */
+/* x_zero of the level-0 half from zero: 2 when the caller's x/r update
+ * already wrote the sweep's first colour into u1 (u1_first) */
+static int u1_start(struct bn_solver *s) {
+  const int z = s->u1_first ? 2 : 1;
+  s->u1_first = 0;
+  return z;
+}
+
 static void ichol_core(struct bn_solver *s, const void *gv, void *zv) {
   const char vt = s->vt;
   const bn_dcsr *A = &s->Asolve;
   /* u1 = backslash(g) from zero */
-  bn_vc_backslash(s, 0, gv, s->u1, 1);
+  bn_vc_backslash(s, 0, gv, s->u1, u1_start(s));
   /* r1 = g - A u1; u2 = u1 + (L L^T)^{-1} r1 (u2 kept in icx's partner).
    * With Jacobi-swept IC and the row-block kernel for A, the residual
    * kernel also writes the lower triangle's first step r1_i / l_ii */
@@ -571,7 +581,7 @@ static void ichol_core(struct bn_solver *s, const void *gv, void *zv) {
 }
 
 static void vcycle_core(struct bn_solver *s, const void *gv, void *zv) {
-  bn_vc_backslash(s, 0, gv, s->u1, 1);
+  bn_vc_backslash(s, 0, gv, s->u1, u1_start(s));
   apply(s, &s->lev[0].A, BN_OP_RESID, s->u1, s->r1, gv);
   fwdslash_rec(s, 0, s->r1, s->corr, 1, s->u1, zv);
 }

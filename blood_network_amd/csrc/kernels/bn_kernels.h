@@ -331,6 +331,13 @@ void bn_pcg_update_p(int n, double *p, const double *z, const double *s,
 void bn_pcg_update_xr_s(int n, double *x, const double *p, double *r,
                         const double *q, const double *s, double *partial,
                         float *rs, hipStream_t stream);
+/* bn_pcg_update_xr_s that also writes the first colour of a multicolour
+ * sweep from zero with the FP32 r as right-hand side: x0_i = rs_i dinv_i
+ * where colour_i == c, zero elsewhere (bn_gs_color_zero_colour's x, to the
+ * last bit) */
+void bn_pcg_update_xr_s0(int n, double *x, const double *p, double *r, const double *q, const double *s,
+                         double *partial, float *rs, const unsigned char *colour, int c, const float *dinv,
+                         float *x0, hipStream_t stream);
 void bn_dot_rz_s(int n, const double *r, const float *zs, const double *q,
                  double *partial, double *partial2, hipStream_t stream);
 void bn_pcg_update_p_s(int n, double *p, const float *zs, const double *s,
@@ -352,6 +359,10 @@ void bn_axpy1(char vt, int n, const void *x, void *y, hipStream_t stream);
 void bn_scalar_finish(int which, const double *p1, const double *p2,
                       const double *p3, double *s, int slot,
                       hipStream_t stream);
+/* the same (slot 0), also copying the BN_S_COUNT scalars to hs, host
+ * memory the device writes coherently */
+void bn_scalar_finish_host(int which, const double *p1, const double *p2, const double *p3, double *s, double *hs,
+                           hipStream_t stream);
 
 /* gather/scatter for permuted level numbering: y[k] = x[perm[k]] (gather)
  * or y[perm[k]] = x[k] (scatter), FP64 */

@@ -1512,11 +1512,16 @@ __global__ __launch_bounds__(BLOCK) void k_dot_rz_s(int n, int chunk, const doub
 }
 
 // WRITE_S: also store r's FP32 rounding into rs (the V-cycle's input)
+// WRITE_S: also r rounded to FP32 into rs, and when x0 != NULL the first
+// colour of a multicolour sweep from zero with rs as right-hand side
+// (k_diag_scale_colour's x0_i = rs_i dinv_i on colour c, zero elsewhere)
 template <bool WRITE_S>
 __global__ __launch_bounds__(BLOCK) void k_update_xr(int n, int chunk, double *__restrict__ x,
                                                      const double *__restrict__ p, double *__restrict__ r,
                                                      const double *__restrict__ q, const double *__restrict__ s,
-                                                     double *__restrict__ part, float *__restrict__ rs) {
+                                                     double *__restrict__ part, float *__restrict__ rs,
+                                                     const unsigned char *__restrict__ colour, int c,
+                                                     const float *__restrict__ dinv, float *__restrict__ x0) {
   const int lb = logical_block();
   const double alpha = s[BN_S_ALPHA];
   const long long i0 = (long long)lb * chunk * BLOCK;
@@ -1527,7 +1532,14 @@ __global__ __launch_bounds__(BLOCK) void k_update_xr(int n, int chunk, double *_
     x[i] = x[i] + p[i] * alpha;
     const double rn = r[i] + q[i] * (-alpha);
     r[i] = rn;
-    if (WRITE_S) rs[i] = (float)rn;
+    if (WRITE_S) {
+      const float rf = (float)rn;
+      rs[i] = rf;
+      if (x0) {
+        const float v = rf * dinv[i];
+        x0[i] = colour[i] == c ? v : 0.0f;
+      }
+    }
     a += rn * rn;
   }
   a = block_sum(a);
@@ -1551,8 +1563,10 @@ __global__ void k_update_p_s(int n, double *__restrict__ p, const float *__restr
   if (i < n) p[i] = beta * p[i] + (double)zs[i];
 }
 
+// hs != NULL: the scalars are also copied to hs (host memory), so the host
+// reads them without a copy of its own after the step
 __global__ __launch_bounds__(BLOCK) void k_scalar_finish(int which, const double *p1, const double *p2,
-                                                         const double *p3, double *s, int slot) {
+                                                         const double *p3, double *s, int slot, double *hs) {
   // fixed-order finish of BN_RED_BLOCKS partials per array by one block:
   // each thread sums its strided slots in ascending order, then the block
   // sum; the three arrays' loads are issued in one pass
@@ -1602,6 +1616,8 @@ __global__ __launch_bounds__(BLOCK) void k_scalar_finish(int which, const double
       s[slot] = a;
       break;
   }
+  if (hs)
+    for (int k = 0; k < BN_S_COUNT; k++) hs[k] = s[k];
 }
 
 __global__ void k_permute(int n, const int *__restrict__ perm, const double *__restrict__ x,
@@ -2149,7 +2165,7 @@ void bn_pcg_update_xr(int n, double *x, const double *p, double *r, const double
   const int g = grid_for(nb);
   const int chunk = (int)((nb + g - 1) / g);
   BN_LAUNCH(k_update_xr<false>, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, p, r, q, s, partial,
-                     (float *)NULL);
+                     (float *)NULL, (const unsigned char *)nullptr, 0, (const float *)nullptr, (float *)nullptr);
 }
 
 void bn_pcg_update_xr_s(int n, double *x, const double *p, double *r, const double *q, const double *s,
@@ -2161,7 +2177,22 @@ void bn_pcg_update_xr_s(int n, double *x, const double *p, double *r, const doub
   const long long nb = ((long long)n + BLOCK - 1) / BLOCK;
   const int g = grid_for(nb);
   const int chunk = (int)((nb + g - 1) / g);
-  BN_LAUNCH(k_update_xr<true>, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, p, r, q, s, partial, rs);
+  BN_LAUNCH(k_update_xr<true>, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, p, r, q, s, partial, rs,
+            (const unsigned char *)nullptr, 0, (const float *)nullptr, (float *)nullptr);
+}
+
+void bn_pcg_update_xr_s0(int n, double *x, const double *p, double *r, const double *q, const double *s,
+                         double *partial, float *rs, const unsigned char *colour, int c, const float *dinv, float *x0,
+                         hipStream_t stream) {
+  if (n <= 0) {
+    bn_fill('d', BN_RED_BLOCKS, partial, 0.0, stream);
+    return;
+  }
+  const long long nb = ((long long)n + BLOCK - 1) / BLOCK;
+  const int g = grid_for(nb);
+  const int chunk = (int)((nb + g - 1) / g);
+  BN_LAUNCH(k_update_xr<true>, dim3(g), dim3(BLOCK), 0, stream, n, chunk, x, p, r, q, s, partial, rs, colour, c, dinv,
+            x0);
 }
 
 void bn_dot_rz_s(int n, const double *r, const float *zs, const double *q, double *partial, double *partial2,
@@ -2189,7 +2220,12 @@ void bn_pcg_update_p(int n, double *p, const double *z, const double *s, hipStre
 
 void bn_scalar_finish(int which, const double *p1, const double *p2, const double *p3, double *s, int slot,
                       hipStream_t stream) {
-  BN_LAUNCH(k_scalar_finish, dim3(1), dim3(BLOCK), 0, stream, which, p1, p2, p3, s, slot);
+  BN_LAUNCH(k_scalar_finish, dim3(1), dim3(BLOCK), 0, stream, which, p1, p2, p3, s, slot, (double *)nullptr);
+}
+
+void bn_scalar_finish_host(int which, const double *p1, const double *p2, const double *p3, double *s, double *hs,
+                           hipStream_t stream) {
+  BN_LAUNCH(k_scalar_finish, dim3(1), dim3(BLOCK), 0, stream, which, p1, p2, p3, s, 0, hs);
 }
 
 void bn_permute(int n, const int *perm, const double *x, double *y, int scatter, hipStream_t stream) {
