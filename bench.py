@@ -26,10 +26,12 @@ METRIC = "PCG time to 1e-8 rel. residual, 4M-node net: FP32-AMG vs FP64 GPU; HBM
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per launch of the roofline kernel (rocprofv3 FETCH_SIZE + WRITE_SIZE
 # over its launches in one solve, each counter scaled by its reading for a
-# copy kernel of known size), by (workload, kernel symbol).  lattice160_trees17:
-# 153 launches per solve in both passes, FETCH 164.89 MB (counter x 1.999) +
-# WRITE 5.34 MB (x 1.000) per launch, against 155.27 MB algorithmic (1.10x).
-TRAFFIC = {("lattice160_trees17", "k_csr_blocks<float, 1, false>"): 170235015}
+# copy kernel of known size), by (workload, kernel symbol).  lattice160_trees17,
+# per launch: the level-1 Gauss-Seidel colours (527 launches per solve) FETCH
+# 32.45 MB (counter x 1.999) + WRITE 3.00 MB (x 1.000), against 20.06 MB
+# algorithmic (1.77x); the residuals (102 launches) 205.0 + 7.6 MB.
+TRAFFIC = {("lattice160_trees17", "k_rowsolve<float, 8, 2>"): 35445339,
+           ("lattice160_trees17", "k_csr_blocks<float, 1, false>"): 212617119}
 FAST_COLOURS = 16    # GS on levels with <= 16 colours, l1-Jacobi above
 FAST_IC_SWEEPS = 4   # Jacobi sweeps per IC(0) triangle
 
