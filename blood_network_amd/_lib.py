@@ -229,6 +229,8 @@ def _load():
         "bn_dense_gemv": (None, [C.c_char, C.c_int, VP, VP, VP, VP]),
         "bn_dot_partial": (None, [C.c_int, VP, VP, VP, VP, VP, VP, VP]),
         "bn_scalar_finish": (None, [C.c_int, VP, VP, VP, VP, C.c_int, VP]),
+        "bn_scalar_finish_pap": (None, [VP, C.c_int, VP, VP]),
+        "bn_csr_apply_blocks_dot": (None, [C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
         "bn_edge_grad": (None, [C.c_int, VP, VP, VP, VP, VP, VP, VP, VP, VP]),
     }
     for name, (res, args) in sig.items():

@@ -37,6 +37,8 @@ typedef struct {
   int nblk;        /* row blocks for the adaptive kernel               */
   int *blk;        /* device: nblk + 1 row boundaries                  */
   int *blkq;       /* device: their entry offsets, blkq[k] = ia[blk[k]] */
+  double *bpart;   /* device: nblk partials of a dot fused into the     */
+                   /* row-block product                                */
   int adapt;       /* FP64 apply uses the row-block kernel (autotuned) */
   int adapt_s;     /* FP32 likewise                                    */
   int deep;        /* FP64 SpMV load-round variant (BN_LANES_DEEP* bit) */
@@ -208,6 +210,8 @@ struct bn_solver {
   int jac_fuse;         /* Jacobi-smoothed levels fuse their steps into the
                            launches around them (vcycle.c; BN_FUSE_JACOBI=0
                            keeps separate launches) */
+  int dot_fuse;         /* the PCG step's p.Ap rides on the row-block
+                           product q = A p (BN_FUSE_DOT=0: k_dot2 after it) */
   int coarse_dense;     /* coarsest solve: 1 dense inverse, 0 IC-PCG */
   double *cinv_d;       /* dense inverse of the coarsest operator */
   float *cinv_s;
@@ -333,8 +337,12 @@ void bn_launch_gs(struct bn_solver *s, const bn_dlevel *d, char vt, int c, int l
 void bn_launch_jacobi(struct bn_solver *s, const bn_dcsr *T, char vt, const void *dinv, const void *b, const void *xin,
                       void *xout, const void *zbase, void *zsum, void *xs);
 /* FP64 y = op(A, x [, b]) of the outer PCG, with the dotw.y partials */
-void bn_launch_spmv(struct bn_solver *s, const bn_dcsr *A, int op, const void *x, void *y, const void *b,
-                    const void *dotw, double *partial);
+/* FP64 y = op(A) x; with partial, also the dotw.y partials: returns 0 when
+ * they are in partial (BN_RED_BLOCKS slots, bn_scalar_finish), or the
+ * number of slots of A->bpart that hold them (the row-block product with
+ * the dot fused: bn_scalar_finish_pap) */
+int bn_launch_spmv(struct bn_solver *s, const bn_dcsr *A, int op, const void *x, void *y, const void *b,
+                   const void *dotw, double *partial);
 int bn_ensure_device(struct bn_solver *s);
 
 /* segments.c */

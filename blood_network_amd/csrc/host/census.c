@@ -37,6 +37,12 @@ typedef struct {
   const unsigned char *colour; /* K_APPLY: the fused first colour's colour array and colour */
 } ckey;
 
+/* a K_SPMV launch that bn_launch_spmv runs with its dot fused */
+static int spmv_dot_fused(const struct bn_solver *s, const ckey *k) {
+  const bn_dcsr *M = (const bn_dcsr *)k->m;
+  return M->adapt && M->blk && M->bpart && k->v[3] && k->v[4] && k->op == BN_OP_SET && s->dot_fuse;
+}
+
 typedef struct {
   ckey key;
   char name[80];
@@ -76,7 +82,9 @@ static void shape_describe(struct bn_solver *s, cshape *sh) {
       const bn_dcsr *M = (const bn_dcsr *)k->m;
       const int blocks = (k->kind == K_RESID_SCALED) ||
                          (k->kind == K_SPMV ? (M->adapt && M->blk) : (k->vt == 's' ? M->adapt_s : M->adapt));
-      if (blocks)
+      if (blocks && k->kind == K_SPMV && spmv_dot_fused(s, k))
+        snprintf(sh->name, sizeof(sh->name), "k_csr_blocks<double, %d, false, true>", k->op);
+      else if (blocks)
         snprintf(sh->name, sizeof(sh->name), "k_csr_blocks<%s, %d, %s>", tname(k->vt), k->op,
                  k->kind == K_RESID_SCALED ? "true" : "false");
       else {
@@ -92,6 +100,7 @@ static void shape_describe(struct bn_solver *s, cshape *sh) {
       double b = (double)M->nnz * (4.0 + v) + ((double)M->rows + 1.0) * 4.0 + (double)M->cols * v +
                  (double)M->rows * v;
       if (k->op == BN_OP_RESID || k->op == BN_OP_ADD || k->op == BN_OP_JAC) b += (double)M->rows * v;
+      if (k->kind == K_SPMV && spmv_dot_fused(s, k)) b += (double)M->rows * v; /* the dot's w */
       if (k->kind == K_RESID_SCALED) b += 2.0 * M->rows * v;
       /* a fused Jacobi step: its weights read, and for SET the step written */
       if (k->kind == K_APPLY && k->v[4]) b += (double)M->rows * v * (k->op == BN_OP_SET ? 2.0 : 1.0);
@@ -245,8 +254,8 @@ void bn_launch_jacobi(struct bn_solver *s, const bn_dcsr *T, char vt, const void
   census_end(s, ev);
 }
 
-void bn_launch_spmv(struct bn_solver *s, const bn_dcsr *A, int op, const void *x, void *y, const void *b,
-                    const void *dotw, double *partial) {
+int bn_launch_spmv(struct bn_solver *s, const bn_dcsr *A, int op, const void *x, void *y, const void *b,
+                   const void *dotw, double *partial) {
   int ev = -1;
   if (s->census) {
     ckey k = mk(K_SPMV, 'd', op, A);
@@ -254,16 +263,24 @@ void bn_launch_spmv(struct bn_solver *s, const bn_dcsr *A, int op, const void *x
     ev = census_begin(s, &k);
   }
   if (A->adapt && A->blk) {
+    if (partial && dotw && op == BN_OP_SET && s->dot_fuse && A->bpart) {
+      /* the product with the dot's row-block partials fused */
+      bn_csr_apply_blocks_dot(A->nblk, A->blk, A->blkq, A->ia, A->ja, A->vd, (const double *)x, (double *)y,
+                              (const double *)dotw, A->bpart, s->stream);
+      census_end(s, ev);
+      return A->nblk;
+    }
     /* the row-block product, then (outside the record) the deterministic
      * dot partials it feeds */
     bn_csr_apply_blocks('d', op, A->nblk, A->blk, A->blkq, A->ia, A->ja, A->vd, x, y, b, s->stream);
     census_end(s, ev);
     if (partial)
       bn_dot_partial(A->rows, (const double *)dotw, (const double *)y, NULL, NULL, partial, NULL, s->stream);
-    return;
+    return 0;
   }
   bn_dcsr_apply_dot(A, 'd', op, x, y, b, dotw, partial, s->stream);
   census_end(s, ev);
+  return 0;
 }
 
 /* ---------------------------------------------------------------------
@@ -289,8 +306,12 @@ static void relaunch(struct bn_solver *s, const ckey *k) {
       break;
     case K_SPMV:
       /* the product alone (the dot partials that follow it are not part
-       * of the kernel timed) */
-      bn_launch_spmv(s, (const bn_dcsr *)k->m, k->op, k->v[0], (void *)k->v[1], k->v[2], NULL, NULL);
+       * of the kernel timed), or with the dot when it is fused */
+      if (spmv_dot_fused(s, k))
+        bn_launch_spmv(s, (const bn_dcsr *)k->m, k->op, k->v[0], (void *)k->v[1], k->v[2], k->v[4],
+                       (double *)k->v[3]);
+      else
+        bn_launch_spmv(s, (const bn_dcsr *)k->m, k->op, k->v[0], (void *)k->v[1], k->v[2], NULL, NULL);
       break;
   }
 }

@@ -158,7 +158,8 @@ static int upload_csr(struct bn_solver *s, const dcsr_mat *m, bn_dcsr *d) {
     d->nblk = bn_row_blocks(m->ia, m->row, BN_ROWBLOCK_NNZ, BN_ROWBLOCK_ROWS, bk);
     d->blk = (int *)dmalloc(s, ((size_t)d->nblk + 1) * sizeof(int));
     d->blkq = (int *)dmalloc(s, ((size_t)d->nblk + 1) * sizeof(int));
-    if (!d->blk || !d->blkq) {
+    d->bpart = (double *)dmalloc(s, ((size_t)d->nblk + 1) * sizeof(double));
+    if (!d->blk || !d->blkq || !d->bpart) {
       free(bk);
       return BN_ERR_HIP;
     }
@@ -537,7 +538,7 @@ static void autotune_csr(struct bn_solver *s, const bn_tune_env *env, bn_dcsr *d
 
 /* BN_APPLY in the environment pins the level-operator kernel choice:
  * "lanes" (lanes-per-row kernel only), "blocks" (row-block kernel for
- * every level operator), anything else: timed choice. */
+ * every level operator and the outer PCG's), anything else: timed choice. */
 static int apply_mode(void) {
   const char *e = getenv("BN_APPLY");
   if (!e) return 0;
@@ -553,6 +554,9 @@ static void pin_blocks(struct bn_solver *s) {
     for (int k = 0; k < 3; k++)
       if (ops[k]->blk) ops[k]->adapt = ops[k]->adapt_s = 1;
   }
+  /* and the outer PCG's operator */
+  if (s->asolve_shared) s->Asolve = s->lev[0].A;
+  else if (s->Asolve.blk) s->Asolve.adapt = 1;
 }
 
 /* the tuned operators in a fixed order (NULL entries skipped) */
@@ -1307,6 +1311,8 @@ int bn_device_upload(struct bn_solver *s) {
   {
     const char *fenv = getenv("BN_FUSE_JACOBI");
     s->jac_fuse = !(fenv && fenv[0] == '0');
+    const char *denv = getenv("BN_FUSE_DOT");
+    s->dot_fuse = !(denv && denv[0] == '0');
   }
   s->vt = (s->opt.precision == BN_PREC_FP32) ? 's' : 'd';
   const amg_data *amg = s->amg;

@@ -45,8 +45,9 @@ static void enqueue_step(struct bn_solver *s) {
   if (f32) bn_pcg_update_p_s(n, s->p, (const float *)s->zout, sc, st);
   else bn_pcg_update_p(n, s->p, s->z, sc, st);
   /* q = A p, p'q -> alpha */
-  bn_launch_spmv(s, A, BN_OP_SET, s->p, s->q, NULL, s->p, s->part1);
-  bn_scalar_finish(1, s->part1, NULL, NULL, sc, 0, st);
+  const int nslot = bn_launch_spmv(s, A, BN_OP_SET, s->p, s->q, NULL, s->p, s->part1);
+  if (nslot > 0) bn_scalar_finish_pap(A->bpart, nslot, sc, st);
+  else bn_scalar_finish(1, s->part1, NULL, NULL, sc, 0, st);
   if (f32) {
     const bn_dlevel *d0 = &s->lev[0];
     if (bn_vc_first_colour_fused(s)) {

@@ -145,6 +145,12 @@ void bn_csr_apply_blocks(char vt, int op, int nblk, const int *blocks, const int
                          const int *ia, const int *ja, const void *val,
                          const void *x, void *y, const void *b,
                          hipStream_t stream);
+/* FP64 y = A x with the w.y dot fused: row block k also writes
+ * partial[k] = sum of w_r y_r over its rows, so partial holds nblk
+ * slots; bn_scalar_finish_pap sums them. */
+void bn_csr_apply_blocks_dot(int nblk, const int *blocks, const int *blkq, const int *ia, const int *ja,
+                             const double *val, const double *x, double *y, const double *w, double *partial,
+                             hipStream_t stream);
 /* The same with a Jacobi step fused in, as bn_csr_apply_seg_jac. */
 void bn_csr_apply_blocks_jac(char vt, int op, int nblk, const int *blocks, const int *blkq,
                              const int *ia, const int *ja, const void *val,
@@ -363,6 +369,9 @@ void bn_scalar_finish(int which, const double *p1, const double *p2,
  * memory the device writes coherently */
 void bn_scalar_finish_host(int which, const double *p1, const double *p2, const double *p3, double *s, double *hs,
                            hipStream_t stream);
+/* which = 1 (PAP and ALPHA) over nslot partials of p.Ap, as written by
+ * bn_csr_apply_blocks_dot (slots summed in a fixed order) */
+void bn_scalar_finish_pap(const double *part, int nslot, double *s, hipStream_t stream);
 
 /* gather/scatter for permuted level numbering: y[k] = x[perm[k]] (gather)
  * or y[perm[k]] = x[k] (scatter), FP64 */

@@ -496,9 +496,11 @@ void launch_long(int op, int nlong, const int *rows, const int *ia, const int *j
 // The block's work over an access policy `io`: io.gx(j) gathers x_j,
 // io.base(r) is the value the row's sum is combined with (b_r for RESID,
 // y_r for ADD), io.dsc(r) the SCALED factor and io.out(r, s, base, dsc)
-// forms and stores the result.  prod: BN_ROWBLOCK_NNZ + 1 words of LDS.
+// forms and stores the result and returns its term of a fused dot (0 when
+// the policy has none).  prod: BN_ROWBLOCK_NNZ + 1 words of LDS.  Returns
+// the sum of this thread's dot terms.
 template <typename T, int OP, typename IO>
-__device__ __forceinline__ void csr_block(int bid, const int *__restrict__ blocks, const int *__restrict__ blkq,
+__device__ __forceinline__ double csr_block(int bid, const int *__restrict__ blocks, const int *__restrict__ blkq,
                                           const int *__restrict__ ia, const int *__restrict__ ja,
                                           const T *__restrict__ val, const IO &io, T *prod) {
   constexpr int PER = BN_ROWBLOCK_NNZ / BLOCK;
@@ -511,8 +513,7 @@ __device__ __forceinline__ void csr_block(int bid, const int *__restrict__ block
     const T base = io.base(r0);
     const T s = row_dot_deep_g<T>(q0 + threadIdx.x, q1, ja, val, [&io](int c) { return io.gx(c); });
     const double t = block_sum((double)s);
-    if (threadIdx.x == 0) io.out(r0, (T)t, base, io.dsc(r0));
-    return;
+    return threadIdx.x == 0 ? io.out(r0, (T)t, base, io.dsc(r0)) : 0.0;
   }
   // tpr = largest power of two <= 256 / rows, at most one wave; with at
   // most 256 rows a block (bn_row_blocks' limit) one pass reduces them all
@@ -550,6 +551,7 @@ __device__ __forceinline__ void csr_block(int bid, const int *__restrict__ block
     }
   }
   __syncthreads();
+  double acc = 0.0;
   for (int rb = r0; rb < r1; rb += BLOCK >> lg) {
     const int r = rb + (threadIdx.x >> lg);
     const bool first = (rb == r0);
@@ -562,15 +564,17 @@ __device__ __forceinline__ void csr_block(int bid, const int *__restrict__ block
     if (r < r1)
       for (int k = ks + sub; k < ke; k += tpr) s += prod[k];
     for (int o = tpr >> 1; o > 0; o >>= 1) s += __shfl_xor(s, o);
-    if (sub == 0 && r < r1) io.out(r, s, first ? base0 : io.base(r), first ? dsc0 : io.dsc(r));
+    if (sub == 0 && r < r1) acc += io.out(r, s, first ? base0 : io.base(r), first ? dsc0 : io.dsc(r));
   }
+  return acc;
 }
 
 // plain loads and stores: y = A x | y = b - A x | y += A x, and with
 // SCALED (RESID only) also ys_r = y_r dsc_r, the first Jacobi step from
 // zero of a triangle whose right-hand side is this residual (the product
-// k_diag_scale would form)
-template <typename T, int OP, bool SCALED>
+// k_diag_scale would form); with DOT, out returns w_r y_r (in double) for
+// the fused w.y dot
+template <typename T, int OP, bool SCALED, bool DOT = false>
 struct blocks_io {
   const T *__restrict__ x;
   T *__restrict__ y;
@@ -578,23 +582,29 @@ struct blocks_io {
   const T *__restrict__ dsc_;
   T *__restrict__ ys;
   jacobi_epi<T> e;
+  const T *__restrict__ w;
   __device__ __forceinline__ T gx(int j) const { return x[j]; }
   __device__ __forceinline__ T base(int r) const {
     return (OP == BN_OP_RESID || OP == BN_OP_JAC) ? b[r] : OP == BN_OP_ADD ? y[r] : T(0);
   }
   __device__ __forceinline__ T dsc(int r) const { return SCALED ? dsc_[r] : T(0); }
-  __device__ __forceinline__ void out(int r, T s, T base, T d) const {
+  __device__ __forceinline__ double out(int r, T s, T base, T d) const {
+    T o;
     if (OP == BN_OP_SET) {
+      o = s;
       csr_finish_store<T, OP>(y, r, s, e);
     } else if (OP == BN_OP_RESID) {
-      const T o = base - s;
+      o = base - s;
       y[r] = o;
       if (SCALED) ys[r] = o * d;
     } else if (OP == BN_OP_JAC) {
-      y[r] = csr_finish<T, OP>(s, base, x, r, e);
+      o = csr_finish<T, OP>(s, base, x, r, e);
+      y[r] = o;
     } else {
-      y[r] = base + s;
+      o = base + s;
+      y[r] = o;
     }
+    return DOT ? (double)w[r] * (double)o : 0.0;
   }
 };
 
@@ -605,7 +615,9 @@ struct blocks_io {
 // reduces and its b (or y) and dsc; (3) the x gathers.  Without blkq the
 // entry bounds come from ia, one round trip more.  The reduction reads
 // LDS only.
-template <typename T, int OP, bool SCALED = false>
+// With DOT, block k also writes partial[k] = sum of w_r y_r over its rows
+// (a fixed order: each thread's rows in turn, then the block sum).
+template <typename T, int OP, bool SCALED = false, bool DOT = false>
 __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__restrict__ blocks,
                                                       const int *__restrict__ blkq,
                                                       const int *__restrict__ ia, const int *__restrict__ ja,
@@ -613,12 +625,18 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__res
                                                       T *__restrict__ y, const T *__restrict__ b,
                                                       const T *__restrict__ dsc = nullptr,
                                                       T *__restrict__ ys = nullptr,
-                                                      jacobi_epi<T> e = jacobi_epi<T>{nullptr, nullptr, T(0), nullptr, 0}) {
+                                                      jacobi_epi<T> e = jacobi_epi<T>{nullptr, nullptr, T(0), nullptr, 0},
+                                                      const T *__restrict__ w = nullptr,
+                                                      double *__restrict__ partial = nullptr) {
   __shared__ T prod[BN_ROWBLOCK_NNZ + 1];  // + a spare word padding slots write
   const int bid = logical_block();
   if (bid >= nblk) return;
-  const blocks_io<T, OP, SCALED> io{x, y, b, dsc, ys, e};
-  csr_block<T, OP>(bid, blocks, blkq, ia, ja, val, io, prod);
+  const blocks_io<T, OP, SCALED, DOT> io{x, y, b, dsc, ys, e, w};
+  const double acc = csr_block<T, OP>(bid, blocks, blkq, ia, ja, val, io, prod);
+  if (DOT) {
+    const double t = block_sum(acc);
+    if (threadIdx.x == 0) partial[bid] = t;
+  }
 }
 
 template <typename T>
@@ -626,20 +644,21 @@ void launch_blocks(int op, int nblk, const int *blocks, const int *blkq, const i
                    const T *x, T *y, const T *b, jacobi_epi<T> e, hipStream_t st) {
   if (nblk <= 0) return;
   const int g = grid_rows(nblk);
-  const T *no_dsc = nullptr;
+  const T *no_dsc = nullptr, *no_w = nullptr;
   T *no_ys = nullptr;
+  double *no_part = nullptr;
   if (op == BN_OP_SET)
     BN_LAUNCH((k_csr_blocks<T, BN_OP_SET>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
-              no_dsc, no_ys, e);
+              no_dsc, no_ys, e, no_w, no_part);
   else if (op == BN_OP_RESID)
     BN_LAUNCH((k_csr_blocks<T, BN_OP_RESID>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
-              no_dsc, no_ys, e);
+              no_dsc, no_ys, e, no_w, no_part);
   else if (op == BN_OP_JAC)
     BN_LAUNCH((k_csr_blocks<T, BN_OP_JAC>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
-              no_dsc, no_ys, e);
+              no_dsc, no_ys, e, no_w, no_part);
   else
     BN_LAUNCH((k_csr_blocks<T, BN_OP_ADD>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
-              no_dsc, no_ys, e);
+              no_dsc, no_ys, e, no_w, no_part);
 }
 
 // ---------------------------------------------------------------------
@@ -1566,16 +1585,31 @@ __global__ void k_update_p_s(int n, double *__restrict__ p, const float *__restr
 // hs != NULL: the scalars are also copied to hs (host memory), so the host
 // reads them without a copy of its own after the step
 __global__ __launch_bounds__(BLOCK) void k_scalar_finish(int which, const double *p1, const double *p2,
-                                                         const double *p3, double *s, int slot, double *hs) {
+                                                         const double *p3, double *s, int slot, double *hs,
+                                                         int n1) {
   // fixed-order finish of BN_RED_BLOCKS partials per array by one block:
   // each thread sums its strided slots in ascending order, then the block
   // sum; the three arrays' loads are issued in one pass
   // (an absent array reads p1's slots instead, so no load sits behind a
-  // branch; its sum is not used)
+  // branch; its sum is not used).  n1 > 0: p1 alone, with n1 slots (one
+  // per row block of a fused product), 64 loads in flight per thread (a
+  // 4.4M-row product's 17k slots in two rounds) and each thread's slots
+  // summed in ascending order
   const double *q1 = p1 ? p1 : (p2 ? p2 : p3);
   const double *q2 = p2 ? p2 : q1, *q3 = p3 ? p3 : q1;
   double va = 0.0, vb = 0.0, vc = 0.0;
-  if (q1) {
+  if (n1 > 0) {
+    for (int k0 = threadIdx.x; k0 < n1; k0 += 64 * BLOCK) {
+      double t[64];
+#pragma unroll
+      for (int u = 0; u < 64; u++) {
+        const int k = k0 + u * BLOCK;
+        t[u] = k < n1 ? p1[k] : 0.0;
+      }
+#pragma unroll
+      for (int u = 0; u < 64; u++) va += t[u];
+    }
+  } else if (q1) {
 #pragma unroll
     for (int k = threadIdx.x; k < BN_RED_BLOCKS; k += BLOCK) {
       va += q1[k];
@@ -1717,11 +1751,13 @@ void bn_csr_resid_blocks_scaled(char vt, int nblk, const int *blocks, const int 
   if (vt == 's')
     BN_LAUNCH((k_csr_blocks<float, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, stream, nblk, blocks, blkq,
                        ia, ja, (const float *)val, (const float *)x, (float *)y, (const float *)b, (const float *)dsc,
-                       (float *)ys, jacobi_epi<float>{nullptr, nullptr, 0.0f, nullptr, 0});
+                       (float *)ys, jacobi_epi<float>{nullptr, nullptr, 0.0f, nullptr, 0}, (const float *)nullptr,
+                       (double *)nullptr);
   else
     BN_LAUNCH((k_csr_blocks<double, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, stream, nblk, blocks, blkq,
                        ia, ja, (const double *)val, (const double *)x, (double *)y, (const double *)b,
-                       (const double *)dsc, (double *)ys, jacobi_epi<double>{nullptr, nullptr, 0.0, nullptr, 0});
+                       (const double *)dsc, (double *)ys, jacobi_epi<double>{nullptr, nullptr, 0.0, nullptr, 0},
+                       (const double *)nullptr, (double *)nullptr);
 }
 
 void bn_csr_apply_blocks_jac(char vt, int op, int nblk, const int *blocks, const int *blkq, const int *ia,
@@ -1740,6 +1776,17 @@ void bn_csr_apply_blocks_jac(char vt, int op, int nblk, const int *blocks, const
 void bn_csr_apply_blocks(char vt, int op, int nblk, const int *blocks, const int *blkq, const int *ia, const int *ja,
                          const void *val, const void *x, void *y, const void *b, hipStream_t stream) {
   bn_csr_apply_blocks_jac(vt, op, nblk, blocks, blkq, ia, ja, val, x, y, b, NULL, NULL, 0.0, NULL, 0, stream);
+}
+
+void bn_csr_apply_blocks_dot(int nblk, const int *blocks, const int *blkq, const int *ia, const int *ja,
+                             const double *val, const double *x, double *y, const double *w, double *partial,
+                             hipStream_t stream) {
+  if (nblk <= 0) return;
+  const double *no_b = nullptr, *no_dsc = nullptr;
+  double *no_ys = nullptr;
+  BN_LAUNCH((k_csr_blocks<double, BN_OP_SET, false, true>), dim3(grid_rows(nblk)), dim3(BLOCK), 0, stream, nblk,
+            blocks, blkq, ia, ja, val, x, y, no_b, no_dsc, no_ys, jacobi_epi<double>{nullptr, nullptr, 0.0, nullptr, 0},
+            w, partial);
 }
 
 void bn_block_gs(char vt, int nchunk, int max_rows, const int *chunk, const int *ccol, const int *cbnd,
@@ -2220,12 +2267,17 @@ void bn_pcg_update_p(int n, double *p, const double *z, const double *s, hipStre
 
 void bn_scalar_finish(int which, const double *p1, const double *p2, const double *p3, double *s, int slot,
                       hipStream_t stream) {
-  BN_LAUNCH(k_scalar_finish, dim3(1), dim3(BLOCK), 0, stream, which, p1, p2, p3, s, slot, (double *)nullptr);
+  BN_LAUNCH(k_scalar_finish, dim3(1), dim3(BLOCK), 0, stream, which, p1, p2, p3, s, slot, (double *)nullptr, 0);
+}
+
+void bn_scalar_finish_pap(const double *part, int nslot, double *s, hipStream_t stream) {
+  BN_LAUNCH(k_scalar_finish, dim3(1), dim3(BLOCK), 0, stream, 1, part, (const double *)nullptr,
+            (const double *)nullptr, s, 0, (double *)nullptr, nslot);
 }
 
 void bn_scalar_finish_host(int which, const double *p1, const double *p2, const double *p3, double *s, double *hs,
                            hipStream_t stream) {
-  BN_LAUNCH(k_scalar_finish, dim3(1), dim3(BLOCK), 0, stream, which, p1, p2, p3, s, 0, hs);
+  BN_LAUNCH(k_scalar_finish, dim3(1), dim3(BLOCK), 0, stream, which, p1, p2, p3, s, 0, hs, 0);
 }
 
 void bn_permute(int n, const int *perm, const double *x, double *y, int scatter, hipStream_t stream) {

@@ -136,6 +136,48 @@ def test_row_block_kernel_against_oracle(oracle, vt):
         assert np.array_equal(dys.numpy().view(bits), (g2 * dsc).astype(dt).view(bits)), name
 
 
+def test_row_block_product_with_fused_dot():
+    """q = A p with p.q fused: q bit for bit the plain row-block product's,
+    one partial per row block (the block's w_r y_r, hub rows by the
+    one-long-row path), and the finish over those slots giving p'Ap and
+    alpha = r'z / p'Ap."""
+    s, net = coarse_level_matrix()
+    mats = [("A%d" % lev, s.level(lev)["A"]) for lev in range(s.num_levels())]
+    mats.append(("hubs", hub_matrix()))
+    rng = np.random.default_rng(12)
+    for name, (ia, ja, val) in mats:
+        ia = np.ascontiguousarray(ia, np.int32)
+        ja = np.ascontiguousarray(ja, np.int32)
+        n = len(ia) - 1
+        blk = np.zeros(n + 1, np.int32)
+        nb = lib.bn_row_blocks(ip(ia), n, 2048, 256, ip(blk))
+        case(f"row-block csr with fused dot {name} n={n} nnz={len(ja)} blocks={nb}")
+        x, w = rng.random(n) - 0.5, rng.random(n) - 0.5
+        dia, dja, dv, dblk = dev(ia), dev(ja), dev(np.asarray(val, np.float64)), dev(blk[: nb + 1])
+        dblkq = dev(ia[blk[: nb + 1]].astype(np.int32))
+        dx, dw = dev(x), dev(w)
+        dy0, dy1 = dev(np.zeros(n)), dev(np.zeros(n))
+        dpart = dev(np.full(nb, np.nan))
+        lib.bn_csr_apply_blocks(b"d", 0, nb, dblk.ptr, dblkq.ptr, dia.ptr, dja.ptr, dv.ptr, dx.ptr, dy0.ptr, None,
+                                None)
+        lib.bn_csr_apply_blocks_dot(nb, dblk.ptr, dblkq.ptr, dia.ptr, dja.ptr, dv.ptr, dx.ptr, dy1.ptr, dw.ptr,
+                                    dpart.ptr, None)
+        y0, y1, part = dy0.numpy(), dy1.numpy(), dpart.numpy()
+        assert np.array_equal(y0.view(np.uint64), y1.view(np.uint64)), name
+        terms = w * y1
+        want = np.array([terms[blk[k]:blk[k + 1]].sum() for k in range(nb)])
+        scale = np.array([np.abs(terms[blk[k]:blk[k + 1]]).sum() for k in range(nb)]) + 1e-300
+        assert np.all(np.abs(part - want) <= 1e-13 * scale), name
+        sc = np.zeros(16)
+        sc[0] = 3.25                                     # r'z
+        dsc = dev(sc)
+        lib.bn_scalar_finish_pap(dpart.ptr, nb, dsc.ptr, None)
+        got = dsc.numpy()
+        pap = terms.sum()
+        assert abs(got[2] - pap) <= 1e-13 * np.abs(terms).sum(), name
+        assert abs(got[3] - 3.25 / got[2]) <= 1e-15 * abs(got[3]), name
+
+
 # the sentinel of values not written yet (a signalling-NaN bit pattern)
 SENTINEL64 = np.uint64(0x7ff4dead0000beef)
 
