@@ -83,7 +83,7 @@ static void shape_describe(struct bn_solver *s, cshape *sh) {
       const int blocks = (k->kind == K_RESID_SCALED) ||
                          (k->kind == K_SPMV ? (M->adapt && M->blk) : (k->vt == 's' ? M->adapt_s : M->adapt));
       if (blocks && k->kind == K_SPMV && spmv_dot_fused(s, k))
-        snprintf(sh->name, sizeof(sh->name), "k_csr_blocks<double, %d, false, true>", k->op);
+        snprintf(sh->name, sizeof(sh->name), "k_csr_blocks_dot<double>");
       else if (blocks)
         snprintf(sh->name, sizeof(sh->name), "k_csr_blocks<%s, %d, %s>", tname(k->vt), k->op,
                  k->kind == K_RESID_SCALED ? "true" : "false");

@@ -617,17 +617,14 @@ struct blocks_io {
 // LDS only.
 // With DOT, block k also writes partial[k] = sum of w_r y_r over its rows
 // (a fixed order: each thread's rows in turn, then the block sum).
-template <typename T, int OP, bool SCALED = false, bool DOT = false>
-__global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__restrict__ blocks,
-                                                      const int *__restrict__ blkq,
-                                                      const int *__restrict__ ia, const int *__restrict__ ja,
-                                                      const T *__restrict__ val, const T *__restrict__ x,
-                                                      T *__restrict__ y, const T *__restrict__ b,
-                                                      const T *__restrict__ dsc = nullptr,
-                                                      T *__restrict__ ys = nullptr,
-                                                      jacobi_epi<T> e = jacobi_epi<T>{nullptr, nullptr, T(0), nullptr, 0},
-                                                      const T *__restrict__ w = nullptr,
-                                                      double *__restrict__ partial = nullptr) {
+template <typename T, int OP, bool SCALED, bool DOT>
+__device__ __forceinline__ void csr_blocks_body(int nblk, const int *__restrict__ blocks,
+                                                const int *__restrict__ blkq, const int *__restrict__ ia,
+                                                const int *__restrict__ ja, const T *__restrict__ val,
+                                                const T *__restrict__ x, T *__restrict__ y,
+                                                const T *__restrict__ b, const T *__restrict__ dsc,
+                                                T *__restrict__ ys, jacobi_epi<T> e, const T *__restrict__ w,
+                                                double *__restrict__ partial) {
   __shared__ T prod[BN_ROWBLOCK_NNZ + 1];  // + a spare word padding slots write
   const int bid = logical_block();
   if (bid >= nblk) return;
@@ -639,26 +636,48 @@ __global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__res
   }
 }
 
+template <typename T, int OP, bool SCALED = false>
+__global__ __launch_bounds__(BLOCK) void k_csr_blocks(int nblk, const int *__restrict__ blocks,
+                                                      const int *__restrict__ blkq,
+                                                      const int *__restrict__ ia, const int *__restrict__ ja,
+                                                      const T *__restrict__ val, const T *__restrict__ x,
+                                                      T *__restrict__ y, const T *__restrict__ b,
+                                                      const T *__restrict__ dsc = nullptr,
+                                                      T *__restrict__ ys = nullptr,
+                                                      jacobi_epi<T> e = jacobi_epi<T>{nullptr, nullptr, T(0), nullptr, 0}) {
+  csr_blocks_body<T, OP, SCALED, false>(nblk, blocks, blkq, ia, ja, val, x, y, b, dsc, ys, e, nullptr, nullptr);
+}
+
+// y = A x with the w.y dot fused: one partial per row block
+template <typename T>
+__global__ __launch_bounds__(BLOCK) void k_csr_blocks_dot(int nblk, const int *__restrict__ blocks,
+                                                          const int *__restrict__ blkq, const int *__restrict__ ia,
+                                                          const int *__restrict__ ja, const T *__restrict__ val,
+                                                          const T *__restrict__ x, T *__restrict__ y,
+                                                          const T *__restrict__ w, double *__restrict__ partial) {
+  csr_blocks_body<T, BN_OP_SET, false, true>(nblk, blocks, blkq, ia, ja, val, x, y, nullptr, nullptr, nullptr,
+                                             jacobi_epi<T>{nullptr, nullptr, T(0), nullptr, 0}, w, partial);
+}
+
 template <typename T>
 void launch_blocks(int op, int nblk, const int *blocks, const int *blkq, const int *ia, const int *ja, const T *val,
                    const T *x, T *y, const T *b, jacobi_epi<T> e, hipStream_t st) {
   if (nblk <= 0) return;
   const int g = grid_rows(nblk);
-  const T *no_dsc = nullptr, *no_w = nullptr;
+  const T *no_dsc = nullptr;
   T *no_ys = nullptr;
-  double *no_part = nullptr;
   if (op == BN_OP_SET)
     BN_LAUNCH((k_csr_blocks<T, BN_OP_SET>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
-              no_dsc, no_ys, e, no_w, no_part);
+              no_dsc, no_ys, e);
   else if (op == BN_OP_RESID)
     BN_LAUNCH((k_csr_blocks<T, BN_OP_RESID>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
-              no_dsc, no_ys, e, no_w, no_part);
+              no_dsc, no_ys, e);
   else if (op == BN_OP_JAC)
     BN_LAUNCH((k_csr_blocks<T, BN_OP_JAC>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
-              no_dsc, no_ys, e, no_w, no_part);
+              no_dsc, no_ys, e);
   else
     BN_LAUNCH((k_csr_blocks<T, BN_OP_ADD>), dim3(g), dim3(BLOCK), 0, st, nblk, blocks, blkq, ia, ja, val, x, y, b,
-              no_dsc, no_ys, e, no_w, no_part);
+              no_dsc, no_ys, e);
 }
 
 // ---------------------------------------------------------------------
@@ -1751,13 +1770,11 @@ void bn_csr_resid_blocks_scaled(char vt, int nblk, const int *blocks, const int 
   if (vt == 's')
     BN_LAUNCH((k_csr_blocks<float, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, stream, nblk, blocks, blkq,
                        ia, ja, (const float *)val, (const float *)x, (float *)y, (const float *)b, (const float *)dsc,
-                       (float *)ys, jacobi_epi<float>{nullptr, nullptr, 0.0f, nullptr, 0}, (const float *)nullptr,
-                       (double *)nullptr);
+                       (float *)ys, jacobi_epi<float>{nullptr, nullptr, 0.0f, nullptr, 0});
   else
     BN_LAUNCH((k_csr_blocks<double, BN_OP_RESID, true>), dim3(g), dim3(BLOCK), 0, stream, nblk, blocks, blkq,
                        ia, ja, (const double *)val, (const double *)x, (double *)y, (const double *)b,
-                       (const double *)dsc, (double *)ys, jacobi_epi<double>{nullptr, nullptr, 0.0, nullptr, 0},
-                       (const double *)nullptr, (double *)nullptr);
+                       (const double *)dsc, (double *)ys, jacobi_epi<double>{nullptr, nullptr, 0.0, nullptr, 0});
 }
 
 void bn_csr_apply_blocks_jac(char vt, int op, int nblk, const int *blocks, const int *blkq, const int *ia,
@@ -1782,11 +1799,8 @@ void bn_csr_apply_blocks_dot(int nblk, const int *blocks, const int *blkq, const
                              const double *val, const double *x, double *y, const double *w, double *partial,
                              hipStream_t stream) {
   if (nblk <= 0) return;
-  const double *no_b = nullptr, *no_dsc = nullptr;
-  double *no_ys = nullptr;
-  BN_LAUNCH((k_csr_blocks<double, BN_OP_SET, false, true>), dim3(grid_rows(nblk)), dim3(BLOCK), 0, stream, nblk,
-            blocks, blkq, ia, ja, val, x, y, no_b, no_dsc, no_ys, jacobi_epi<double>{nullptr, nullptr, 0.0, nullptr, 0},
-            w, partial);
+  BN_LAUNCH(k_csr_blocks_dot<double>, dim3(grid_rows(nblk)), dim3(BLOCK), 0, stream, nblk, blocks, blkq, ia, ja, val,
+            x, y, w, partial);
 }
 
 void bn_block_gs(char vt, int nchunk, int max_rows, const int *chunk, const int *ccol, const int *cbnd,
