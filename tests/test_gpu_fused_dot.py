@@ -2,7 +2,7 @@
 product followed by its own dot pass (BN_FUSE_DOT=0).  Only the order of
 the dot's sums differs (a partial per row block instead of per strided
 slot), so both solves take the same iterations
-(C_amg1/src/amg_pcg.c:40-60's alpha = r'z / p'Ap) and reach the same
+(C_amg1/src/amg_pcg.c:56-59: alpha = r'z / p'Ap) and reach the same
 solution to rounding; the fused solve is deterministic from one graph
 replay to the next."""
 import sys
